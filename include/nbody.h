@@ -1,0 +1,167 @@
+/*
+ * nbody.h — C ABI of the MI355X (gfx950) 2-D gravitational N-body step.
+ *
+ * Drop-in boundary for the reference's `Simulation` step interface
+ * (src/Simulation/include/Simulation/Simulation.hpp:13-67): a context owns the
+ * bodies on the GPU, a step advances them, and the state can be read back in
+ * the caller's original body order at any time. The C++ adapter
+ * `nbody2d::GpuSimulation` (n-body-2d_amd/host/) wraps these calls behind the
+ * reference's run/pause/get_stats/set_timestep API.
+ *
+ * Conventions
+ *   - Every function returns NB_OK (0) or an NB_ERR_* code; nothing throws or
+ *     exits across the ABI (the reference's CUDA_CHECK calls ::exit,
+ *     BarnesHutCuda.cu:12-19). nb_last_error() gives the message.
+ *   - Positions/velocities/accelerations are interleaved x,y FP64 arrays,
+ *     byte-compatible with the reference's std::vector<sf::Vector2<double>>
+ *     (Body.hpp:34-37) and `Vector2 {double x, y}` (BarnesHutCuda.hpp:5-7).
+ *   - Host pointers are borrowed for the duration of a call only.
+ *   - A context is used by one thread at a time.
+ */
+#ifndef NBODY_H
+#define NBODY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NB_ABI_VERSION 1
+
+typedef struct nb_ctx nb_ctx; /* opaque context */
+
+enum nb_status {
+    NB_OK = 0,
+    NB_ERR_INVALID_ARG = 1,   /* bad parameter (range checks of Config.cpp:283-312) */
+    NB_ERR_HIP = 2,           /* a HIP runtime call failed */
+    NB_ERR_OUT_OF_MEMORY = 3,
+    NB_ERR_OVERFLOW = 4,      /* traversal stack overflow reported by the kernel */
+    NB_ERR_NOT_SUPPORTED = 5,
+    NB_ERR_NO_DEVICE = 6,
+    NB_ERR_NUMERIC = 7        /* non-finite positions found while building the tree */
+};
+
+/* Force evaluation. BARNES_HUT follows BarnesHutCuda (square Morton cells,
+ * side MAC, Plummer kernel); ALL_PAIRS is the direct sum of AllPairsSim
+ * (AllPairs.cpp:38-49) with the Plummer kernel. */
+enum nb_algorithm { NB_ALGO_BARNES_HUT = 0, NB_ALGO_ALL_PAIRS = 1 };
+
+/* Update order. KD = kick then drift (BarnesHut.cpp:78-79, BarnesHutCuda.cpp:35-36);
+ * DK = drift then kick (AllPairs.cpp:25-26). */
+enum nb_integrator { NB_INTEG_KD = 0, NB_INTEG_DK = 1 };
+
+/* Far field of an accepted cell: the reference's monopole, or the monopole
+ * plus a learned correction (nb_load_mlp). */
+enum nb_far_field { NB_FAR_MONOPOLE = 0, NB_FAR_LEARNED = 1 };
+
+typedef struct nb_params {
+    double G;           /* gravitational constant; the reference hard-codes
+                           6.67430e-11 (Constants.hpp:17) */
+    double dt;          /* "timestep" (Config.cpp:89) */
+    double theta;       /* "theta" (Config.cpp:92); MAC: side^2 < theta^2 d^2 */
+    double softening;   /* absolute Plummer epsilon; the reference derives it from
+                           "softening_factor" (Simulation.cpp:82-126) */
+    int32_t algorithm;  /* enum nb_algorithm */
+    int32_t integrator; /* enum nb_integrator */
+    int32_t self_guard; /* 1: never accept a cell that contains the body itself
+                           (the reference can, for theta > 1/sqrt(2)) */
+    int32_t far_field;  /* enum nb_far_field */
+    int32_t use_graph;  /* 1: steps replay a captured hipGraph (default) */
+    int32_t reserved[7];
+} nb_params;
+
+typedef struct nb_info {
+    uint64_t n;               /* bodies */
+    uint64_t iteration;       /* completed steps (Simulation::Stats::iteration) */
+    double simulated_time;    /* sum of dt over completed steps */
+    uint64_t tree_nodes;      /* node slots of the last tree (1 + 4 * internal) */
+    uint64_t internal_nodes;  /* internal nodes of the last tree */
+    uint64_t node_capacity;   /* internal-node capacity currently allocated */
+    uint64_t device_bytes;    /* device memory held by the context */
+} nb_info;
+
+/* Timing phases reported by nb_get_timing (milliseconds, summed over the
+ * steps run since timing was enabled). */
+enum nb_phase {
+    NB_PHASE_BBOX_MORTON = 0, /* bbox + Morton keys */
+    NB_PHASE_SORT = 1,        /* LSD radix sort + gather */
+    NB_PHASE_BUILD = 2,       /* quadtree topology + mass/COM */
+    NB_PHASE_FORCE = 3,       /* traversal (or all-pairs) + kick */
+    NB_PHASE_DRIFT = 4,       /* drift */
+    NB_PHASE_COUNT = 5
+};
+
+/* Exported tree slot for parity checks (canonical topology + COM). */
+typedef struct nb_node {
+    int32_t level;
+    int32_t bstart, bend; /* body range in sorted order, -1/-1 = empty slot */
+    int32_t c0;           /* first child slot, -1 = leaf, -2 = empty */
+    double mass, cx, cy, wsq;
+    float ixx, ixy, iyy;  /* second moments about the COM */
+    int32_t child_mask;   /* bit q set = child slot c0+q holds bodies */
+} nb_node;
+
+/* Fills *p with the defaults: G=1, dt=1e-3, theta=0.5, softening=0.01,
+ * Barnes-Hut, KD, no self guard, monopole, graphs on. */
+int nb_params_default(nb_params* p);
+
+/* Replaces: the BarnesHutCuda constructor + init_device_resources
+ * (BarnesHutCuda.cpp:6-10, BarnesHutCuda.cu:550-606). Copies the host arrays
+ * to the GPU; the context owns every device buffer. n may be 0. */
+int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* pos_xy,
+              const double* vel_xy, nb_ctx** ctx_out);
+
+/* Replaces: k iterations of BarnesHutCuda::simulate's loop body
+ * (BarnesHutCuda.cpp:30-40) / AllPairsSim::simulate (AllPairs.cpp:23-29) /
+ * BarnesHut::simulate (BarnesHut.cpp:48-69). Returns after the GPU finished. */
+int nb_step(nb_ctx* ctx, uint64_t k);
+
+/* Replaces: sync_from_gpu_bodies (BarnesHutCuda.cu:632-646): the state in the
+ * caller's original body order. Either pointer may be NULL. */
+int nb_get_state(nb_ctx* ctx, double* pos_xy, double* vel_xy);
+
+/* Accelerations (G included) of the current state in original order, and the
+ * number of force terms per body (may be NULL) and in total (may be NULL).
+ * Replaces the force half of compute_force (BarnesHutCuda.cu:382-486) without
+ * the kick. */
+int nb_compute_forces(nb_ctx* ctx, double* acc_xy, uint32_t* interactions, uint64_t* total);
+
+/* Replaces: Simulation::set_timestep (Simulation.cpp:50-52); the new dt is
+ * used from the next step on (the latch of Simulation.cpp:70). */
+int nb_set_dt(nb_ctx* ctx, double dt);
+
+int nb_get_info(nb_ctx* ctx, nb_info* info);
+
+/* Per-phase GPU timing with hipEvents on the context's stream; enabling it
+ * launches the step eagerly (no graph) so every phase can be bracketed. */
+int nb_set_timing(nb_ctx* ctx, int enable);
+int nb_get_timing(nb_ctx* ctx, double* phase_ms, int nphase, uint64_t* steps_timed);
+
+/* Replaces: ~BarnesHutCuda + release_device_resources (BarnesHutCuda.cu:608-630). */
+void nb_destroy(nb_ctx* ctx);
+
+const char* nb_last_error(const nb_ctx* ctx);
+const char* nb_status_string(int status);
+
+/* ---- parity hooks (the stages of one step, exposed for tests) ---- */
+
+/* Runs bbox -> Morton -> sort -> gather -> tree on the current state without
+ * moving any body, then copies out the squared box {x_min, x_max, y_min,
+ * y_max}, the sorted keys, the original index of each sorted body, and the
+ * tree slot count. Any output pointer may be NULL. */
+int nb_debug_build(nb_ctx* ctx, double* box4, uint64_t* sorted_keys, uint32_t* orig_index,
+                   uint64_t* node_count);
+
+/* Copies the tree built by the last step or nb_debug_build. */
+int nb_debug_tree(nb_ctx* ctx, nb_node* nodes, uint64_t cap);
+
+/* Stateless FP64 direct sum on the GPU, the judge of force errors:
+ * acc[t] = G * sum_{j != targets[t]} m_j (r_j - r_t) / (|r|^2 + eps^2)^{3/2}. */
+int nb_direct_sum_f64(uint64_t n, const double* pos_xy, const double* mass, double G,
+                      double softening, uint64_t m, const int64_t* targets, double* acc_xy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

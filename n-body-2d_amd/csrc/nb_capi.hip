@@ -1,0 +1,708 @@
+// nb_capi.hip — the C ABI (include/nbody.h): context lifetime, step
+// sequencing with hipGraph replay, readback in original order, error codes.
+//
+// Step sequence (Barnes-Hut, kick-drift), everything on the context stream:
+//   bbox -> Morton -> 8 radix passes -> gather -> tree delta/scan/nodes ->
+//   30 COM levels -> traversal + kick -> drift
+// (BarnesHutCuda::simulate, BarnesHutCuda.cpp:30-40, without the per-level
+// host round trips or the per-step device-to-host copy of :632-646).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "nb_internal.h"
+
+using namespace nb;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(nb_ctx* c, int code, const std::string& msg) {
+    if (c)
+        c->err = msg;
+    else
+        g_err = msg;
+    return code;
+}
+
+#define NB_HIP(c, expr)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail((c), e_ == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP, \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(nb_ctx* c, T** p, uint64_t count) {
+    uint64_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = 256;
+    const hipError_t e = hipMalloc((void**)p, bytes);
+    if (e == hipSuccess) c->device_bytes += bytes;
+    return e;
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+bool is_bh(const nb_ctx* c) { return c->p.algorithm == NB_ALGO_BARNES_HUT; }
+bool is_dk(const nb_ctx* c) { return c->p.integrator == NB_INTEG_DK; }
+
+uint64_t tree_bytes(uint64_t icap) {
+    return (1 + 4 * icap) * sizeof(NodeRec) + icap * (sizeof(uint32_t) + 2);
+}
+
+void free_tree(nb_ctx* c) {
+    if (c->icap) c->device_bytes -= tree_bytes(c->icap);
+    dfree(c->b.nodes);
+    dfree(c->b.inode_slot);
+    dfree(c->b.inode_level);
+    dfree(c->b.inode_mask);
+    c->icap = 0;
+}
+
+int alloc_tree(nb_ctx* c, uint64_t icap) {
+    free_tree(c);
+    const uint64_t before = c->device_bytes;
+    NB_HIP(c, dalloc(c, &c->b.nodes, 1 + 4 * icap));
+    NB_HIP(c, dalloc(c, &c->b.inode_slot, icap));
+    NB_HIP(c, dalloc(c, &c->b.inode_level, icap));
+    NB_HIP(c, dalloc(c, &c->b.inode_mask, icap));
+    c->device_bytes = before + tree_bytes(icap);
+    c->icap = icap;
+    return NB_OK;
+}
+
+void destroy_graphs(nb_ctx* c) {
+    for (auto& g : c->graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
+void free_all(nb_ctx* c) {
+    destroy_graphs(c);
+    free_tree(c);
+    Buffers& b = c->b;
+    for (int k = 0; k < 2; k++) {
+        dfree(b.pos[k]);
+        dfree(b.vel[k]);
+        dfree(b.mass[k]);
+        dfree(b.orig[k]);
+        dfree(b.keys[k]);
+        dfree(b.vals[k]);
+    }
+    dfree(b.sort_hist);
+    dfree(b.sort_tot);
+    dfree(b.box);
+    dfree(b.bbox_part);
+    dfree(b.delta);
+    dfree(b.cnt);
+    dfree(b.scan_tmp);
+    dfree(b.multi_list);
+    dfree(b.acc);
+    dfree(b.inter);
+    dfree(b.ap_part);
+    dfree(b.meta);
+    dfree(b.consts);
+    for (auto& e : c->ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    if (c->host_meta) (void)hipHostFree(c->host_meta);
+    c->host_meta = nullptr;
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = nullptr;
+}
+
+int check_params(const nb_params* p, std::string* why) {
+    if (!p) return *why = "params is NULL", NB_ERR_INVALID_ARG;
+    if (!std::isfinite(p->G)) return *why = "G must be finite", NB_ERR_INVALID_ARG;
+    if (!std::isfinite(p->dt) || p->dt <= 0.0)
+        return *why = "dt must be finite and > 0", NB_ERR_INVALID_ARG;
+    if (!std::isfinite(p->theta) || p->theta < 0.0)
+        return *why = "theta must be finite and >= 0", NB_ERR_INVALID_ARG;
+    if (!std::isfinite(p->softening) || p->softening < 0.0)
+        return *why = "softening must be finite and >= 0", NB_ERR_INVALID_ARG;
+    if (p->algorithm != NB_ALGO_BARNES_HUT && p->algorithm != NB_ALGO_ALL_PAIRS)
+        return *why = "unknown algorithm", NB_ERR_INVALID_ARG;
+    if (p->integrator != NB_INTEG_KD && p->integrator != NB_INTEG_DK)
+        return *why = "unknown integrator", NB_ERR_INVALID_ARG;
+    if (p->far_field != NB_FAR_MONOPOLE && p->far_field != NB_FAR_LEARNED)
+        return *why = "unknown far_field", NB_ERR_INVALID_ARG;
+    if (p->far_field == NB_FAR_LEARNED)
+        return *why = "far_field=LEARNED needs the MLP correction, not built in this library",
+               NB_ERR_NOT_SUPPORTED;
+    return NB_OK;
+}
+
+int upload_consts(nb_ctx* c) {
+    StepConsts k;
+    k.dt = c->dt;
+    k.G = c->p.G;
+    k.G_dt = c->p.G * c->dt;
+    k.theta_sq = c->p.theta * c->p.theta;
+    k.eps2 = c->p.softening * c->p.softening;
+    NB_HIP(c, hipMemcpyAsync(c->b.consts, &k, sizeof(k), hipMemcpyHostToDevice, c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+// ---- phase marks for eager timing ----
+struct Marks {
+    nb_ctx* c;
+    void begin(int ph) {
+        if (c->timing) (void)hipEventRecord(c->ev[2 * ph], c->stream);
+    }
+    void end(int ph) {
+        if (c->timing) (void)hipEventRecord(c->ev[2 * ph + 1], c->stream);
+    }
+};
+
+// bbox -> Morton -> sort -> gather -> tree -> traversal (or all-pairs).
+// Returns the buffer that holds the state afterwards.
+int enqueue_force(nb_ctx* c, int src, bool kick, bool count_step) {
+    Marks mk{c};
+    int s = src;
+    if (is_bh(c)) {
+        mk.begin(NB_PHASE_BBOX_MORTON);
+        launch_bbox(c, s);
+        launch_morton(c, s);
+        mk.end(NB_PHASE_BBOX_MORTON);
+        mk.begin(NB_PHASE_SORT);
+        launch_radix_sort(c);
+        launch_gather(c, s);
+        mk.end(NB_PHASE_SORT);
+        s ^= 1;
+        mk.begin(NB_PHASE_BUILD);
+        launch_tree_build(c, s);
+        mk.end(NB_PHASE_BUILD);
+        mk.begin(NB_PHASE_FORCE);
+        launch_traverse(c, s, kick, count_step);
+        mk.end(NB_PHASE_FORCE);
+    } else {
+        mk.begin(NB_PHASE_FORCE);
+        launch_allpairs(c, s, kick, count_step);
+        mk.end(NB_PHASE_FORCE);
+    }
+    return s;
+}
+
+int enqueue_step(nb_ctx* c, int src) {
+    Marks mk{c};
+    int s = src;
+    if (is_dk(c)) {
+        mk.begin(NB_PHASE_DRIFT);
+        launch_drift(c, s, true, false);
+        mk.end(NB_PHASE_DRIFT);
+        s = enqueue_force(c, s, true, true);
+    } else {
+        s = enqueue_force(c, s, true, false);
+        mk.begin(NB_PHASE_DRIFT);
+        launch_drift(c, s, true, true);
+        mk.end(NB_PHASE_DRIFT);
+    }
+    return s;
+}
+
+int build_graphs(nb_ctx* c) {
+    destroy_graphs(c);
+    if (!c->p.use_graph || c->n == 0) return NB_OK;
+    const bool timing = c->timing;
+    c->timing = false;
+    const int variants = is_bh(c) ? 2 : 1;
+    for (int v = 0; v < variants; v++) {
+        hipGraph_t g = nullptr;
+        NB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_step(c, v);
+        const hipError_t le = hipGetLastError();
+        NB_HIP(c, hipStreamEndCapture(c->stream, &g));
+        NB_HIP(c, le);
+        const hipError_t ie = hipGraphInstantiate(&c->graph[v], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        NB_HIP(c, ie);
+    }
+    c->timing = timing;
+    return NB_OK;
+}
+
+int clear_meta(nb_ctx* c) {
+    NB_HIP(c, hipMemsetAsync(c->b.meta, 0, META_COUNT * sizeof(uint32_t), c->stream));
+    return NB_OK;
+}
+
+int read_meta(nb_ctx* c) {
+    NB_HIP(c, hipMemcpyAsync(c->host_meta, c->b.meta, META_COUNT * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+int grow_tree(nb_ctx* c, uint64_t needed) {
+    const uint64_t icap = needed + needed / 4 + 1024;
+    int rc = alloc_tree(c, icap);
+    if (rc) return rc;
+    return build_graphs(c);
+}
+
+// Builds the tree of the current state (moving it into Morton order) and,
+// if requested, evaluates the forces without a kick. Grows the node pool
+// until the tree fits.
+int build_and_force(nb_ctx* c, bool forces) {
+    for (int attempt = 0; attempt < 4; attempt++) {
+        int rc = clear_meta(c);
+        if (rc) return rc;
+        if (forces) {
+            c->cur = enqueue_force(c, c->cur, false, false);
+        } else {
+            Marks mk{c};
+            (void)mk;
+            launch_bbox(c, c->cur);
+            launch_morton(c, c->cur);
+            launch_radix_sort(c);
+            launch_gather(c, c->cur);
+            c->cur ^= 1;
+            launch_tree_build(c, c->cur);
+        }
+        NB_HIP(c, hipGetLastError());
+        rc = read_meta(c);
+        if (rc) return rc;
+        const uint32_t flags = c->host_meta[META_FLAGS];
+        c->last_internal = c->host_meta[META_INTERNAL];
+        if (flags & FLAG_NONFINITE) return fail(c, NB_ERR_NUMERIC, "non-finite body positions");
+        if (flags & FLAG_STACK) return fail(c, NB_ERR_OVERFLOW, "traversal stack overflow");
+        if (!(flags & FLAG_TREE_CAPACITY)) return NB_OK;
+        rc = grow_tree(c, c->last_internal);
+        if (rc) return rc;
+    }
+    return fail(c, NB_ERR_HIP, "tree capacity did not converge");
+}
+
+}  // namespace
+
+extern "C" {
+
+int nb_params_default(nb_params* p) {
+    if (!p) return NB_ERR_INVALID_ARG;
+    std::memset(p, 0, sizeof(*p));
+    p->G = 1.0;
+    p->dt = 1e-3;
+    p->theta = 0.5;
+    p->softening = 0.01;
+    p->algorithm = NB_ALGO_BARNES_HUT;
+    p->integrator = NB_INTEG_KD;
+    p->self_guard = 0;
+    p->far_field = NB_FAR_MONOPOLE;
+    p->use_graph = 1;
+    return NB_OK;
+}
+
+int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* pos_xy,
+              const double* vel_xy, nb_ctx** ctx_out) {
+    if (!ctx_out) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx_out is NULL");
+    *ctx_out = nullptr;
+    std::string why;
+    int rc = check_params(p, &why);
+    if (rc) return fail(nullptr, rc, why);
+    if (n > 0x7fffffffull) return fail(nullptr, NB_ERR_INVALID_ARG, "n exceeds 2^31-1 bodies");
+    if (n && (!mass || !pos_xy || !vel_xy))
+        return fail(nullptr, NB_ERR_INVALID_ARG, "NULL body array");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, NB_ERR_NO_DEVICE, "no HIP device");
+    nb_ctx* c = new (std::nothrow) nb_ctx();
+    if (!c) return fail(nullptr, NB_ERR_OUT_OF_MEMORY, "host allocation failed");
+    c->p = *p;
+    c->n = n;
+    c->dt = p->dt;
+    auto bail = [&](int code) {
+        g_err = c->err;
+        free_all(c);
+        delete c;
+        return code;
+    };
+#define NB_TRY(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            c->err = std::string(#expr) + ": " + hipGetErrorString(e_);                    \
+            return bail(e_ == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP);    \
+        }                                                                                  \
+    } while (0)
+    NB_TRY(hipGetDevice(&c->device));
+    NB_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) NB_TRY(hipEventCreate(&e));
+    NB_TRY(hipHostMalloc((void**)&c->host_meta, META_COUNT * sizeof(uint32_t)));
+    Buffers& b = c->b;
+    const bool bh = is_bh(c);
+    for (int k = 0; k < 2; k++) {
+        const uint64_t nk = (k == 0 || bh) ? n : 0;
+        NB_TRY(dalloc(c, &b.pos[k], nk));
+        NB_TRY(dalloc(c, &b.vel[k], nk));
+        NB_TRY(dalloc(c, &b.mass[k], nk));
+        NB_TRY(dalloc(c, &b.orig[k], nk));
+        NB_TRY(dalloc(c, &b.keys[k], bh ? n : 0));
+        NB_TRY(dalloc(c, &b.vals[k], bh ? n : 0));
+    }
+    c->sort_tiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
+    c->bbox_blocks = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((n + 2047) / 2048, 1), 1024);
+    const uint64_t scan_blocks = (n + 1023) / 1024 + 1;
+    NB_TRY(dalloc(c, &b.sort_hist, bh ? (uint64_t)kSortBins * c->sort_tiles : 0));
+    NB_TRY(dalloc(c, &b.sort_tot, kSortBins));
+    NB_TRY(dalloc(c, &b.box, 8));
+    NB_TRY(dalloc(c, &b.bbox_part, 4ull * c->bbox_blocks));
+    NB_TRY(dalloc(c, &b.delta, bh ? n : 0));
+    NB_TRY(dalloc(c, &b.cnt, bh ? n : 0));
+    NB_TRY(dalloc(c, &b.scan_tmp, scan_blocks));
+    NB_TRY(dalloc(c, &b.multi_list, bh ? n / 2 + 1 : 0));
+    NB_TRY(dalloc(c, &b.acc, n));
+    NB_TRY(dalloc(c, &b.inter, n));
+    NB_TRY(dalloc(c, &b.meta, META_COUNT));
+    NB_TRY(dalloc(c, &b.consts, 1));
+    if (!bh && n) {
+        const uint32_t ib = (uint32_t)((n + 1023) / 1024);
+        uint32_t split = (2048 + ib - 1) / ib;
+        split = std::max<uint32_t>(1, std::min<uint32_t>(split, 64));
+        while (split > 1 && (n + split - 1) / split < 256) split--;
+        c->ap_split = split;
+        NB_TRY(dalloc(c, &b.ap_part, (uint64_t)split * n));
+    }
+    if (n) {
+        NB_TRY(hipMemcpyAsync(b.pos[0], pos_xy, n * sizeof(double2), hipMemcpyHostToDevice,
+                              c->stream));
+        NB_TRY(hipMemcpyAsync(b.vel[0], vel_xy, n * sizeof(double2), hipMemcpyHostToDevice,
+                              c->stream));
+        NB_TRY(hipMemcpyAsync(b.mass[0], mass, n * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+        std::vector<uint32_t> iota(n);
+        for (uint64_t i = 0; i < n; i++) iota[i] = (uint32_t)i;
+        NB_TRY(hipMemcpyAsync(b.orig[0], iota.data(), n * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, c->stream));
+        NB_TRY(hipStreamSynchronize(c->stream));
+    }
+#undef NB_TRY
+    rc = upload_consts(c);
+    if (rc) return bail(rc);
+    if (bh && n) {
+        rc = alloc_tree(c, std::max<uint64_t>(n, 1024));
+        if (rc) return bail(rc);
+        rc = build_and_force(c, false);  // sizes the node pool to the first tree
+        if (rc) return bail(rc);
+        if (c->last_internal + c->last_internal / 4 > c->icap) {
+            rc = alloc_tree(c, c->last_internal + c->last_internal / 4 + 1024);
+            if (rc) return bail(rc);
+        }
+    }
+    rc = build_graphs(c);
+    if (rc) return bail(rc);
+    *ctx_out = c;
+    return NB_OK;
+}
+
+int nb_step(nb_ctx* c, uint64_t k) {
+    if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
+    if (c->n == 0) {
+        c->iteration += k;
+        c->sim_time += (double)k * c->dt;
+        return NB_OK;
+    }
+    uint64_t remaining = k;
+    while (remaining > 0) {
+        int rc = clear_meta(c);
+        if (rc) return rc;
+        for (uint64_t s = 0; s < remaining; s++) {
+            if (c->timing) {
+                const int dst = enqueue_step(c, c->cur);
+                NB_HIP(c, hipGetLastError());
+                NB_HIP(c, hipStreamSynchronize(c->stream));
+                for (int ph = 0; ph < NB_PHASE_COUNT; ph++) {
+                    float ms = 0.f;
+                    if (hipEventElapsedTime(&ms, c->ev[2 * ph], c->ev[2 * ph + 1]) == hipSuccess)
+                        c->phase_ms[ph] += ms;
+                }
+                c->steps_timed++;
+                c->cur = dst;
+            } else if (c->graph[c->cur]) {
+                NB_HIP(c, hipGraphLaunch(c->graph[c->cur], c->stream));
+                if (is_bh(c)) c->cur ^= 1;
+            } else {
+                c->cur = enqueue_step(c, c->cur);
+                NB_HIP(c, hipGetLastError());
+            }
+            // Reset the phase events so a phase absent from this variant
+            // contributes nothing next time.
+        }
+        rc = read_meta(c);
+        if (rc) return rc;
+        const uint32_t flags = c->host_meta[META_FLAGS];
+        const uint64_t done = std::min<uint64_t>(c->host_meta[META_STEPS_DONE], remaining);
+        c->last_internal = c->host_meta[META_INTERNAL];
+        c->iteration += done;
+        c->sim_time += (double)done * c->dt;
+        remaining -= done;
+        if (flags & FLAG_NONFINITE) return fail(c, NB_ERR_NUMERIC, "non-finite body positions");
+        if (flags & FLAG_STACK) return fail(c, NB_ERR_OVERFLOW, "traversal stack overflow");
+        if (flags & FLAG_TREE_CAPACITY) {
+            rc = grow_tree(c, c->last_internal);
+            if (rc) return rc;
+            if (is_dk(c) && remaining > 0) {
+                // The halted step drifted but did not kick: finish it.
+                rc = clear_meta(c);
+                if (rc) return rc;
+                c->cur = enqueue_force(c, c->cur, true, true);
+                NB_HIP(c, hipGetLastError());
+                rc = read_meta(c);
+                if (rc) return rc;
+                if (c->host_meta[META_FLAGS] & FLAG_TREE_CAPACITY)
+                    return fail(c, NB_ERR_HIP, "tree capacity did not converge");
+                c->iteration += 1;
+                c->sim_time += c->dt;
+                remaining -= 1;
+            }
+            continue;
+        }
+        if (remaining > 0) return fail(c, NB_ERR_HIP, "steps did not complete");
+    }
+    return NB_OK;
+}
+
+int nb_get_state(nb_ctx* c, double* pos_xy, double* vel_xy) {
+    if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
+    const uint64_t n = c->n;
+    if (n == 0) return NB_OK;
+    std::vector<double> p(pos_xy ? 2 * n : 0), v(vel_xy ? 2 * n : 0);
+    std::vector<uint32_t> o(n);
+    if (pos_xy)
+        NB_HIP(c, hipMemcpyAsync(p.data(), c->b.pos[c->cur], n * sizeof(double2),
+                                 hipMemcpyDeviceToHost, c->stream));
+    if (vel_xy)
+        NB_HIP(c, hipMemcpyAsync(v.data(), c->b.vel[c->cur], n * sizeof(double2),
+                                 hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipMemcpyAsync(o.data(), c->b.orig[c->cur], n * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    for (uint64_t i = 0; i < n; i++) {
+        const uint64_t j = o[i];
+        if (pos_xy) {
+            pos_xy[2 * j] = p[2 * i];
+            pos_xy[2 * j + 1] = p[2 * i + 1];
+        }
+        if (vel_xy) {
+            vel_xy[2 * j] = v[2 * i];
+            vel_xy[2 * j + 1] = v[2 * i + 1];
+        }
+    }
+    return NB_OK;
+}
+
+int nb_compute_forces(nb_ctx* c, double* acc_xy, uint32_t* interactions, uint64_t* total) {
+    if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
+    const uint64_t n = c->n;
+    if (total) *total = 0;
+    if (n == 0) return NB_OK;
+    int rc = build_and_force(c, true);
+    if (rc) return rc;
+    std::vector<double> a(2 * n);
+    std::vector<uint32_t> k(n), o(n);
+    NB_HIP(c, hipMemcpyAsync(a.data(), c->b.acc, n * sizeof(double2), hipMemcpyDeviceToHost,
+                             c->stream));
+    NB_HIP(c, hipMemcpyAsync(k.data(), c->b.inter, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    NB_HIP(c, hipMemcpyAsync(o.data(), c->b.orig[c->cur], n * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    uint64_t t = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        const uint64_t j = o[i];
+        if (acc_xy) {
+            acc_xy[2 * j] = a[2 * i];
+            acc_xy[2 * j + 1] = a[2 * i + 1];
+        }
+        if (interactions) interactions[j] = k[i];
+        t += k[i];
+    }
+    if (total) *total = t;
+    return NB_OK;
+}
+
+int nb_set_dt(nb_ctx* c, double dt) {
+    if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
+    if (!std::isfinite(dt) || dt <= 0.0) return fail(c, NB_ERR_INVALID_ARG, "dt must be > 0");
+    c->dt = dt;
+    return upload_consts(c);
+}
+
+int nb_get_info(nb_ctx* c, nb_info* info) {
+    if (!c || !info) return fail(c, NB_ERR_INVALID_ARG, "NULL argument");
+    info->n = c->n;
+    info->iteration = c->iteration;
+    info->simulated_time = c->sim_time;
+    info->internal_nodes = c->last_internal;
+    info->tree_nodes = c->n ? 1 + 4 * c->last_internal : 0;
+    info->node_capacity = c->icap;
+    info->device_bytes = c->device_bytes;
+    return NB_OK;
+}
+
+int nb_set_timing(nb_ctx* c, int enable) {
+    if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
+    c->timing = enable != 0;
+    for (double& m : c->phase_ms) m = 0.0;
+    c->steps_timed = 0;
+    return NB_OK;
+}
+
+int nb_get_timing(nb_ctx* c, double* phase_ms, int nphase, uint64_t* steps_timed) {
+    if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
+    for (int k = 0; k < nphase && k < NB_PHASE_COUNT; k++) phase_ms[k] = c->phase_ms[k];
+    if (steps_timed) *steps_timed = c->steps_timed;
+    return NB_OK;
+}
+
+void nb_destroy(nb_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+}
+
+const char* nb_last_error(const nb_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+const char* nb_status_string(int s) {
+    switch (s) {
+    case NB_OK: return "ok";
+    case NB_ERR_INVALID_ARG: return "invalid argument";
+    case NB_ERR_HIP: return "HIP runtime error";
+    case NB_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case NB_ERR_OVERFLOW: return "traversal stack overflow";
+    case NB_ERR_NOT_SUPPORTED: return "not supported";
+    case NB_ERR_NO_DEVICE: return "no HIP device";
+    case NB_ERR_NUMERIC: return "non-finite state";
+    default: return "unknown status";
+    }
+}
+
+int nb_debug_build(nb_ctx* c, double* box4, uint64_t* sorted_keys, uint32_t* orig_index,
+                   uint64_t* node_count) {
+    if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
+    if (!is_bh(c)) return fail(c, NB_ERR_NOT_SUPPORTED, "no tree for all-pairs");
+    const uint64_t n = c->n;
+    if (n == 0) {
+        if (node_count) *node_count = 0;
+        return NB_OK;
+    }
+    int rc = build_and_force(c, false);
+    if (rc) return rc;
+    if (box4) NB_HIP(c, hipMemcpy(box4, c->b.box, 4 * sizeof(double), hipMemcpyDeviceToHost));
+    if (sorted_keys)
+        NB_HIP(c, hipMemcpy(sorted_keys, c->b.keys[0], n * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost));
+    if (orig_index)
+        NB_HIP(c, hipMemcpy(orig_index, c->b.orig[c->cur], n * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost));
+    if (node_count) *node_count = n > 1 ? 1 + 4 * c->last_internal : 1;
+    return NB_OK;
+}
+
+int nb_debug_tree(nb_ctx* c, nb_node* nodes, uint64_t cap) {
+    if (!c || !nodes) return fail(c, NB_ERR_INVALID_ARG, "NULL argument");
+    if (c->n == 0) return NB_OK;
+    const uint64_t count = c->n > 1 ? 1 + 4 * c->last_internal : 1;
+    if (cap < count) return fail(c, NB_ERR_INVALID_ARG, "cap smaller than the node count");
+    std::vector<NodeRec> h(count);
+    NB_HIP(c, hipMemcpy(h.data(), c->b.nodes, count * sizeof(NodeRec), hipMemcpyDeviceToHost));
+    std::vector<int32_t> level(count, 0);
+    for (uint64_t s = 0; s < count; s++) {
+        const NodeRec& r = h[s];
+        if (r.c0 >= 1 && (uint64_t)r.c0 + 3 < count)
+            for (int q = 0; q < 4; q++) level[r.c0 + q] = level[s] + 1;
+        nb_node& o = nodes[s];
+        o.level = level[s];
+        o.bstart = r.bs;
+        o.bend = r.be;
+        o.c0 = r.c0;
+        o.mass = r.m;
+        o.cx = r.cx;
+        o.cy = r.cy;
+        o.wsq = r.wsq;
+        o.ixx = r.ixx;
+        o.ixy = r.ixy;
+        o.iyy = r.iyy;
+        o.child_mask = r.c0 >= 1 ? r.mask : 0;
+    }
+    return NB_OK;
+}
+
+int nb_direct_sum_f64(uint64_t n, const double* pos_xy, const double* mass, double G,
+                      double softening, uint64_t m, const int64_t* targets, double* acc_xy) {
+    if ((n && (!pos_xy || !mass)) || (m && (!targets || !acc_xy)))
+        return fail(nullptr, NB_ERR_INVALID_ARG, "NULL argument");
+    if (n > 0x7fffffffull || m > 0x7fffffffull)
+        return fail(nullptr, NB_ERR_INVALID_ARG, "size exceeds 2^31-1");
+    for (uint64_t t = 0; t < m; t++)
+        if (targets[t] < 0 || (uint64_t)targets[t] >= n)
+            return fail(nullptr, NB_ERR_INVALID_ARG, "target index out of range");
+    if (m == 0) return NB_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, NB_ERR_NO_DEVICE, "no HIP device");
+    const uint32_t gx = (uint32_t)((m + 255) / 256);
+    uint32_t split = std::max<uint32_t>(1, std::min<uint32_t>((2048 + gx - 1) / gx, 256));
+    while (split > 1 && (n + split - 1) / split < 256) split--;
+    double2 *dpos = nullptr, *dpart = nullptr;
+    double* dmass = nullptr;
+    int64_t* dt = nullptr;
+    hipStream_t s = nullptr;
+    int rc = NB_OK;
+    std::vector<double2> part((uint64_t)split * m);
+    auto run = [&]() -> hipError_t {
+        hipError_t e;
+        if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking))) return e;
+        if ((e = hipMalloc((void**)&dpos, n * sizeof(double2)))) return e;
+        if ((e = hipMalloc((void**)&dmass, n * sizeof(double)))) return e;
+        if ((e = hipMalloc((void**)&dt, m * sizeof(int64_t)))) return e;
+        if ((e = hipMalloc((void**)&dpart, (uint64_t)split * m * sizeof(double2)))) return e;
+        if ((e = hipMemcpyAsync(dpos, pos_xy, n * sizeof(double2), hipMemcpyHostToDevice, s)))
+            return e;
+        if ((e = hipMemcpyAsync(dmass, mass, n * sizeof(double), hipMemcpyHostToDevice, s)))
+            return e;
+        if ((e = hipMemcpyAsync(dt, targets, m * sizeof(int64_t), hipMemcpyHostToDevice, s)))
+            return e;
+        if ((e = direct_sum_f64(n, dpos, dmass, softening * softening, m, dt, dpart, split, s)))
+            return e;
+        if ((e = hipMemcpyAsync(part.data(), dpart, (uint64_t)split * m * sizeof(double2),
+                                hipMemcpyDeviceToHost, s)))
+            return e;
+        return hipStreamSynchronize(s);
+    };
+    const hipError_t e = run();
+    if (e != hipSuccess)
+        rc = fail(nullptr, e == hipErrorOutOfMemory ? NB_ERR_OUT_OF_MEMORY : NB_ERR_HIP,
+                  std::string("direct sum: ") + hipGetErrorString(e));
+    if (dpos) (void)hipFree(dpos);
+    if (dmass) (void)hipFree(dmass);
+    if (dt) (void)hipFree(dt);
+    if (dpart) (void)hipFree(dpart);
+    if (s) (void)hipStreamDestroy(s);
+    if (rc) return rc;
+    for (uint64_t t = 0; t < m; t++) {
+        double ax = 0.0, ay = 0.0;
+        for (uint32_t k = 0; k < split; k++) {
+            ax += part[(uint64_t)k * m + t].x;
+            ay += part[(uint64_t)k * m + t].y;
+        }
+        acc_xy[2 * t] = ax * G;
+        acc_xy[2 * t + 1] = ay * G;
+    }
+    return NB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,358 @@
+// nb_force.hip — force evaluation and integration kernels.
+//
+//   k_traverse    Barnes-Hut force, one wavefront per 64 Morton-adjacent
+//                 bodies sharing one LDS node stack (compute_force,
+//                 BarnesHutCuda.cu:382-486, re-designed for wave64)
+//   k_allpairs    LDS-tiled direct sum (AllPairsSim, AllPairs.cpp:38-49)
+//   k_drift       x += v dt (update_positions, BarnesHutCuda.cu:489-495)
+//   k_direct64    FP64 direct sum, the judge of force errors
+//
+// Precision: the cell offset dx = com - x, its square d^2 = fma(dx, dx, dy*dy)
+// and the opening test wsq < theta^2 d^2 are evaluated in FP64 on the FP64
+// state, exactly the reference's expressions, so every body accepts the same
+// cells as the reference's per-body FP64 DFS. The interaction itself
+// (d^2 + eps^2, rsqrt, r^-3, m r^-3, the two products) runs in FP32 and is
+// summed per body with Kahan compensation; the kick v += a G dt is FP64.
+#include "nb_device.h"
+#include "nb_internal.h"
+
+namespace nb {
+
+__device__ __forceinline__ void kahan_add(float& sum, float& comp, float v) {
+    const float y = v - comp;
+    const float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+struct Acc {
+    float ax = 0.f, ay = 0.f, cx = 0.f, cy = 0.f;
+    uint32_t n = 0;
+};
+
+// One softened monopole term m d / (d^2 + eps^2)^{3/2}.
+__device__ __forceinline__ void interact(Acc& a, double dx, double dy, double d2, float m,
+                                         float eps2f) {
+    const float r2 = (float)d2 + eps2f;
+    const float rs = __builtin_amdgcn_rsqf(r2);
+    const float s = m * (rs * rs * rs);
+    kahan_add(a.ax, a.cx, (float)dx * s);
+    kahan_add(a.ay, a.cy, (float)dy * s);
+    a.n++;
+}
+
+constexpr int kTravThreads = 256;
+constexpr int kTravWaves = kTravThreads / 64;
+constexpr uint32_t kHaltFlags = FLAG_TREE_CAPACITY | FLAG_NONFINITE;
+
+// Wavefront-cooperative traversal. The stack entries are (slot, lane mask):
+// the mask holds the lanes that still need the subtree. A popped internal
+// node is tested by every lane in its mask; lanes that accept take its
+// monopole, the others are pushed with its non-empty children (c0+3 .. c0,
+// so c0 pops first, BarnesHutCuda.cu:474-481). Each lane therefore sees
+// exactly the nodes, in exactly the order, of its own per-body DFS, while the
+// wave fetches each node once (a wave-uniform scalar load).
+template <bool KICK, bool GUARD>
+__global__ __launch_bounds__(kTravThreads) void k_traverse(
+    const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
+    const double* __restrict__ mass, double2* __restrict__ vel, double2* __restrict__ acc_out,
+    uint32_t* __restrict__ inter_out, uint32_t n, const StepConsts* __restrict__ k,
+    uint32_t* __restrict__ meta, int count_step) {
+    if (meta[META_FLAGS] & kHaltFlags) return;
+    __shared__ int32_t s_node[kTravWaves][kStackDepth];
+    __shared__ uint64_t s_mask[kTravWaves][kStackDepth];
+    const int w = threadIdx.x >> 6, l = lane_id();
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t i = (blk * kTravWaves + w) * 64 + l;
+    const bool valid = i < n;
+    const uint64_t act = ballot64(valid);
+    if (count_step && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
+    if (act == 0) return;
+    const double2 p = valid ? pos[i] : make_double2(0.0, 0.0);
+    const double theta_sq = k->theta_sq;
+    const float eps2f = (float)k->eps2;
+    const int32_t me = (int32_t)i;
+    Acc a;
+    int sp = 1;
+    s_node[w][0] = 0;
+    s_mask[w][0] = act;
+    bool overflow = false;
+    while (sp > 0) {
+        --sp;
+        const int32_t node = readfirst(s_node[w][sp]);
+        const uint64_t m = readfirst64(s_mask[w][sp]);
+        const NodeRec* r = nodes + node;
+        const double rcx = r->cx, rcy = r->cy;
+        const int32_t c0 = r->c0, bs = r->bs, be = r->be;
+        const bool mine = (m >> l) & 1ull;
+        const double dx = rcx - p.x, dy = rcy - p.y;
+        const double d2 = fma(dx, dx, dy * dy);
+        if (c0 < 0) {
+            // Leaf (empty slots are never pushed). BarnesHutCuda.cu:421-458.
+            if (mine) {
+                if (bs == be) {
+                    if (bs != me) interact(a, dx, dy, d2, r->mf, eps2f);
+                } else if (me < bs || me > be) {
+                    interact(a, dx, dy, d2, r->mf, eps2f);
+                } else {
+                    for (int32_t j = bs; j <= be; j++) {
+                        if (j == me) continue;
+                        const double2 q = pos[j];
+                        const double ex = q.x - p.x, ey = q.y - p.y;
+                        interact(a, ex, ey, fma(ex, ex, ey * ey), (float)mass[j], eps2f);
+                    }
+                }
+            }
+        } else {
+            bool accept = mine && (r->wsq < theta_sq * d2);
+            if (GUARD) accept = accept && !(me >= bs && me <= be);
+            if (accept) interact(a, dx, dy, d2, r->mf, eps2f);
+            const uint64_t open = ballot64(mine && !accept);
+            if (open) {
+                if (sp + 4 > kStackDepth) {
+                    overflow = true;
+                    break;
+                }
+                const int32_t cm = r->mask;
+#pragma unroll
+                for (int q = 3; q >= 0; q--) {
+                    if ((cm >> q) & 1) {
+                        s_node[w][sp] = c0 + q;
+                        s_mask[w][sp] = open;
+                        sp++;
+                    }
+                }
+            }
+        }
+    }
+    if (overflow && l == 0) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_STACK);
+    if (!valid) return;
+    const double ax = (double)a.ax - (double)a.cx;
+    const double ay = (double)a.ay - (double)a.cy;
+    if (KICK) {
+        const double gdt = k->G_dt;
+        double2 v = vel[i];
+        v.x += ax * gdt;
+        v.y += ay * gdt;
+        vel[i] = v;
+    } else {
+        const double G = k->G;
+        acc_out[i] = make_double2(ax * G, ay * G);
+        inter_out[i] = a.n;
+    }
+}
+
+void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
+    const uint32_t n = (uint32_t)c->n;
+    if (n == 0) return;
+    const uint32_t blocks = (n + kTravThreads - 1) / kTravThreads;
+    const bool guard = c->p.self_guard != 0;
+#define NB_TRAV(K, G)                                                                          \
+    hipLaunchKernelGGL((k_traverse<K, G>), dim3(blocks), dim3(kTravThreads), 0, c->stream,     \
+                       c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src], c->b.acc,     \
+                       c->b.inter, n, c->b.consts, c->b.meta, (int)count_step)
+    if (kick) {
+        if (guard) NB_TRAV(true, true); else NB_TRAV(true, false);
+    } else {
+        if (guard) NB_TRAV(false, true); else NB_TRAV(false, false);
+    }
+#undef NB_TRAV
+}
+
+// ---------------------------------------------------------------------------
+// All-pairs: each thread owns R bodies, the workgroup streams 256-body source
+// tiles through LDS as FP32 {x, y, m}; the j range is split across
+// blockIdx.y so a 262,144-body launch still fills 256 CUs. Each 256-term tile
+// is summed plainly, and tile sums are added to the body's running total with
+// Kahan compensation.
+// ---------------------------------------------------------------------------
+constexpr int kApThreads = 256;
+constexpr int kApR = 4;
+
+template <bool ZERO_EPS>
+__global__ __launch_bounds__(kApThreads) void k_allpairs(const double2* __restrict__ pos,
+                                                         const double* __restrict__ mass,
+                                                         uint32_t n, uint32_t chunk,
+                                                         const StepConsts* __restrict__ k,
+                                                         double2* __restrict__ part) {
+    __shared__ float4 tile[kApThreads];
+    const uint32_t ib = blockIdx.x * (kApThreads * kApR) + threadIdx.x;
+    float xi[kApR], yi[kApR];
+    float ax[kApR], ay[kApR], cx[kApR], cy[kApR];
+#pragma unroll
+    for (int r = 0; r < kApR; r++) {
+        const uint32_t i = ib + r * kApThreads;
+        const double2 p = i < n ? pos[i] : make_double2(0.0, 0.0);
+        xi[r] = (float)p.x;
+        yi[r] = (float)p.y;
+        ax[r] = ay[r] = cx[r] = cy[r] = 0.f;
+    }
+    const float eps2f = (float)k->eps2;
+    const uint32_t j0 = blockIdx.y * chunk;
+    const uint32_t j1 = min(j0 + chunk, n);
+    for (uint32_t jt = j0; jt < j1; jt += kApThreads) {
+        const uint32_t j = jt + threadIdx.x;
+        if (j < j1) {
+            const double2 q = pos[j];
+            tile[threadIdx.x] = make_float4((float)q.x, (float)q.y, (float)mass[j], 0.f);
+        } else {
+            tile[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+        float px[kApR], py[kApR];
+#pragma unroll
+        for (int r = 0; r < kApR; r++) px[r] = py[r] = 0.f;
+#pragma unroll 8
+        for (int jj = 0; jj < kApThreads; jj++) {
+            const float4 s = tile[jj];
+#pragma unroll
+            for (int r = 0; r < kApR; r++) {
+                const float dx = s.x - xi[r], dy = s.y - yi[r];
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2f));
+                float rs = __builtin_amdgcn_rsqf(r2);
+                if (ZERO_EPS) rs = r2 > 0.f ? rs : 0.f;
+                const float w = s.z * (rs * rs * rs);
+                px[r] = fmaf(dx, w, px[r]);
+                py[r] = fmaf(dy, w, py[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kApR; r++) {
+            kahan_add(ax[r], cx[r], px[r]);
+            kahan_add(ay[r], cy[r], py[r]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < kApR; r++) {
+        const uint32_t i = ib + r * kApThreads;
+        if (i < n)
+            part[(uint64_t)blockIdx.y * n + i] =
+                make_double2((double)ax[r] - (double)cx[r], (double)ay[r] - (double)cy[r]);
+    }
+}
+
+// Sums the split partials in a fixed order, then kicks or stores the result.
+__global__ __launch_bounds__(256) void k_allpairs_finish(
+    const double2* __restrict__ part, uint32_t split, uint32_t n, const StepConsts* __restrict__ k,
+    double2* __restrict__ vel, double2* __restrict__ acc_out, uint32_t* __restrict__ inter_out,
+    int kick, uint32_t* __restrict__ meta, int count_step) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (count_step && i == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
+    if (i >= n) return;
+    double ax = 0.0, ay = 0.0;
+    for (uint32_t s = 0; s < split; s++) {
+        const double2 v = part[(uint64_t)s * n + i];
+        ax += v.x;
+        ay += v.y;
+    }
+    if (kick) {
+        double2 v = vel[i];
+        v.x += ax * k->G_dt;
+        v.y += ay * k->G_dt;
+        vel[i] = v;
+    } else {
+        acc_out[i] = make_double2(ax * k->G, ay * k->G);
+        inter_out[i] = n - 1;
+    }
+}
+
+void launch_allpairs(nb_ctx* c, int src, bool kick, bool count_step) {
+    const uint32_t n = (uint32_t)c->n;
+    if (n == 0) return;
+    const uint32_t ib = (n + kApThreads * kApR - 1) / (kApThreads * kApR);
+    const uint32_t split = c->ap_split;
+    uint32_t chunk = (n + split - 1) / split;
+    chunk = (chunk + kApThreads - 1) / kApThreads * kApThreads;
+    if (c->p.softening > 0.0)
+        hipLaunchKernelGGL((k_allpairs<false>), dim3(ib, split), dim3(kApThreads), 0, c->stream,
+                           c->b.pos[src], c->b.mass[src], n, chunk, c->b.consts, c->b.ap_part);
+    else
+        hipLaunchKernelGGL((k_allpairs<true>), dim3(ib, split), dim3(kApThreads), 0, c->stream,
+                           c->b.pos[src], c->b.mass[src], n, chunk, c->b.consts, c->b.ap_part);
+    hipLaunchKernelGGL(k_allpairs_finish, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                       c->b.ap_part, split, n, c->b.consts, c->b.vel[src], c->b.acc, c->b.inter,
+                       (int)kick, c->b.meta, (int)count_step);
+}
+
+// ---------------------------------------------------------------------------
+// Drift x += v dt. HBM traffic: 16 B vel + 16 B pos read, 16 B pos written.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_drift(double2* __restrict__ pos,
+                                               const double2* __restrict__ vel, uint32_t n,
+                                               const StepConsts* __restrict__ k,
+                                               uint32_t* __restrict__ meta, int check,
+                                               int count_step) {
+    if (check && (meta[META_FLAGS] & kHaltFlags)) return;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (count_step && i == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
+    if (i >= n) return;
+    const double dt = k->dt;
+    double2 x = pos[i];
+    const double2 v = vel[i];
+    x.x += v.x * dt;
+    x.y += v.y * dt;
+    pos[i] = x;
+}
+
+void launch_drift(nb_ctx* c, int src, bool check, bool count_step) {
+    const uint32_t n = (uint32_t)c->n;
+    hipLaunchKernelGGL(k_drift, dim3(n ? (n + 255) / 256 : 1), dim3(256), 0, c->stream,
+                       c->b.pos[src], c->b.vel[src], n, c->b.consts, c->b.meta, (int)check,
+                       (int)count_step);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 direct sum (judge): thread per target, sources through LDS in FP64,
+// j range split across blockIdx.y, Kahan-compensated FP64 sums, IEEE sqrt and
+// division.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_direct64(const double2* __restrict__ pos,
+                                                  const double* __restrict__ mass, uint32_t n,
+                                                  double eps2, const int64_t* __restrict__ tgt,
+                                                  uint32_t m, uint32_t chunk,
+                                                  double2* __restrict__ part) {
+    __shared__ double3 tile[256];
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    const int64_t ti = t < m ? tgt[t] : -1;
+    const double2 p = ti >= 0 ? pos[ti] : make_double2(0.0, 0.0);
+    double sx = 0.0, sy = 0.0, cx = 0.0, cy = 0.0;
+    const uint32_t j0 = blockIdx.y * chunk, j1 = min(j0 + chunk, n);
+    for (uint32_t jt = j0; jt < j1; jt += 256) {
+        const uint32_t j = jt + threadIdx.x;
+        if (j < j1) {
+            const double2 q = pos[j];
+            tile[threadIdx.x] = make_double3(q.x, q.y, mass[j]);
+        }
+        __syncthreads();
+        const uint32_t cnt = min(256u, j1 - jt);
+        for (uint32_t jj = 0; jj < cnt; jj++) {
+            if ((int64_t)(jt + jj) == ti) continue;
+            const double3 s = tile[jj];
+            const double dx = s.x - p.x, dy = s.y - p.y;
+            const double r2 = dx * dx + dy * dy + eps2;
+            if (r2 == 0.0) continue;
+            const double w = s.z / (r2 * sqrt(r2));
+            const double vx = dx * w - cx, tx = sx + vx;
+            cx = (tx - sx) - vx;
+            sx = tx;
+            const double vy = dy * w - cy, ty = sy + vy;
+            cy = (ty - sy) - vy;
+            sy = ty;
+        }
+        __syncthreads();
+    }
+    if (t < m) part[(uint64_t)blockIdx.y * m + t] = make_double2(sx - cx, sy - cy);
+}
+
+hipError_t direct_sum_f64(uint64_t n, const double2* pos, const double* mass, double eps2,
+                          uint64_t m, const int64_t* targets, double2* part, uint32_t split,
+                          hipStream_t s) {
+    uint32_t chunk = (uint32_t)((n + split - 1) / split);
+    chunk = (chunk + 255) / 256 * 256;
+    hipLaunchKernelGGL(k_direct64, dim3((uint32_t)((m + 255) / 256), split), dim3(256), 0, s, pos,
+                       mass, (uint32_t)n, eps2, targets, (uint32_t)m, chunk, part);
+    return hipGetLastError();
+}
+
+}  // namespace nb

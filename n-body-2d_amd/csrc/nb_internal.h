@@ -1,0 +1,145 @@
+// nb_internal.h — shared declarations of the gfx950 N-body library.
+//
+// Device data layout (one context, N bodies):
+//   bodies   pos double2[N], vel double2[N], mass double[N], orig u32[N], two
+//            copies (ping-pong). Barnes-Hut keeps them in Morton order across
+//            steps, as BarnesHutCuda does (BarnesHutCuda.cu:701-708);
+//            orig[i] is the caller's index of sorted body i.
+//   sort     keys u64[N] x2, vals u32[N] x2, per-tile digit histograms.
+//   tree     NodeRec[1 + 4*I] slots (I = internal nodes). Internal node p
+//            (numbered in pre-order) owns the four child slots 1+4p .. 4+4p,
+//            the layout of the reference's contiguous children
+//            (BarnesHutCuda.cu:212-221) with a deterministic numbering.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/nbody.h"
+
+namespace nb {
+
+constexpr int kWave = 64;
+constexpr int kMaxLevel = 30;          // level-30 nodes are leaves (BarnesHutCuda.cu:182)
+constexpr int kStackDepth = 128;       // >= 3*30+1, the deepest masked DFS stack
+constexpr int kSortBits = 8;           // radix digit width
+constexpr int kSortBins = 1 << kSortBits;
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;         // keys per thread per tile
+constexpr int kSortTile = kSortThreads * kSortItems;
+
+// Meta words kept in device memory (u32 each).
+enum Meta : int {
+    META_INTERNAL = 0,   // internal nodes of the last build
+    META_FLAGS = 1,      // FLAG_* bits of the current step
+    META_STEPS_DONE = 2, // completed steps since the host last cleared it
+    META_MULTI_LEAVES = 3, // multi-body leaves queued by the current build
+    META_COUNT = 8
+};
+enum Flag : uint32_t {
+    FLAG_TREE_CAPACITY = 1u,  // internal nodes exceed the allocated capacity
+    FLAG_STACK = 2u,          // traversal stack overflow
+    FLAG_NONFINITE = 4u,      // non-finite bounding box
+};
+
+// One tree slot, 64 bytes so one wave-uniform node fetch is a single 64-byte
+// scalar load.
+struct __attribute__((aligned(64))) NodeRec {
+    double cx, cy;   // centre of mass
+    double m;        // mass
+    double wsq;      // squared cell side (BarnesHutCuda.cu:216-217)
+    int32_t c0;      // first child slot (internal), -1 leaf, -2 empty
+    int32_t mask;    // internal: bit q set = child c0+q holds bodies
+    int32_t bs, be;  // body range [bs, be] in sorted order
+    float ixx, ixy, iyy;  // second moments about the COM
+    float mf;        // mass as FP32 for the pair arithmetic
+};
+static_assert(sizeof(NodeRec) == 64, "NodeRec must be 64 bytes");
+
+// Device-side constants of a step, read by the kernels so set_dt needs no
+// graph rebuild.
+struct StepConsts {
+    double dt;
+    double G;
+    double G_dt;      // G * dt, as BarnesHutCuda.cu:814
+    double theta_sq;  // theta * theta (BarnesHutCuda.cpp:8)
+    double eps2;      // softening^2
+};
+
+struct Buffers {
+    double2* pos[2];
+    double2* vel[2];
+    double* mass[2];
+    uint32_t* orig[2];
+    uint64_t* keys[2];
+    uint32_t* vals[2];
+    uint32_t* sort_hist;   // [bins][tiles]
+    uint32_t* sort_tot;    // [bins]
+    double* box;           // x_min, x_max, y_min, y_max, root_wsq
+    double* bbox_part;     // [blocks][4]
+    int8_t* delta;         // [N] common-prefix digit count of pair (i, i+1)
+    uint32_t* cnt;         // [N] internal nodes owned by pair i, then its offset
+    uint32_t* scan_tmp;    // block sums of the scans
+    uint32_t* inode_slot;  // [icap] slot of internal node p
+    uint8_t* inode_level;  // [icap]
+    uint8_t* inode_mask;   // [icap]
+    NodeRec* nodes;        // [1 + 4*icap]
+    uint32_t* multi_list;  // [N/2 + 1] slots of multi-body leaves
+    double2* acc;          // [N] accelerations (nb_compute_forces) / partial sums
+    uint32_t* inter;       // [N] force terms per body
+    double2* ap_part;      // all-pairs partial sums [split][N]
+    uint32_t* meta;        // [META_COUNT]
+    StepConsts* consts;
+};
+
+}  // namespace nb
+
+struct nb_ctx {
+    nb_params p;
+    uint64_t n = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    nb::Buffers b{};
+    int cur = 0;                 // which copy holds the state
+    uint64_t icap = 0;           // internal-node capacity
+    uint32_t bbox_blocks = 0;
+    uint32_t sort_tiles = 0;
+    uint32_t scan_blocks = 0;
+    uint32_t ap_split = 1;
+    uint64_t device_bytes = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    bool timing = false;
+    hipEvent_t ev[2 * NB_PHASE_COUNT] = {};  // start/stop per phase
+    double phase_ms[NB_PHASE_COUNT] = {};
+    uint64_t steps_timed = 0;
+    uint64_t iteration = 0;
+    double sim_time = 0.0;
+    double dt = 0.0;
+    uint64_t last_internal = 0;
+    uint32_t* host_meta = nullptr;  // pinned copy of meta
+    std::string err;
+};
+
+namespace nb {
+
+// ---- launchers (nb_sort.hip) ----
+void launch_bbox(nb_ctx* c, int src);
+void launch_morton(nb_ctx* c, int src);
+void launch_radix_sort(nb_ctx* c);
+void launch_gather(nb_ctx* c, int src);
+void launch_scan_u32(nb_ctx* c, uint32_t* data, uint32_t count, uint32_t* total_out);
+
+// ---- launchers (nb_tree.hip) ----
+void launch_tree_build(nb_ctx* c, int src);
+
+// ---- launchers (nb_force.hip) ----
+void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step);
+void launch_allpairs(nb_ctx* c, int src, bool kick, bool count_step);
+void launch_drift(nb_ctx* c, int src, bool check, bool count_step);
+hipError_t direct_sum_f64(uint64_t n, const double2* pos, const double* mass, double eps2,
+                          uint64_t m, const int64_t* targets, double2* part, uint32_t split,
+                          hipStream_t s);
+
+}  // namespace nb

@@ -1,0 +1,386 @@
+// nb_sort.hip — bounding box, Morton keys, LDS-staged LSD radix sort, gather.
+//
+// Semantics follow BarnesHutCuda.cu: the squared bounding box (:78-114,
+// :648-680), 64-bit Morton keys with x in the odd bits (:54-69, :117-132), a
+// stable ascending sort over all 64 key bits with the current index as payload
+// (cub::DeviceRadixSort::SortPairs, :691-693) and the gather of every body
+// array through the permutation (:135-147). The squaring runs on the GPU, so
+// a step has no host round trip.
+#include <float.h>
+
+#include "nb_device.h"
+#include "nb_internal.h"
+
+namespace nb {
+
+// ---------------------------------------------------------------------------
+// Bounding box: grid-stride min/max, wave shuffles, one partial per block,
+// then a single-block kernel squares the box about the shorter axis.
+// HBM traffic: 16 B/body read.
+// ---------------------------------------------------------------------------
+constexpr int kBboxThreads = 256;
+
+__global__ __launch_bounds__(kBboxThreads) void k_bbox_partial(const double2* __restrict__ pos,
+                                                               uint32_t n,
+                                                               double* __restrict__ part) {
+    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    for (uint32_t i = blockIdx.x * kBboxThreads + threadIdx.x; i < n;
+         i += gridDim.x * kBboxThreads) {
+        const double2 p = pos[i];
+        xmin = fmin(xmin, p.x);
+        xmax = fmax(xmax, p.x);
+        ymin = fmin(ymin, p.y);
+        ymax = fmax(ymax, p.y);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        xmin = fmin(xmin, __shfl_xor(xmin, d, 64));
+        xmax = fmax(xmax, __shfl_xor(xmax, d, 64));
+        ymin = fmin(ymin, __shfl_xor(ymin, d, 64));
+        ymax = fmax(ymax, __shfl_xor(ymax, d, 64));
+    }
+    __shared__ double s[4][kBboxThreads / 64];
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        s[0][w] = xmin;
+        s[1][w] = xmax;
+        s[2][w] = ymin;
+        s[3][w] = ymax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBboxThreads / 64; k++) {
+            xmin = fmin(xmin, s[0][k]);
+            xmax = fmax(xmax, s[1][k]);
+            ymin = fmin(ymin, s[2][k]);
+            ymax = fmax(ymax, s[3][k]);
+        }
+        part[4 * blockIdx.x + 0] = xmin;
+        part[4 * blockIdx.x + 1] = xmax;
+        part[4 * blockIdx.x + 2] = ymin;
+        part[4 * blockIdx.x + 3] = ymax;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ part, uint32_t nparts,
+                                                   double* __restrict__ box,
+                                                   uint32_t* __restrict__ meta) {
+    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    for (uint32_t k = threadIdx.x; k < nparts; k += 64) {
+        xmin = fmin(xmin, part[4 * k + 0]);
+        xmax = fmax(xmax, part[4 * k + 1]);
+        ymin = fmin(ymin, part[4 * k + 2]);
+        ymax = fmax(ymax, part[4 * k + 3]);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        xmin = fmin(xmin, __shfl_xor(xmin, d, 64));
+        xmax = fmax(xmax, __shfl_xor(xmax, d, 64));
+        ymin = fmin(ymin, __shfl_xor(ymin, d, 64));
+        ymax = fmax(ymax, __shfl_xor(ymax, d, 64));
+    }
+    if (threadIdx.x == 0) {
+        // Square about the midpoint of the shorter axis (BarnesHutCuda.cu:665-677).
+        const double xr = xmax - xmin, yr = ymax - ymin;
+        if (xr > yr) {
+            const double mid = (ymin + ymax) * 0.5;
+            ymin = mid - xr * 0.5;
+            ymax = mid + xr * 0.5;
+        } else {
+            const double mid = (xmin + xmax) * 0.5;
+            xmin = mid - yr * 0.5;
+            xmax = mid + yr * 0.5;
+        }
+        box[0] = xmin;
+        box[1] = xmax;
+        box[2] = ymin;
+        box[3] = ymax;
+        const double w = xmax - xmin;
+        box[4] = w * w;  // root width_sq (BarnesHutCuda.cu:717)
+        uint32_t flags = 0;
+        if (!isfinite(xmin) || !isfinite(xmax) || !isfinite(ymin) || !isfinite(ymax))
+            flags |= FLAG_NONFINITE;
+        meta[META_FLAGS] = flags;
+    }
+}
+
+void launch_bbox(nb_ctx* c, int src) {
+    hipLaunchKernelGGL(k_bbox_partial, dim3(c->bbox_blocks), dim3(kBboxThreads), 0, c->stream,
+                       c->b.pos[src], (uint32_t)c->n, c->b.bbox_part);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, c->stream, c->b.bbox_part,
+                       c->bbox_blocks, c->b.box, c->b.meta);
+}
+
+// ---------------------------------------------------------------------------
+// Morton keys. HBM traffic: 16 B read + 12 B written per body.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t spread_bits(uint32_t v) {
+    uint64_t x = v;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+// floor(clamp(nrm * 2^32, 0, 2^32 - 1)); fmax maps NaN to 0.
+__device__ __forceinline__ uint32_t quantise(double v, double lo, double inv) {
+    const double nrm = (v - lo) * inv;
+    const double s = fmin(fmax(nrm * 4294967296.0, 0.0), 4294967295.0);
+    return (uint32_t)floor(s);
+}
+
+__global__ __launch_bounds__(256) void k_morton(const double2* __restrict__ pos, uint32_t n,
+                                                const double* __restrict__ box,
+                                                uint64_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double x_min = box[0], x_max = box[1], y_min = box[2], y_max = box[3];
+    const double inv_w = 1.0 / (x_max - x_min);
+    const double inv_h = 1.0 / (y_max - y_min);
+    const double2 p = pos[i];
+    const uint32_t ix = quantise(p.x, x_min, inv_w);
+    const uint32_t iy = quantise(p.y, y_min, inv_h);
+    keys[i] = (spread_bits(ix) << 1) | spread_bits(iy);
+    vals[i] = i;
+}
+
+void launch_morton(nb_ctx* c, int src) {
+    const uint32_t n = (uint32_t)c->n;
+    hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->b.pos[src], n,
+                       c->b.box, c->b.keys[0], c->b.vals[0]);
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort, 8-bit digits, 8 passes (keys ping-pong keys[0] -> keys[1]
+// -> ... -> keys[0]). Per pass:
+//   hist    per-tile digit histogram (wave-aggregated LDS adds)
+//   scan    one workgroup per digit scans that digit's column across tiles
+//   scatter stable rank inside the tile (8 ballots per key give the lanes of
+//           the wave holding the same digit), then a coalesced-by-digit store
+// HBM traffic per pass: 8 B (hist) + 12 B read + 12 B written per key.
+// ---------------------------------------------------------------------------
+
+// Lanes of this wave whose digit equals mine (valid lanes only).
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
+    uint64_t peers = ballot64(valid);
+#pragma unroll
+    for (int b = 0; b < kSortBits; b++) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = ballot64(valid && bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_rs_hist(const uint64_t* __restrict__ keys,
+                                                          uint32_t n, int shift,
+                                                          uint32_t* __restrict__ hist,
+                                                          uint32_t tiles) {
+    __shared__ uint32_t h[kSortBins];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int r = 0; r < kSortItems; r++) {
+        const uint64_t idx = base + (uint64_t)r * kSortThreads + threadIdx.x;
+        const bool valid = idx < n;
+        const uint32_t d = valid ? (uint32_t)(keys[idx] >> shift) & (kSortBins - 1) : 0u;
+        const uint64_t peers = digit_peers(d, valid);
+        if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_rs_scan(uint32_t* __restrict__ hist,
+                                                          uint32_t tiles,
+                                                          uint32_t* __restrict__ tot) {
+    __shared__ uint32_t s[kSortThreads / 64 + 1];
+    uint32_t* col = hist + (uint64_t)blockIdx.x * tiles;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < tiles; b0 += kSortThreads) {
+        const uint32_t k = b0 + threadIdx.x;
+        const uint32_t v = k < tiles ? col[k] : 0u;
+        uint32_t sum;
+        const uint32_t ex = block_excl_scan<kSortThreads>(v, s, &sum);
+        if (k < tiles) col[k] = carry + ex;
+        carry += sum;
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_rs_scatter(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, uint32_t n, int shift, const uint32_t* __restrict__ hist,
+    const uint32_t* __restrict__ tot, uint32_t tiles) {
+    constexpr int NW = kSortThreads / 64;
+    __shared__ uint32_t run[kSortBins];      // next output position of each digit
+    __shared__ uint32_t wcnt[NW][kSortBins]; // keys of each digit per wave, this round
+    __shared__ uint32_t s[NW + 1];
+    const int t = threadIdx.x, w = t >> 6;
+    {
+        uint32_t sum;
+        const uint32_t ex = block_excl_scan<kSortThreads>(tot[t], s, &sum);
+        run[t] = ex + hist[(uint64_t)t * tiles + blockIdx.x];
+#pragma unroll
+        for (int k = 0; k < NW; k++) wcnt[k][t] = 0;
+    }
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    uint64_t key[kSortItems];
+    uint32_t val[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const uint64_t idx = base + (uint64_t)r * kSortThreads + t;
+        key[r] = idx < n ? kin[idx] : 0ull;
+        val[r] = idx < n ? vin[idx] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const uint64_t idx = base + (uint64_t)r * kSortThreads + t;
+        const bool valid = idx < n;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & (kSortBins - 1);
+        const uint64_t peers = digit_peers(d, valid);
+        const uint64_t below = peers & lanemask_lt();
+        if (valid && below == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t dst = run[d] + (uint32_t)__popcll(below);
+            for (int k = 0; k < w; k++) dst += wcnt[k][d];
+            kout[dst] = key[r];
+            vout[dst] = val[r];
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            add += wcnt[k][t];
+            wcnt[k][t] = 0;
+        }
+        run[t] += add;
+        __syncthreads();
+    }
+}
+
+void launch_radix_sort(nb_ctx* c) {
+    const uint32_t n = (uint32_t)c->n;
+    const uint32_t tiles = c->sort_tiles;
+    for (int pass = 0; pass < 64 / kSortBits; pass++) {
+        const int shift = pass * kSortBits;
+        const int in = pass & 1, out = in ^ 1;
+        hipLaunchKernelGGL(k_rs_hist, dim3(tiles), dim3(kSortThreads), 0, c->stream,
+                           c->b.keys[in], n, shift, c->b.sort_hist, tiles);
+        hipLaunchKernelGGL(k_rs_scan, dim3(kSortBins), dim3(kSortThreads), 0, c->stream,
+                           c->b.sort_hist, tiles, c->b.sort_tot);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(tiles), dim3(kSortThreads), 0, c->stream,
+                           c->b.keys[in], c->b.vals[in], c->b.keys[out], c->b.vals[out], n, shift,
+                           c->b.sort_hist, c->b.sort_tot, tiles);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gather every body array through the permutation (BarnesHutCuda.cu:135-147).
+// HBM traffic: 4 B perm + 44 B read + 44 B written per body.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gather(
+    const uint32_t* __restrict__ perm, const double2* __restrict__ pin,
+    const double2* __restrict__ vin, const double* __restrict__ min_,
+    const uint32_t* __restrict__ oin, double2* __restrict__ pout, double2* __restrict__ vout,
+    double* __restrict__ mout, uint32_t* __restrict__ oout, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = perm[i];
+    pout[i] = pin[s];
+    vout[i] = vin[s];
+    mout[i] = min_[s];
+    oout[i] = oin[s];
+}
+
+void launch_gather(nb_ctx* c, int src) {
+    const uint32_t n = (uint32_t)c->n;
+    const int dst = src ^ 1;
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->b.vals[0],
+                       c->b.pos[src], c->b.vel[src], c->b.mass[src], c->b.orig[src],
+                       c->b.pos[dst], c->b.vel[dst], c->b.mass[dst], c->b.orig[dst], n);
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive in-place scan of u32 (reduce / top-level scan / downsweep).
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ data,
+                                                              uint32_t count,
+                                                              uint32_t* __restrict__ sums) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    uint32_t v = 0;
+#pragma unroll
+    for (int r = 0; r < kScanItems; r++) {
+        const uint64_t idx = base + (uint64_t)r * kScanThreads + threadIdx.x;
+        if (idx < count) v += data[idx];
+    }
+    v = wave_sum(v);
+    __shared__ uint32_t s[kScanThreads / 64];
+    if (lane_id() == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int k = 0; k < kScanThreads / 64; k++) t += s[k];
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* __restrict__ sums, uint32_t nb,
+                                                   uint32_t* __restrict__ total) {
+    __shared__ uint32_t s[1024 / 64 + 1];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+        const uint32_t k = b0 + threadIdx.x;
+        const uint32_t v = k < nb ? sums[k] : 0u;
+        uint32_t sum;
+        const uint32_t ex = block_excl_scan<1024>(v, s, &sum);
+        if (k < nb) sums[k] = carry + ex;
+        carry += sum;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict__ data,
+                                                            uint32_t count,
+                                                            const uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s[kScanThreads / 64 + 1];
+    // Blocked layout: thread t owns items t*kScanItems .. +kScanItems-1.
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int r = 0; r < kScanItems; r++) {
+        v[r] = (base + r) < count ? data[base + r] : 0u;
+        tsum += v[r];
+    }
+    uint32_t total;
+    uint32_t run = sums[blockIdx.x] + block_excl_scan<kScanThreads>(tsum, s, &total);
+#pragma unroll
+    for (int r = 0; r < kScanItems; r++) {
+        if ((base + r) < count) data[base + r] = run;
+        run += v[r];
+    }
+}
+
+void launch_scan_u32(nb_ctx* c, uint32_t* data, uint32_t count, uint32_t* total_out) {
+    const uint32_t nb = (count + kScanTile - 1) / kScanTile;
+    const uint32_t g = nb ? nb : 1;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(g), dim3(kScanThreads), 0, c->stream, data, count,
+                       c->b.scan_tmp);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, c->stream, c->b.scan_tmp, g,
+                       total_out);
+    hipLaunchKernelGGL(k_scan_down, dim3(g), dim3(kScanThreads), 0, c->stream, data, count,
+                       c->b.scan_tmp);
+}
+
+}  // namespace nb

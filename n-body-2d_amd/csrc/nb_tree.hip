@@ -1,0 +1,343 @@
+// nb_tree.hip — bottom-up quadtree build from sorted Morton keys, then a
+// level-synchronous mass / centre-of-mass / second-moment reduction.
+//
+// Same tree as the reference's level-by-level build (BarnesHutCuda.cu:153-240,
+// 713-777): a node at level L is a 2L-bit key prefix; it is internal when it
+// holds >= 2 bodies and L < 30, and an internal node materialises all four
+// child slots (empty ones included). The reference discovers the tree top-down
+// with one launch and one host round trip per level; here it is read directly
+// off the sorted keys:
+//   delta_i = number of leading equal base-4 digits of keys i and i+1 (<= 30)
+//   pair i owns the internal nodes at levels (delta_{i-1}, min(delta_i, 29)]
+//   (those are exactly the internal nodes whose first body is i)
+// so an exclusive scan of the per-pair counts numbers every internal node in
+// pre-order (sorted by first body, then level), and internal node p owns child
+// slots 1+4p..4+4p. No atomics, no host synchronisation, deterministic ids.
+#include "nb_device.h"
+#include "nb_internal.h"
+
+namespace nb {
+
+__device__ __forceinline__ int pair_delta(uint64_t a, uint64_t b) {
+    const uint64_t x = a ^ b;
+    if (x == 0) return kMaxLevel;
+    const int d = __clzll(x) >> 1;
+    return d < kMaxLevel ? d : kMaxLevel;
+}
+
+// Per pair: delta and the number of internal nodes it owns.
+// HBM traffic: 8 B keys read (x3 from cache), 5 B written per pair.
+__global__ __launch_bounds__(256) void k_tree_delta(const uint64_t* __restrict__ keys, uint32_t n,
+                                                    int8_t* __restrict__ delta,
+                                                    uint32_t* __restrict__ cnt) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i + 1 >= n) return;
+    const uint64_t k0 = keys[i], k1 = keys[i + 1];
+    const int d = pair_delta(k0, k1);
+    const int prev = i ? pair_delta(keys[i - 1], k0) : -1;
+    const int hi = d < kMaxLevel - 1 ? d : kMaxLevel - 1;
+    delta[i] = (int8_t)d;
+    cnt[i] = hi > prev ? (uint32_t)(hi - prev) : 0u;
+}
+
+__device__ __forceinline__ void write_empty(NodeRec* r, double wsq) {
+    r->cx = 0.0;
+    r->cy = 0.0;
+    r->m = 0.0;
+    r->wsq = wsq;
+    r->c0 = -2;
+    r->mask = 0;
+    r->bs = -1;
+    r->be = -1;
+    r->ixx = 0.f;
+    r->ixy = 0.f;
+    r->iyy = 0.f;
+    r->mf = 0.f;
+}
+
+// Builds internal node p = (first body a, level L): its body range, the four
+// child ranges (binary searches on the digit, lower_bound_quadrant of
+// BarnesHutCuda.cu:153-163) and the child slots. Leaves get their COM here;
+// multi-body leaves (only possible at level 30) are queued for k_leaf_multi.
+__device__ void build_node(uint32_t p, uint32_t a, int L, const uint64_t* __restrict__ keys,
+                           uint32_t n, const double2* __restrict__ pos,
+                           const double* __restrict__ mass, const int8_t* __restrict__ delta,
+                           const uint32_t* __restrict__ off, double root_wsq,
+                           NodeRec* __restrict__ nodes, uint32_t* __restrict__ inode_slot,
+                           uint8_t* __restrict__ inode_level, uint8_t* __restrict__ inode_mask,
+                           uint32_t* __restrict__ multi_list, uint32_t* __restrict__ meta) {
+    uint32_t b;
+    if (L == 0) {
+        b = n - 1;
+    } else {
+        const int sh = 64 - 2 * L;
+        const uint64_t P = keys[a] >> sh;
+        uint32_t lo = a + 1, step = 1;  // keys[a+1] shares the prefix (delta_a >= L)
+        for (;;) {
+            const uint64_t probe = (uint64_t)lo + step;
+            if (probe >= n || (keys[probe] >> sh) != P) break;
+            lo = (uint32_t)probe;
+            step <<= 1;
+        }
+        uint32_t hi = (uint64_t)lo + step < n ? lo + step : n;
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if ((keys[mid] >> sh) == P)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        b = lo;
+    }
+    const int s = 62 - 2 * L;
+    uint32_t bq[5];
+    bq[0] = a;
+    bq[4] = b + 1;
+#pragma unroll
+    for (int q = 1; q < 4; q++) {
+        uint32_t lo = bq[q - 1], hi = b + 1;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (((keys[mid] >> s) & 3ull) < (uint64_t)q)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        bq[q] = lo;
+    }
+    const double cwsq = ldexp(root_wsq, -2 * (L + 1));
+    uint32_t mask = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint32_t cs = bq[q], ce1 = bq[q + 1];
+        NodeRec* r = &nodes[1 + 4ull * p + q];
+        if (cs >= ce1) {
+            write_empty(r, cwsq);
+            continue;
+        }
+        mask |= 1u << q;
+        const uint32_t ce = ce1 - 1;
+        if (cs == ce || L + 1 == kMaxLevel) {
+            r->wsq = cwsq;
+            r->c0 = -1;
+            r->mask = 0;
+            r->bs = (int32_t)cs;
+            r->be = (int32_t)ce;
+            if (cs == ce) {
+                const double2 x = pos[cs];
+                const double m = mass[cs];
+                r->cx = x.x;
+                r->cy = x.y;
+                r->m = m;
+                r->mf = (float)m;
+                r->ixx = 0.f;
+                r->ixy = 0.f;
+                r->iyy = 0.f;
+            } else {
+                const uint32_t k = atomicAdd(&meta[META_MULTI_LEAVES], 1u);
+                multi_list[k] = (uint32_t)(1 + 4ull * p + q);
+            }
+        } else {
+            const int prev = cs ? (int)delta[cs - 1] : -1;
+            const uint32_t cid = off[cs] + (uint32_t)(L + 1 - prev - 1);
+            r->wsq = cwsq;
+            r->c0 = (int32_t)(1 + 4ull * cid);
+            r->bs = (int32_t)cs;
+            r->be = (int32_t)ce;
+            inode_slot[cid] = (uint32_t)(1 + 4ull * p + q);
+        }
+    }
+    inode_mask[p] = (uint8_t)mask;
+    inode_level[p] = (uint8_t)L;
+    if (p == 0) {
+        NodeRec* r = &nodes[0];
+        r->wsq = root_wsq;
+        r->c0 = 1;
+        r->bs = 0;
+        r->be = (int32_t)(n - 1);
+        inode_slot[0] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tree_nodes(
+    const uint64_t* __restrict__ keys, uint32_t n, const double2* __restrict__ pos,
+    const double* __restrict__ mass, const int8_t* __restrict__ delta,
+    const uint32_t* __restrict__ off, const double* __restrict__ box, uint32_t* __restrict__ meta,
+    uint64_t icap, NodeRec* __restrict__ nodes, uint32_t* __restrict__ inode_slot,
+    uint8_t* __restrict__ inode_level, uint8_t* __restrict__ inode_mask,
+    uint32_t* __restrict__ multi_list) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t I = meta[META_INTERNAL];
+    if (I > icap || (meta[META_FLAGS] & FLAG_NONFINITE)) {
+        if (i == 0) meta[META_FLAGS] |= (I > icap ? FLAG_TREE_CAPACITY : 0u);
+        return;
+    }
+    const double root_wsq = box[4];
+    if (n == 1) {
+        if (i == 0) {
+            NodeRec* r = &nodes[0];
+            const double2 x = pos[0];
+            r->cx = x.x;
+            r->cy = x.y;
+            r->m = mass[0];
+            r->mf = (float)mass[0];
+            r->wsq = root_wsq;
+            r->c0 = -1;
+            r->mask = 0;
+            r->bs = 0;
+            r->be = 0;
+            r->ixx = r->ixy = r->iyy = 0.f;
+        }
+        return;
+    }
+    if (i + 1 >= n) return;
+    const int prev = i ? (int)delta[i - 1] : -1;
+    const int d = delta[i];
+    const int hi = d < kMaxLevel - 1 ? d : kMaxLevel - 1;
+    for (int L = prev + 1; L <= hi; L++) {
+        const uint32_t p = off[i] + (uint32_t)(L - prev - 1);
+        build_node(p, i, L, keys, n, pos, mass, delta, off, root_wsq, nodes, inode_slot,
+                   inode_level, inode_mask, multi_list, meta);
+    }
+}
+
+// Multi-body leaf COM, one wave per leaf, in the order of compute_leaf_com
+// (BarnesHutCuda.cu:277-309): 128 strided partial sums (lane l holds slots l
+// and l+64), then the halving tree reduction. Second moments about the COM.
+__global__ __launch_bounds__(256) void k_leaf_multi(const double2* __restrict__ pos,
+                                                    const double* __restrict__ mass,
+                                                    const uint32_t* __restrict__ multi_list,
+                                                    const uint32_t* __restrict__ meta,
+                                                    NodeRec* __restrict__ nodes) {
+    const uint32_t count = meta[META_MULTI_LEAVES];
+    const uint32_t waves = gridDim.x * 4;
+    const int l = lane_id();
+    for (uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6); k < count; k += waves) {
+        NodeRec* r = &nodes[multi_list[k]];
+        const int32_t bs = r->bs, be = r->be;
+        double M[2] = {0.0, 0.0}, X[2] = {0.0, 0.0}, Y[2] = {0.0, 0.0};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            for (int32_t i = bs + l + 64 * h; i <= be; i += 128) {
+                const double m = mass[i];
+                const double2 x = pos[i];
+                M[h] += m;
+                X[h] = fma(m, x.x, X[h]);
+                Y[h] = fma(m, x.y, Y[h]);
+            }
+        }
+        double m = M[0] + M[1], x = X[0] + X[1], y = Y[0] + Y[1];  // stride 64
+#pragma unroll
+        for (int st = 32; st >= 1; st >>= 1) {
+            const double om = __shfl_down(m, st, 64), ox = __shfl_down(x, st, 64),
+                         oy = __shfl_down(y, st, 64);
+            if (l < st) {
+                m += om;
+                x += ox;
+                y += oy;
+            }
+        }
+        m = __shfl(m, 0, 64);
+        const double cx = __shfl(x, 0, 64) / m, cy = __shfl(y, 0, 64) / m;
+        double a = 0.0, b = 0.0, c = 0.0;
+        for (int32_t i = bs + l; i <= be; i += 64) {
+            const double2 p = pos[i];
+            const double w = mass[i], dx = p.x - cx, dy = p.y - cy;
+            a += w * dx * dx;
+            b += w * dx * dy;
+            c += w * dy * dy;
+        }
+#pragma unroll
+        for (int st = 32; st >= 1; st >>= 1) {
+            a += __shfl_xor(a, st, 64);
+            b += __shfl_xor(b, st, 64);
+            c += __shfl_xor(c, st, 64);
+        }
+        if (l == 0) {
+            r->cx = cx;
+            r->cy = cy;
+            r->m = m;
+            r->mf = (float)m;
+            r->ixx = (float)a;
+            r->ixy = (float)b;
+            r->iyy = (float)c;
+        }
+    }
+}
+
+// Internal nodes of one level: children in slot order q = 0..3, M += m,
+// R = fma(m, com, R) (propagate_com_up, BarnesHutCuda.cu:315-344, with the
+// multiply-add nvcc contracts), com = R / M; second moments by the parallel
+// axis theorem. HBM traffic: 4 child slots read, 1 slot written per node.
+__global__ __launch_bounds__(256) void k_tree_com(int L, const uint32_t* __restrict__ meta,
+                                                  uint64_t icap,
+                                                  const uint8_t* __restrict__ inode_level,
+                                                  const uint8_t* __restrict__ inode_mask,
+                                                  const uint32_t* __restrict__ inode_slot,
+                                                  NodeRec* __restrict__ nodes) {
+    const uint32_t I = meta[META_INTERNAL];
+    if (I > icap || (meta[META_FLAGS] & (FLAG_TREE_CAPACITY | FLAG_NONFINITE))) return;
+    for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < I; p += gridDim.x * 256) {
+        if (inode_level[p] != L) continue;
+        const uint32_t mask = inode_mask[p];
+        const NodeRec* ch = &nodes[1 + 4ull * p];
+        double M = 0.0, Rx = 0.0, Ry = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (!((mask >> q) & 1u)) continue;
+            const double m = ch[q].m;
+            M += m;
+            Rx = fma(m, ch[q].cx, Rx);
+            Ry = fma(m, ch[q].cy, Ry);
+        }
+        const double cx = Rx / M, cy = Ry / M;
+        double a = 0.0, b = 0.0, c = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (!((mask >> q) & 1u)) continue;
+            const double m = ch[q].m, dx = ch[q].cx - cx, dy = ch[q].cy - cy;
+            a += (double)ch[q].ixx + m * dx * dx;
+            b += (double)ch[q].ixy + m * dx * dy;
+            c += (double)ch[q].iyy + m * dy * dy;
+        }
+        NodeRec* r = &nodes[inode_slot[p]];
+        r->cx = cx;
+        r->cy = cy;
+        r->m = M;
+        r->mf = (float)M;
+        r->mask = (int32_t)mask;
+        r->ixx = (float)a;
+        r->ixy = (float)b;
+        r->iyy = (float)c;
+    }
+}
+
+__global__ void k_tree_reset(uint32_t* __restrict__ meta) {
+    if (threadIdx.x == 0) meta[META_MULTI_LEAVES] = 0;
+}
+
+void launch_tree_build(nb_ctx* c, int src) {
+    const uint32_t n = (uint32_t)c->n;
+    const uint32_t pairs = n > 1 ? n - 1 : 0;
+    const uint32_t pg = pairs ? (pairs + 255) / 256 : 1;
+    hipLaunchKernelGGL(k_tree_reset, dim3(1), dim3(64), 0, c->stream, c->b.meta);
+    if (pairs) {
+        hipLaunchKernelGGL(k_tree_delta, dim3(pg), dim3(256), 0, c->stream, c->b.keys[0], n,
+                           c->b.delta, c->b.cnt);
+    }
+    launch_scan_u32(c, c->b.cnt, pairs, &c->b.meta[META_INTERNAL]);
+    hipLaunchKernelGGL(k_tree_nodes, dim3(pg), dim3(256), 0, c->stream, c->b.keys[0], n,
+                       c->b.pos[src], c->b.mass[src], c->b.delta, c->b.cnt, c->b.box, c->b.meta,
+                       (uint64_t)c->icap, c->b.nodes, c->b.inode_slot, c->b.inode_level,
+                       c->b.inode_mask, c->b.multi_list);
+    hipLaunchKernelGGL(k_leaf_multi, dim3(64), dim3(256), 0, c->stream, c->b.pos[src],
+                       c->b.mass[src], c->b.multi_list, c->b.meta, c->b.nodes);
+    const uint32_t cg = (uint32_t)((c->icap + 255) / 256 < 2048 ? (c->icap + 255) / 256 : 2048);
+    for (int L = kMaxLevel - 1; L >= 0; L--) {
+        hipLaunchKernelGGL(k_tree_com, dim3(cg ? cg : 1), dim3(256), 0, c->stream, L, c->b.meta,
+                           (uint64_t)c->icap, c->b.inode_level, c->b.inode_mask,
+                           c->b.inode_slot, c->b.nodes);
+    }
+}
+
+}  // namespace nb

@@ -1,0 +1,210 @@
+"""ctypes binding of the C ABI declared in include/nbody.h.
+
+Loads ``libnbody2d.so`` from this directory and fails loudly when it is
+missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "libnbody2d.so"
+
+NB_OK = 0
+NB_ERR_INVALID_ARG = 1
+NB_ERR_HIP = 2
+NB_ERR_OUT_OF_MEMORY = 3
+NB_ERR_OVERFLOW = 4
+NB_ERR_NOT_SUPPORTED = 5
+NB_ERR_NO_DEVICE = 6
+NB_ERR_NUMERIC = 7
+
+ALGO_BARNES_HUT = 0
+ALGO_ALL_PAIRS = 1
+INTEG_KD = 0
+INTEG_DK = 1
+FAR_MONOPOLE = 0
+FAR_LEARNED = 1
+PHASES = ("bbox_morton", "sort", "build", "force", "drift")
+
+# Every symbol include/nbody.h declares (checked by tests/test_abi_symbols.py).
+EXPORTS = (
+    "nb_params_default", "nb_create", "nb_step", "nb_get_state", "nb_compute_forces",
+    "nb_set_dt", "nb_get_info", "nb_set_timing", "nb_get_timing", "nb_destroy",
+    "nb_last_error", "nb_status_string", "nb_debug_build", "nb_debug_tree",
+    "nb_direct_sum_f64",
+)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("G", C.c_double), ("dt", C.c_double), ("theta", C.c_double), ("softening", C.c_double),
+        ("algorithm", C.c_int32), ("integrator", C.c_int32), ("self_guard", C.c_int32),
+        ("far_field", C.c_int32), ("use_graph", C.c_int32), ("reserved", C.c_int32 * 7),
+    ]
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64), ("iteration", C.c_uint64), ("simulated_time", C.c_double),
+        ("tree_nodes", C.c_uint64), ("internal_nodes", C.c_uint64),
+        ("node_capacity", C.c_uint64), ("device_bytes", C.c_uint64),
+    ]
+
+
+NODE_DTYPE = np.dtype([
+    ("level", "<i4"), ("bstart", "<i4"), ("bend", "<i4"), ("c0", "<i4"),
+    ("mass", "<f8"), ("cx", "<f8"), ("cy", "<f8"), ("wsq", "<f8"),
+    ("ixx", "<f4"), ("ixy", "<f4"), ("iyy", "<f4"), ("child_mask", "<i4"),
+])
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            raise RuntimeError(f"{_LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(str(_LIB_PATH))
+        P = C.c_void_p
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        L.nb_params_default.argtypes = [C.POINTER(Params)]
+        L.nb_create.argtypes = [C.POINTER(Params), C.c_uint64, P, P, P, C.POINTER(C.c_void_p)]
+        L.nb_step.argtypes = [P, C.c_uint64]
+        L.nb_get_state.argtypes = [P, P, P]
+        L.nb_compute_forces.argtypes = [P, P, P, C.POINTER(C.c_uint64)]
+        L.nb_set_dt.argtypes = [P, C.c_double]
+        L.nb_get_info.argtypes = [P, C.POINTER(Info)]
+        L.nb_set_timing.argtypes = [P, C.c_int]
+        L.nb_get_timing.argtypes = [P, P, C.c_int, C.POINTER(C.c_uint64)]
+        L.nb_destroy.argtypes = [P]
+        L.nb_destroy.restype = None
+        L.nb_last_error.argtypes = [P]
+        L.nb_last_error.restype = C.c_char_p
+        L.nb_status_string.argtypes = [C.c_int]
+        L.nb_status_string.restype = C.c_char_p
+        L.nb_debug_build.argtypes = [P, P, P, P, C.POINTER(C.c_uint64)]
+        L.nb_debug_tree.argtypes = [P, P, C.c_uint64]
+        L.nb_direct_sum_f64.argtypes = [C.c_uint64, P, P, C.c_double, C.c_double, C.c_uint64,
+                                        P, P]
+        del dp
+        _lib = L
+    return _lib
+
+
+class NBodyError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{msg} (status {status})")
+        self.status = status
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def params(**kw) -> Params:
+    p = Params()
+    lib().nb_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Context:
+    """Owns one nb_ctx (bodies resident on the GPU)."""
+
+    def __init__(self, mass, pos, vel, **kw):
+        self.n = int(len(mass))
+        m = np.ascontiguousarray(mass, dtype=np.float64)
+        x = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        v = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1)
+        if x.size != 2 * self.n or v.size != 2 * self.n:
+            raise ValueError("pos/vel must be (n, 2)")
+        self.params = params(**kw)
+        h = C.c_void_p()
+        L = lib()
+        st = L.nb_create(C.byref(self.params), self.n, _ptr(m), _ptr(x), _ptr(v), C.byref(h))
+        if st != NB_OK:
+            raise NBodyError(st, L.nb_last_error(None).decode())
+        self._h = h
+
+    def _check(self, st: int) -> None:
+        if st != NB_OK:
+            raise NBodyError(st, lib().nb_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().nb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, k: int = 1) -> None:
+        self._check(lib().nb_step(self._h, int(k)))
+
+    def state(self):
+        pos = np.empty((self.n, 2))
+        vel = np.empty((self.n, 2))
+        self._check(lib().nb_get_state(self._h, _ptr(pos), _ptr(vel)))
+        return pos, vel
+
+    def forces(self):
+        acc = np.empty((self.n, 2))
+        inter = np.empty(self.n, dtype=np.uint32)
+        tot = C.c_uint64()
+        self._check(lib().nb_compute_forces(self._h, _ptr(acc), _ptr(inter), C.byref(tot)))
+        return acc, inter, int(tot.value)
+
+    def set_dt(self, dt: float) -> None:
+        self._check(lib().nb_set_dt(self._h, float(dt)))
+
+    def info(self) -> Info:
+        i = Info()
+        self._check(lib().nb_get_info(self._h, C.byref(i)))
+        return i
+
+    def set_timing(self, on: bool) -> None:
+        self._check(lib().nb_set_timing(self._h, int(bool(on))))
+
+    def timing(self):
+        ms = np.zeros(len(PHASES))
+        steps = C.c_uint64()
+        self._check(lib().nb_get_timing(self._h, _ptr(ms), len(PHASES), C.byref(steps)))
+        return dict(zip(PHASES, ms.tolist())), int(steps.value)
+
+    def debug_build(self):
+        box = np.empty(4)
+        keys = np.empty(self.n, dtype=np.uint64)
+        orig = np.empty(self.n, dtype=np.uint32)
+        cnt = C.c_uint64()
+        self._check(lib().nb_debug_build(self._h, _ptr(box), _ptr(keys), _ptr(orig),
+                                         C.byref(cnt)))
+        return box, keys, orig, int(cnt.value)
+
+    def debug_tree(self, count: int):
+        nodes = np.zeros(count, dtype=NODE_DTYPE)
+        self._check(lib().nb_debug_tree(self._h, _ptr(nodes), count))
+        return nodes
+
+
+def direct_sum_f64(pos, mass, G: float, softening: float, targets) -> np.ndarray:
+    x = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    t = np.ascontiguousarray(targets, dtype=np.int64)
+    acc = np.empty((t.size, 2))
+    L = lib()
+    st = L.nb_direct_sum_f64(m.size, _ptr(x), _ptr(m), float(G), float(softening), t.size,
+                             _ptr(t), _ptr(acc))
+    if st != NB_OK:
+        raise NBodyError(st, L.nb_last_error(None).decode())
+    return acc

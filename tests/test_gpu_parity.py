@@ -1,0 +1,236 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bit-exact: squared box, Morton keys, sort permutation, tree topology
+(canonical (level, body range, child occupancy) sets), node mass and COM.
+Interaction sets: every body's count of force terms equals the oracle's
+FP64 per-body DFS (the opening test is evaluated in FP64 on both sides).
+Tolerance: accelerations computed in FP32 with Kahan sums, relative error per
+body against the oracle's FP64 sum over the same terms, RMS <= 2e-6 and
+max <= 5e-5.
+"""
+import numpy as np
+import pytest
+
+import nbody2d
+import oracle_lib as O
+from nbody2d import ics
+
+pytestmark = pytest.mark.gpu
+
+RMS_TOL = 2e-6
+MAX_TOL = 5e-5
+
+
+def _cases():
+    out = {}
+    out["disk4k"] = ics.exp_disk(4096, seed=1)
+    out["uniform3k"] = ics.uniform_disk(3001, seed=2)
+    m, p, v = ics.uniform_disk(257, seed=3)
+    p[100:140] = p[5]  # coincident bodies -> multi-body leaves at level 30
+    out["dups"] = (m, p, v)
+    out["two"] = (np.array([1.0, 3.0]), np.array([[0.0, 0.0], [1.0, 0.5]]), np.zeros((2, 2)))
+    out["ragged65"] = ics.exp_disk(65, seed=4)
+    return out
+
+
+CASES = _cases()
+
+
+def _ctx(m, p, v, **kw):
+    print(f"[gpu] create n={len(m)} {kw}", flush=True)
+    return nbody2d.Context(m, p, v, **kw)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_morton_sort_box_bit_exact(name):
+    m, p, v = CASES[name]
+    with _ctx(m, p, v) as c:
+        box, skeys, orig, cnt = c.debug_build()
+    rbox, keys, perm, _ = O.prepare(p, m)
+    assert np.array_equal(box, rbox)
+    assert np.array_equal(skeys, keys[perm])
+    assert np.array_equal(orig, perm)
+
+
+def _canon(level, bs, be, mask):
+    return sorted(zip(level.tolist(), bs.tolist(), be.tolist(), mask.tolist()))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_tree_topology_and_com(name):
+    m, p, v = CASES[name]
+    with _ctx(m, p, v) as c:
+        box, skeys, orig, cnt = c.debug_build()
+        g = c.debug_tree(cnt)
+    rbox, keys, perm, root_wsq = O.prepare(p, m)
+    r = O.tree_build(keys[perm], p[perm], m[perm], root_wsq)
+    assert len(g) == len(r)
+    # child occupancy mask of the oracle's internal nodes
+    rmask = np.zeros(len(r), dtype=np.int64)
+    internal = r["c0"] >= 0
+    for q in range(4):
+        rmask[internal] |= (r["bstart"][r["c0"][internal] + q] >= 0).astype(np.int64) << q
+    gmask = np.where(g["c0"] >= 1, g["child_mask"], 0)
+    assert _canon(g["level"], g["bstart"], g["bend"], gmask) == _canon(
+        r["level"], r["bstart"], r["bend"], rmask)
+    # mass / COM / wsq bit-exact on non-empty nodes, matched by (level, range)
+    gk = {(a, b, c): i for i, (a, b, c) in enumerate(zip(g["level"], g["bstart"], g["bend"]))
+          if b >= 0}
+    idx = np.array([gk[(a, b, c)] for a, b, c in zip(r["level"], r["bstart"], r["bend"]) if b >= 0])
+    rn = r[r["bstart"] >= 0]
+    gn = g[idx]
+    assert np.array_equal(gn["mass"], rn["mass"])
+    assert np.array_equal(gn["cx"], rn["cx"])
+    assert np.array_equal(gn["cy"], rn["cy"])
+    assert np.array_equal(gn["wsq"], rn["wsq"])
+    scale = np.maximum(rn["ixx"] + rn["iyy"], 1e-300)
+    for f in ("ixx", "ixy", "iyy"):
+        assert np.all(np.abs(gn[f] - rn[f]) <= 1e-5 * scale + 1e-30)
+
+
+@pytest.mark.parametrize("name", ["disk4k", "uniform3k", "dups", "two", "ragged65"])
+@pytest.mark.parametrize("theta,eps,guard", [(0.5, 0.01, 0), (0.0, 0.01, 0), (0.8, 0.01, 0),
+                                             (0.8, 0.01, 1), (0.3, 0.0, 0)])
+def test_forces_match_oracle(name, theta, eps, guard):
+    m, p, v = CASES[name]
+    if eps == 0.0 and name == "dups":
+        pytest.skip("coincident bodies with zero softening are singular")
+    with _ctx(m, p, v, theta=theta, softening=eps, self_guard=guard, G=1.0) as c:
+        acc, inter, total = c.forces()
+    ref, rinter = O.bh_forces(p, m, theta, eps, guard)
+    assert np.array_equal(inter.astype(np.uint64), rinter)
+    assert total == int(rinter.sum())
+    rms, mx = ics.force_errors(acc, ref)
+    assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
+
+
+def test_theta0_is_direct_sum():
+    m, p, v = CASES["disk4k"]
+    with _ctx(m, p, v, theta=0.0, softening=0.01) as c:
+        acc, inter, total = c.forces()
+    assert total == len(m) * (len(m) - 1)
+    ref = O.direct_sum(p, m, 0.01 ** 2)
+    rms, mx = ics.force_errors(acc, ref)
+    assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
+
+
+def test_allpairs_matches_direct_sum():
+    m, p, v = ics.uniform_disk(5000, seed=9)
+    with _ctx(m, p, v, algorithm=nbody2d.abi.ALGO_ALL_PAIRS, softening=0.01, G=2.0) as c:
+        acc, inter, total = c.forces()
+    ref = 2.0 * O.direct_sum(p, m, 0.01 ** 2)
+    rms, mx = ics.force_errors(acc, ref)
+    assert rms <= RMS_TOL and mx <= MAX_TOL, (rms, mx)
+    assert total == 5000 * 4999
+
+
+def test_direct_sum_f64_matches_oracle():
+    m, p, v = ics.exp_disk(3000, seed=11)
+    t = ics.sample_targets(3000, 500, seed=1)
+    acc = nbody2d.direct_sum_f64(p, m, 1.5, 0.02, t)
+    ref = 1.5 * O.direct_sum(p, m, 0.02 ** 2, t)
+    rms, mx = ics.force_errors(acc, ref)
+    assert mx <= 1e-12, (rms, mx)
+
+
+def test_trajectory_matches_oracle_model():
+    m, p, v = ics.exp_disk(2048, seed=5)
+    G, dt, theta, eps = 1.0, 1e-3, 0.5, 0.01
+    with _ctx(m, p, v, G=G, dt=dt, theta=theta, softening=eps) as c:
+        c.step(10)
+        gp, gv = c.state()
+        info = c.info()
+    assert info.iteration == 10
+    op, ov, om, oo = O.model_step(p, v, m, np.arange(len(m)), G, dt, theta, eps * eps, steps=10)
+    rp = np.empty_like(op)
+    rv = np.empty_like(ov)
+    rp[oo] = op
+    rv[oo] = ov
+    # FP32 forces (relative error ~1e-7 of |a|) over 10 kicks of dt = 1e-3
+    assert np.max(np.abs(gp - rp)) <= 1e-8
+    assert np.max(np.abs(gv - rv) / (np.abs(rv) + 1.0)) <= 1e-5
+
+
+def test_graph_eager_timing_identical():
+    m, p, v = ics.exp_disk(3000, seed=6)
+    res = []
+    for kw in ({"use_graph": 1}, {"use_graph": 0}):
+        with _ctx(m, p, v, **kw) as c:
+            c.step(3)
+            res.append(c.state())
+    with _ctx(m, p, v) as c:
+        c.set_timing(True)
+        c.step(3)
+        tm, steps = c.timing()
+        res.append(c.state())
+    assert steps == 3 and tm["force"] > 0.0
+    for r in res[1:]:
+        assert np.array_equal(r[0], res[0][0]) and np.array_equal(r[1], res[0][1])
+
+
+def test_dk_integrator_and_set_dt():
+    m, p, v = ics.exp_disk(1500, seed=8)
+    with _ctx(m, p, v, integrator=nbody2d.abi.INTEG_DK, dt=1e-3) as c:
+        c.step(2)
+        c.set_dt(2e-3)
+        c.step(1)
+        gp, gv = c.state()
+        info = c.info()
+    assert info.iteration == 3
+    assert abs(info.simulated_time - 4e-3) < 1e-15
+    # reference: drift then kick with the Barnes-Hut accelerations at the drifted state
+    x, u = p.copy(), v.copy()
+    for dt in (1e-3, 1e-3, 2e-3):
+        x = x + u * dt
+        a, _ = O.bh_forces(x, m, 0.5, 0.01)
+        u = u + a * (1.0 * dt)
+    assert np.max(np.abs(gp - x)) <= 1e-8
+    assert np.max(np.abs(gv - u) / (np.abs(u) + 1.0)) <= 1e-5
+
+
+def test_allpairs_step_dk_matches_reference_order():
+    m, p, v = ics.uniform_disk(1024, seed=12)
+    v = np.random.default_rng(0).normal(0, 0.1, size=p.shape)
+    with _ctx(m, p, v, algorithm=nbody2d.abi.ALGO_ALL_PAIRS, integrator=nbody2d.abi.INTEG_DK,
+              dt=1e-3, softening=0.05) as c:
+        c.step(4)
+        gp, gv = c.state()
+    x, u = p.copy(), v.copy()
+    for _ in range(4):
+        x = x + u * 1e-3
+        u = u + O.direct_sum(x, m, 0.05 ** 2) * 1e-3
+    assert np.max(np.abs(gp - x)) <= 1e-10
+    assert np.max(np.abs(gv - u)) <= 1e-7
+
+
+def test_edge_sizes():
+    with _ctx(np.zeros(0), np.zeros((0, 2)), np.zeros((0, 2))) as c:
+        c.step(3)
+        assert c.info().iteration == 3
+    m, p, v = np.array([2.0]), np.array([[1.0, 2.0]]), np.array([[0.5, -1.0]])
+    with _ctx(m, p, v, dt=0.1) as c:
+        acc, inter, total = c.forces()
+        assert np.all(acc == 0.0) and total == 0
+        c.step(2)
+        gp, gv = c.state()
+    assert np.allclose(gp, [[1.1, 1.8]], rtol=0, atol=1e-15)
+
+
+def test_full_size_properties():
+    """N = 1,048,576 disk: sortedness, permutation, mass conservation of the
+    root, leaf coverage, and the force error against the FP64 direct sum."""
+    m, p, v = ics.exp_disk(1 << 20, seed=3)
+    with _ctx(m, p, v, theta=0.5, softening=0.01) as c:
+        box, skeys, orig, cnt = c.debug_build()
+        g = c.debug_tree(cnt)
+        acc, inter, total = c.forces()
+    assert np.all(skeys[1:] >= skeys[:-1])
+    assert np.array_equal(np.sort(orig), np.arange(len(m), dtype=np.uint32))
+    assert abs(g["mass"][0] - m.sum()) <= 1e-12 * m.sum()
+    leaves = g[g["c0"] == -1]
+    assert int((leaves["bend"] - leaves["bstart"] + 1).sum()) == len(m)
+    t = ics.sample_targets(len(m), 1024, seed=2)
+    ref = nbody2d.direct_sum_f64(p, m, 1.0, 0.01, t)
+    rms, mx = ics.force_errors(acc[t], ref)
+    print(f"[gpu] N=1M theta=0.5 rms={rms:.3e} max={mx:.3e} inter/body={total/len(m):.1f}")
+    assert rms <= 5e-3 and mx <= 5e-2
