@@ -1,0 +1,70 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+include/nbody.h declares, and rejects invalid arguments before touching a
+device."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from nbody2d import abi
+
+REPO = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    txt = (REPO / "include" / "nbody.h").read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(nb_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = abi.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(abi.EXPORTS)
+
+
+def test_struct_layouts_match_header():
+    assert C.sizeof(abi.Params) == 4 * 8 + 12 * 4
+    assert C.sizeof(abi.Info) == 7 * 8
+    assert abi.NODE_DTYPE.itemsize == 4 * 4 + 4 * 8 + 4 * 4
+
+
+def test_params_default():
+    p = abi.params()
+    assert (p.G, p.dt, p.theta, p.softening) == (1.0, 1e-3, 0.5, 0.01)
+    assert p.algorithm == abi.ALGO_BARNES_HUT and p.integrator == abi.INTEG_KD
+    assert p.use_graph == 1
+
+
+@pytest.mark.parametrize("field,value", [("dt", 0.0), ("dt", float("nan")), ("theta", -1.0),
+                                         ("softening", -0.1), ("G", float("inf")),
+                                         ("algorithm", 7), ("integrator", 3), ("far_field", 9)])
+def test_create_rejects_invalid_params(field, value):
+    m = np.ones(4)
+    x = np.zeros((4, 2))
+    with pytest.raises(abi.NBodyError) as e:
+        abi.Context(m, x, x, **{field: value})
+    assert e.value.status == abi.NB_ERR_INVALID_ARG
+
+
+def test_learned_far_field_not_supported_yet_is_an_error():
+    m = np.ones(4)
+    x = np.zeros((4, 2))
+    with pytest.raises(abi.NBodyError) as e:
+        abi.Context(m, x, x, far_field=abi.FAR_LEARNED)
+    assert e.value.status == abi.NB_ERR_NOT_SUPPORTED
+
+
+def test_null_arrays_and_null_ctx_are_rejected():
+    L = abi.lib()
+    p = abi.params()
+    h = C.c_void_p()
+    assert L.nb_create(C.byref(p), 5, None, None, None, C.byref(h)) == abi.NB_ERR_INVALID_ARG
+    assert L.nb_step(None, 1) == abi.NB_ERR_INVALID_ARG
+    assert L.nb_get_state(None, None, None) == abi.NB_ERR_INVALID_ARG
+    assert L.nb_status_string(abi.NB_ERR_OVERFLOW) == b"traversal stack overflow"

@@ -1,0 +1,327 @@
+"""Pins of the CPU oracle (runs without a GPU).
+
+The reference ships no tests, golden vectors or known-answer tests for this
+path (and its C++ sources need SFML/fmt/csv-parser headers absent here, so it
+is not compiled as an oracle). Each oracle function is pinned instead by
+closed forms, invariants, library routines and brute force, chosen so a
+dropped term, a wrong sign/index or a transposed operand fails.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from nbody2d import ics
+
+L = O.load()
+
+
+# ---------------- Morton / box / sort ----------------
+
+def spread_masks(v):
+    x = np.uint64(v)
+    for s, m in ((16, 0x0000FFFF0000FFFF), (8, 0x00FF00FF00FF00FF), (4, 0x0F0F0F0F0F0F0F0F),
+                 (2, 0x3333333333333333), (1, 0x5555555555555555)):
+        x = (x | (x << np.uint64(s))) & np.uint64(m)
+    return int(x)
+
+
+def test_spread_bits_matches_mask_form():
+    rng = np.random.default_rng(0)
+    vals = [0, 1, 2, 3, 0xFFFFFFFF, 0x80000000, 0x55555555] + list(
+        rng.integers(0, 2**32, 2000, dtype=np.uint64))
+    for v in vals:
+        assert L.or_spread_bits(int(v)) == spread_masks(int(v))
+
+
+def test_morton_known_answers():
+    box = np.array([0.0, 1.0, 0.0, 1.0])
+    bp = O.ptr(box)
+    assert L.or_morton_one(0.0, 0.0, bp) == 0
+    assert L.or_morton_one(1.0, 1.0, bp) == 0xFFFFFFFFFFFFFFFF      # clamp to 2^32-1
+    assert L.or_morton_one(0.5, 0.0, bp) == 1 << 63                 # x in the odd bits
+    assert L.or_morton_one(0.0, 0.5, bp) == 1 << 62
+    assert L.or_morton_one(-3.0, 7.0, bp) == 0x5555555555555555     # clamp low / high
+    assert L.or_morton_one(0.25, 0.75, bp) == (1 << 61) | (1 << 62) | (1 << 60)
+
+
+def test_morton_per_bit_brute_force():
+    rng = np.random.default_rng(1)
+    pos = rng.uniform(-3, 5, size=(500, 2))
+    box = O.bbox_square(pos)
+    keys = O.morton(pos, box)
+    for (x, y), k in zip(pos, keys):
+        ix = int(math.floor(min(max((x - box[0]) * (1.0 / (box[1] - box[0])) * 2.0**32, 0.0),
+                                2.0**32 - 1)))
+        iy = int(math.floor(min(max((y - box[2]) * (1.0 / (box[3] - box[2])) * 2.0**32, 0.0),
+                                2.0**32 - 1)))
+        ref = 0
+        for b in range(32):
+            ref |= ((ix >> b) & 1) << (2 * b + 1)
+            ref |= ((iy >> b) & 1) << (2 * b)
+        assert int(k) == ref
+
+
+def test_bbox_square_hand_computed():
+    pos = np.array([[0.0, 0.0], [4.0, 1.0], [2.0, -1.0]])
+    # x range 4 > y range 2: y squared about its midpoint 0
+    assert np.array_equal(O.bbox_square(pos), [0.0, 4.0, -2.0, 2.0])
+    pos = np.array([[1.0, 0.0], [2.0, 10.0]])
+    assert np.array_equal(O.bbox_square(pos), [-3.5, 6.5, 0.0, 10.0])
+
+
+def test_stable_sort_equals_numpy_stable_argsort():
+    rng = np.random.default_rng(2)
+    keys = rng.integers(0, 50, 5000).astype(np.uint64) << np.uint64(40)
+    assert np.array_equal(O.stable_sort(keys), np.argsort(keys, kind="stable"))
+    keys = rng.integers(0, 2**63, 3001, dtype=np.uint64)
+    assert np.array_equal(O.stable_sort(keys), np.argsort(keys, kind="stable"))
+
+
+def test_root_width_sq_pow_equals_product():
+    # The step computes the root width^2 as w*w; the reference writes pow(w, 2.0).
+    rng = np.random.default_rng(3)
+    w = rng.uniform(1e-6, 1e6, 100000)
+    assert np.array_equal(np.power(w, 2.0), w * w)
+
+
+# ---------------- tree topology ----------------
+
+def brute_force_tree(skeys):
+    """All (level, bstart, bend) of non-empty nodes: level L node = 2L-bit prefix,
+    present iff L == 0 or its parent holds >= 2 bodies at level < 30."""
+    n = len(skeys)
+    k = [int(v) for v in skeys]
+    out = set()
+    groups = {(): (0, n - 1)}
+    out.add((0, 0, n - 1))
+    for lvl in range(1, 31):
+        nxt = {}
+        for i, kk in enumerate(k):
+            pre = kk >> (64 - 2 * lvl)
+            par = kk >> (64 - 2 * (lvl - 1)) if lvl > 1 else 0
+            nxt.setdefault((par, pre), []).append(i)
+        prev_counts = {}
+        for i, kk in enumerate(k):
+            par = kk >> (64 - 2 * (lvl - 1)) if lvl > 1 else 0
+            prev_counts[par] = prev_counts.get(par, 0) + 1
+        for (par, pre), idx in nxt.items():
+            if prev_counts[par] >= 2:
+                out.add((lvl, idx[0], idx[-1]))
+    return out
+
+
+@pytest.mark.parametrize("seed,n", [(0, 200), (1, 777), (2, 2)])
+def test_tree_matches_brute_force(seed, n):
+    m, p, v = ics.exp_disk(n, seed) if n > 2 else (np.ones(2), np.array([[0, 0], [1, 1.0]]),
+                                                   np.zeros((2, 2)))
+    box, keys, perm, root_wsq = O.prepare(p, m)
+    nodes = O.tree_build(keys[perm], p[perm], m[perm], root_wsq)
+    got = {(int(a), int(b), int(c)) for a, b, c in zip(nodes["level"], nodes["bstart"],
+                                                       nodes["bend"]) if b >= 0}
+    assert got == brute_force_tree(keys[perm])
+    internal = nodes["c0"] >= 0
+    assert len(nodes) == 1 + 4 * internal.sum()
+    # root conservation
+    assert nodes["mass"][0] == pytest.approx(m.sum(), rel=1e-14)
+    assert nodes["cx"][0] == pytest.approx((m * p[:, 0]).sum() / m.sum(), rel=1e-12, abs=1e-13)
+
+
+def test_tree_coincident_bodies_cap_at_level_30():
+    m, p, v = ics.uniform_disk(64, 4)
+    p[10:20] = p[3]
+    box, keys, perm, root_wsq = O.prepare(p, m)
+    nodes = O.tree_build(keys[perm], p[perm], m[perm], root_wsq)
+    assert nodes["level"].max() == 30
+    multi = nodes[(nodes["c0"] < 0) & (nodes["bend"] > nodes["bstart"])]
+    assert len(multi) == 1 and multi["bend"][0] - multi["bstart"][0] + 1 == 11
+    assert multi["mass"][0] == pytest.approx(m[[3] + list(range(10, 20))].sum(), rel=1e-15)
+
+
+def test_tree_matches_reference_cpu_quadtree_on_dyadic_lattice():
+    """Where geometric splits coincide with Morton bits (dyadic lattice, square
+    extent, body 0 not the maximum), the reference CPU quadtree and the
+    CUDA-semantics tree have the same cells: CPU slot k = xhi | yhi<<1 maps to
+    Morton digit q = xhi<<1 | yhi."""
+    rng = np.random.default_rng(5)
+    S = 64
+    pts = {(0, 0), (S, S)}
+    while len(pts) < 300:
+        pts.add((int(rng.integers(0, S + 1)), int(rng.integers(0, S + 1))))
+    pts = sorted(pts, key=lambda t: (t != (32, 17), t))  # body 0 in the interior
+    pts = [(32, 17)] + [t for t in pts if t != (32, 17)]
+    pos = np.array(pts, dtype=np.float64)
+    mass = rng.uniform(0.5, 2.0, len(pos))
+    tl, bc, rect, com, tm = O.refcpu_quadtree(pos, mass)
+    assert np.array_equal(O.refcpu_bbox(pos), [0.0, 0.0, S, S])
+    box, keys, perm, root_wsq = O.prepare(pos, mass)
+    nodes = O.tree_build(keys[perm], pos[perm], mass[perm], root_wsq)
+    # CPU cells: level by BFS, Morton prefix from the cell's integer position
+    lvl = np.zeros(len(tl), dtype=int)
+    cpu = {}
+    for k in range(len(tl)):
+        if tl[k] != 0:
+            lvl[tl[k]:tl[k] + 4] = lvl[k] + 1
+    for k in range(len(tl)):
+        L_ = lvl[k]
+        side = S / 2 ** L_
+        ix, iy = int(rect[k, 0] / side), int(rect[k, 1] / side)
+        pre = 0
+        for b in range(L_):
+            pre |= ((ix >> b) & 1) << (2 * b + 1)
+            pre |= ((iy >> b) & 1) << (2 * b)
+        cpu[(L_, pre)] = (int(bc[k]), tm[k], com[k])
+    gpu = {}
+    for x in nodes:
+        gpu[(int(x["level"]), int(x["prefix"]))] = (
+            0 if x["bstart"] < 0 else int(x["bend"] - x["bstart"] + 1), x["mass"],
+            (x["cx"], x["cy"]))
+    assert set(cpu) == set(gpu)
+    for key, (cnt, mm, c) in cpu.items():
+        g = gpu[key]
+        assert cnt == g[0]
+        if cnt:
+            assert mm == pytest.approx(g[1], rel=1e-14)
+            assert np.allclose(c, g[2], rtol=1e-13, atol=1e-12)
+
+
+def test_refcpu_bbox_else_if_quirk():
+    # body 0 holds the largest x: the `else if` never lets it update the max
+    pos = np.array([[9.0, 1.0], [1.0, 2.0], [5.0, 3.0]])
+    r = O.refcpu_bbox(pos)
+    assert r[0] == 1.0 and r[0] + r[2] == 5.0
+    assert r[1] == 1.0 and r[1] + r[3] == 3.0
+
+
+# ---------------- forces ----------------
+
+def test_direct_sum_two_body_and_ring_closed_forms():
+    pos = np.array([[0.0, 0.0], [3.0, 4.0]])
+    mass = np.array([2.0, 5.0])
+    acc = O.direct_sum(pos, mass, 0.0)
+    assert np.allclose(acc[0], 5.0 * np.array([3.0, 4.0]) / 125.0, rtol=1e-15)
+    assert np.allclose(acc[1], -2.0 * np.array([3.0, 4.0]) / 125.0, rtol=1e-15)
+    K, R = 12, 2.0
+    th = 2 * np.pi * np.arange(K) / K
+    ring = np.stack([R * np.cos(th), R * np.sin(th)], axis=1)
+    a = O.direct_sum(ring, np.ones(K), 0.0)
+    expect = -(1.0 / (4 * R * R)) * sum(1.0 / math.sin(math.pi * k / K) for k in range(1, K))
+    assert a[0, 0] == pytest.approx(expect, rel=1e-13)
+    assert abs(a[0, 1]) < 1e-13
+    # softening: (d^2 + eps^2)^{-3/2}
+    acc = O.direct_sum(pos, mass, 0.25)
+    assert np.allclose(acc[0], 5.0 * np.array([3.0, 4.0]) / 25.25 ** 1.5, rtol=1e-15)
+
+
+def test_direct_sum_newton3():
+    m, p, v = ics.uniform_disk(800, 6)
+    a = O.direct_sum(p, m, 1e-4)
+    f = (m[:, None] * a).sum(axis=0)
+    assert np.all(np.abs(f) <= 1e-12 * np.abs(m[:, None] * a).sum())
+
+
+@pytest.mark.parametrize("guard", [0, 1])
+def test_tree_forces_theta0_equals_direct_sum(guard):
+    m, p, v = ics.exp_disk(700, 7)
+    acc, inter = O.bh_forces(p, m, 0.0, 0.01, guard)
+    ref = O.direct_sum(p, m, 1e-4)
+    rms, mx = ics.force_errors(acc, ref)
+    assert mx < 1e-12
+    assert np.all(inter == len(m) - 1)
+
+
+def test_tree_forces_converge_with_theta():
+    m, p, v = ics.exp_disk(3000, 8)
+    ref = O.direct_sum(p, m, 1e-4)
+    errs = [ics.force_errors(O.bh_forces(p, m, th, 0.01)[0], ref)[0] for th in (0.2, 0.4, 0.8)]
+    assert errs[0] < errs[1] < errs[2] < 0.05
+
+
+def test_tree_forces_two_body_analytic():
+    pos = np.array([[0.0, 0.0], [1.0, 2.0]])
+    mass = np.array([3.0, 0.5])
+    acc, inter = O.bh_forces(pos, mass, 0.5, 0.1)
+    r2 = 5.0 + 0.01
+    assert np.allclose(acc[0], 0.5 * np.array([1.0, 2.0]) / r2 ** 1.5, rtol=1e-15)
+    assert np.allclose(acc[1], -3.0 * np.array([1.0, 2.0]) / r2 ** 1.5, rtol=1e-15)
+    assert list(inter) == [1, 1]
+
+
+def test_refcpu_accel_theta0_eps0_is_direct_sum():
+    m, p, v = ics.uniform_disk(600, 9)
+    for kernel in (O.FORCE_REF_CPU, O.FORCE_PLUMMER):
+        acc, inter = O.refcpu_accel(p, m, 2.0, 0.0, 0.0, kernel)
+        ref = 2.0 * O.direct_sum(p, m, 0.0)
+        assert ics.force_errors(acc, ref)[1] < 1e-12
+        assert np.all(inter == len(m) - 1)
+
+
+def test_refcpu_kernel_differs_from_plummer_inside_softening():
+    # the reference CPU force G m M/(d^2+eps^2) along d/|d| (H2): ratio d/sqrt(d^2+eps^2)
+    pos = np.array([[0.0, 0.0], [0.1, 0.0]])
+    mass = np.array([1.0, 1.0])
+    a_ref, _ = O.refcpu_accel(pos, mass, 1.0, 0.5, 0.01, O.FORCE_REF_CPU)
+    a_pl, _ = O.refcpu_accel(pos, mass, 1.0, 0.5, 0.01, O.FORCE_PLUMMER)
+    assert a_ref[0, 0] == pytest.approx(1.0 / 0.02, rel=1e-14)
+    assert a_pl[0, 0] == pytest.approx(0.1 / 0.02 ** 1.5, rel=1e-14)
+
+
+def test_refcpu_bh_threads_are_deterministic_and_validated():
+    m, p, v = ics.exp_disk(2000, 10)
+    runs = [O.refcpu_bh_run(p, v, m, 1.0, 1e-3, 0.5, 1e-4, t, 3)[1:3] for t in (1, 3, 8)]
+    for r in runs[1:]:
+        assert np.array_equal(r[0], runs[0][0]) and np.array_equal(r[1], runs[0][1])
+    assert O.refcpu_bh_run(p, v, m, 1.0, 1e-3, 0.5, 1e-4, 0, 1)[0] == -1
+    assert O.refcpu_bh_run(p[:3], v[:3], m[:3], 1.0, 1e-3, 0.5, 1e-4, 4, 1)[0] == -1
+
+
+def test_refcpu_bh_kick_then_drift_single_step():
+    m, p, v = ics.exp_disk(500, 11)
+    dt = 1e-3
+    acc, _ = O.refcpu_accel(p, m, 1.0, 0.5, 1e-4, O.FORCE_REF_CPU)
+    rc, p1, v1, _ = O.refcpu_bh_run(p, v, m, 1.0, dt, 0.5, 1e-4, 2, 1)
+    # F/m*dt vs acc*dt: identical up to one rounding of F/m
+    assert np.allclose(v1, v + acc * dt, rtol=1e-14, atol=1e-16)
+    assert np.array_equal(p1, p + v1 * dt)
+
+
+def test_allpairs_two_body_closed_form_and_momentum():
+    pos = np.array([[0.0, 0.0], [2.0, 0.0]])
+    vel = np.array([[0.0, 0.1], [0.0, -0.1]])
+    mass = np.array([1.0, 3.0])
+    dt = 0.01
+    p1, v1 = O.refcpu_allpairs_run(pos, vel, mass, 1.0, dt, 0.0, 1)
+    x = pos + vel * dt
+    d = x[1] - x[0]
+    r = np.linalg.norm(d)
+    f = 1.0 * 3.0 / r ** 2 * d / r
+    assert np.allclose(p1, x, rtol=0, atol=1e-16)
+    assert np.allclose(v1[0], vel[0] + f / 1.0 * dt, rtol=1e-14)
+    assert np.allclose(v1[1], vel[1] - f / 3.0 * dt, rtol=1e-14)
+    m, p, v = ics.uniform_disk(300, 12)
+    v = np.random.default_rng(0).normal(size=p.shape)
+    p2, v2 = O.refcpu_allpairs_run(p, v, m, 1.0, 1e-3, 0.01, 5)
+    mom0, mom1 = (m[:, None] * v).sum(0), (m[:, None] * v2).sum(0)
+    assert np.allclose(mom0, mom1, rtol=0, atol=1e-11)
+
+
+def test_plummer_softening_exact_branch():
+    rng = np.random.default_rng(13)
+    pos = rng.normal(size=(40, 2))
+    eps = L.or_plummer_softening_exact(40, O.ptr(pos), 0.05, 1e6)
+    d = np.linalg.norm(pos[:, None, :] - pos[None, :, :], axis=2)
+    expect = d[np.triu_indices(40, 1)].sum() / 40 * 0.05   # divided by n, not pairs
+    assert eps == pytest.approx(expect, rel=1e-13)
+    assert L.or_plummer_softening_exact(40, O.ptr(pos), 0.0, 1e6) == 0.0
+    assert L.or_plummer_softening_exact(2000, O.ptr(rng.normal(size=(2000, 2))), 0.1, 1e6) == -1
+
+
+def test_model_step_free_particles_and_permutation():
+    m, p, v = ics.exp_disk(1000, 14)
+    op, ov, om, oo = O.model_step(p, v, m, np.arange(1000), 0.0, 0.5, 0.5, 1e-4, steps=2)
+    assert np.array_equal(np.sort(oo), np.arange(1000))
+    rp = np.empty_like(op)
+    rp[oo] = op
+    assert np.allclose(rp, p + 2 * 0.5 * v, rtol=0, atol=1e-12)
+    assert np.array_equal(om, m[oo])
