@@ -81,6 +81,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=N_BODIES)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--error-sample", type=int, default=4096)
+    ap.add_argument("--traversal", choices=("group", "exact"), default="group")
+    ap.add_argument("--theta", type=float, default=THETA)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -93,7 +95,10 @@ def main() -> int:
     n = args.n
     mass, pos, vel = ics.exp_disk(n, seed=SEED)
     print(f"[bench] n={n} steps={args.steps} warmup={args.warmup}", flush=True)
-    ctx = nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, theta=THETA, softening=EPS)
+    trav = nbody2d.abi.TRAV_GROUP if args.traversal == "group" else nbody2d.abi.TRAV_EXACT
+    theta = args.theta
+    ctx = nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, theta=theta, softening=EPS,
+                          traversal=trav)
     ctx.step(args.warmup)
     t0 = time.perf_counter()
     ctx.step(args.steps)
@@ -126,8 +131,10 @@ def main() -> int:
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS,
-        "traffic": traffic_from_profile("k_traverse"),
-        "kernel": "k_traverse (FP32 vector; 14 flops per interaction)",
+        "traffic": traffic_from_profile("k_traverse_group" if args.traversal == "group"
+                                        else "k_traverse"),
+        "kernel": ("k_traverse_group" if args.traversal == "group" else "k_traverse")
+                  + " (FP32 vector; 14 flops per interaction)",
         "interactions_per_launch": total,
         "avg_launch_ms": force_ms,
     }
@@ -145,10 +152,10 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"exp_disk N={n} plain Barnes-Hut theta={THETA} eps={EPS} dt={DT} "
+            "workload": f"exp_disk N={n} plain Barnes-Hut theta={theta} eps={EPS} dt={DT} "
                         "(FP32 Kahan pair forces, FP64 state/leapfrog, hipGraph steps)",
-            "n_bodies": n, "theta": THETA, "softening": EPS, "dt": DT, "seed": SEED,
-            "far_field": "monopole",
+            "n_bodies": n, "theta": theta, "softening": EPS, "dt": DT, "seed": SEED,
+            "far_field": "monopole", "traversal": args.traversal,
         },
         "force_error": {"rms": rms, "max": mx, "sample": int(len(t)),
                         "judge": "GPU FP64 direct sum"},

@@ -55,6 +55,18 @@ enum nb_integrator { NB_INTEG_KD = 0, NB_INTEG_DK = 1 };
  * plus a learned correction (nb_load_mlp). */
 enum nb_far_field { NB_FAR_MONOPOLE = 0, NB_FAR_LEARNED = 1 };
 
+/* Barnes-Hut traversal. EXACT: every body applies the reference's opening
+ * test to every cell it reaches (per-body MAC of BarnesHutCuda.cu:461-471), so
+ * each body gets exactly the reference's force terms; a wavefront of 64
+ * Morton-adjacent bodies walks the tree together with one shared stack of
+ * (cell, lane mask) entries. GROUP: the same wavefront accepts a cell only
+ * when the test holds for the whole group's bounding box (a conservative
+ * superset of every member's test), collects the accepted cells and leaf
+ * bodies in an LDS interaction list, then every lane sums the whole list.
+ * GROUP applies more (never fewer) force terms per body than EXACT at the
+ * same theta, so its force error is no larger. */
+enum nb_traversal { NB_TRAV_EXACT = 0, NB_TRAV_GROUP = 1 };
+
 typedef struct nb_params {
     double G;           /* gravitational constant; the reference hard-codes
                            6.67430e-11 (Constants.hpp:17) */
@@ -68,7 +80,10 @@ typedef struct nb_params {
                            (the reference can, for theta > 1/sqrt(2)) */
     int32_t far_field;  /* enum nb_far_field */
     int32_t use_graph;  /* 1: steps replay a captured hipGraph (default) */
-    int32_t reserved[7];
+    int32_t traversal;  /* enum nb_traversal */
+    int32_t group_budget; /* GROUP: interaction-list entries after which a group
+                             is re-evaluated by the EXACT walk (0 = 4096) */
+    int32_t reserved[5];
 } nb_params;
 
 typedef struct nb_info {
@@ -103,7 +118,7 @@ typedef struct nb_node {
 } nb_node;
 
 /* Fills *p with the defaults: G=1, dt=1e-3, theta=0.5, softening=0.01,
- * Barnes-Hut, KD, no self guard, monopole, graphs on. */
+ * Barnes-Hut, KD, no self guard, monopole, graphs on, EXACT traversal. */
 int nb_params_default(nb_params* p);
 
 /* Replaces: the BarnesHutCuda constructor + init_device_resources

@@ -57,7 +57,9 @@ bool is_bh(const nb_ctx* c) { return c->p.algorithm == NB_ALGO_BARNES_HUT; }
 bool is_dk(const nb_ctx* c) { return c->p.integrator == NB_INTEG_DK; }
 
 uint64_t tree_bytes(uint64_t icap) {
-    return (1 + 4 * icap) * sizeof(NodeRec) + icap * (sizeof(uint32_t) + 2);
+    const uint64_t ng = (icap + 255) / 256;
+    return (1 + 4 * icap) * sizeof(NodeRec) + icap * (3 * sizeof(uint32_t) + 2) +
+           kMaxLevel * ng * sizeof(uint32_t);
 }
 
 void free_tree(nb_ctx* c) {
@@ -66,6 +68,9 @@ void free_tree(nb_ctx* c) {
     dfree(c->b.inode_slot);
     dfree(c->b.inode_level);
     dfree(c->b.inode_mask);
+    dfree(c->b.node_a);
+    dfree(c->b.by_level);
+    dfree(c->b.lvl_cnt);
     c->icap = 0;
 }
 
@@ -76,6 +81,9 @@ int alloc_tree(nb_ctx* c, uint64_t icap) {
     NB_HIP(c, dalloc(c, &c->b.inode_slot, icap));
     NB_HIP(c, dalloc(c, &c->b.inode_level, icap));
     NB_HIP(c, dalloc(c, &c->b.inode_mask, icap));
+    NB_HIP(c, dalloc(c, &c->b.node_a, icap));
+    NB_HIP(c, dalloc(c, &c->b.by_level, icap));
+    NB_HIP(c, dalloc(c, &c->b.lvl_cnt, kMaxLevel * ((icap + 255) / 256)));
     c->device_bytes = before + tree_bytes(icap);
     c->icap = icap;
     return NB_OK;
@@ -136,6 +144,9 @@ int check_params(const nb_params* p, std::string* why) {
         return *why = "unknown algorithm", NB_ERR_INVALID_ARG;
     if (p->integrator != NB_INTEG_KD && p->integrator != NB_INTEG_DK)
         return *why = "unknown integrator", NB_ERR_INVALID_ARG;
+    if (p->traversal != NB_TRAV_EXACT && p->traversal != NB_TRAV_GROUP)
+        return *why = "unknown traversal", NB_ERR_INVALID_ARG;
+    if (p->group_budget < 0) return *why = "group_budget must be >= 0", NB_ERR_INVALID_ARG;
     if (p->far_field != NB_FAR_MONOPOLE && p->far_field != NB_FAR_LEARNED)
         return *why = "unknown far_field", NB_ERR_INVALID_ARG;
     if (p->far_field == NB_FAR_LEARNED)
@@ -302,6 +313,7 @@ int nb_params_default(nb_params* p) {
     p->self_guard = 0;
     p->far_field = NB_FAR_MONOPOLE;
     p->use_graph = 1;
+    p->traversal = NB_TRAV_EXACT;
     return NB_OK;
 }
 
@@ -354,7 +366,10 @@ int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* 
     }
     c->sort_tiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
     c->bbox_blocks = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((n + 2047) / 2048, 1), 1024);
-    const uint64_t scan_blocks = (n + 1023) / 1024 + 1;
+    // block sums of the widest scan: the level counts of the largest node pool
+    // (internal nodes are fewer than 30 per body pair)
+    const uint64_t scan_blocks = std::max<uint64_t>((n + 1023) / 1024,
+                                                    (uint64_t)kMaxLevel * (30 * n / 256 + 2) / 1024) + 2;
     NB_TRY(dalloc(c, &b.sort_hist, bh ? (uint64_t)kSortBins * c->sort_tiles : 0));
     NB_TRY(dalloc(c, &b.sort_tot, kSortBins));
     NB_TRY(dalloc(c, &b.box, 8));

@@ -52,35 +52,24 @@ constexpr uint32_t kHaltFlags = FLAG_TREE_CAPACITY | FLAG_NONFINITE;
 // so c0 pops first, BarnesHutCuda.cu:474-481). Each lane therefore sees
 // exactly the nodes, in exactly the order, of its own per-body DFS, while the
 // wave fetches each node once (a wave-uniform scalar load).
-template <bool KICK, bool GUARD>
-__global__ __launch_bounds__(kTravThreads) void k_traverse(
-    const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
-    const double* __restrict__ mass, double2* __restrict__ vel, double2* __restrict__ acc_out,
-    uint32_t* __restrict__ inter_out, uint32_t n, const StepConsts* __restrict__ k,
-    uint32_t* __restrict__ meta, int count_step) {
-    if (meta[META_FLAGS] & kHaltFlags) return;
-    __shared__ int32_t s_node[kTravWaves][kStackDepth];
-    __shared__ uint64_t s_mask[kTravWaves][kStackDepth];
-    const int w = threadIdx.x >> 6, l = lane_id();
-    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t i = (blk * kTravWaves + w) * 64 + l;
-    const bool valid = i < n;
+// Masked walk of one wavefront (see above). s_node/s_mask: kStackDepth
+// entries of LDS owned by this wave. Returns false on stack overflow.
+template <bool GUARD>
+__device__ __forceinline__ bool exact_walk(const NodeRec* __restrict__ nodes,
+                                           const double2* __restrict__ pos,
+                                           const double* __restrict__ mass, double2 p,
+                                           int32_t me, bool valid, double theta_sq,
+                                           float eps2f, int32_t* s_node, uint64_t* s_mask,
+                                           Acc& a) {
+    const int l = lane_id();
     const uint64_t act = ballot64(valid);
-    if (count_step && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
-    if (act == 0) return;
-    const double2 p = valid ? pos[i] : make_double2(0.0, 0.0);
-    const double theta_sq = k->theta_sq;
-    const float eps2f = (float)k->eps2;
-    const int32_t me = (int32_t)i;
-    Acc a;
     int sp = 1;
-    s_node[w][0] = 0;
-    s_mask[w][0] = act;
-    bool overflow = false;
+    s_node[0] = 0;
+    s_mask[0] = act;
     while (sp > 0) {
         --sp;
-        const int32_t node = readfirst(s_node[w][sp]);
-        const uint64_t m = readfirst64(s_mask[w][sp]);
+        const int32_t node = readfirst(s_node[sp]);
+        const uint64_t m = readfirst64(s_mask[sp]);
         const NodeRec* r = nodes + node;
         const double rcx = r->cx, rcy = r->cy;
         const int32_t c0 = r->c0, bs = r->bs, be = r->be;
@@ -109,23 +98,42 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
             if (accept) interact(a, dx, dy, d2, r->mf, eps2f);
             const uint64_t open = ballot64(mine && !accept);
             if (open) {
-                if (sp + 4 > kStackDepth) {
-                    overflow = true;
-                    break;
-                }
+                if (sp + 4 > kStackDepth) return false;
                 const int32_t cm = r->mask;
 #pragma unroll
                 for (int q = 3; q >= 0; q--) {
                     if ((cm >> q) & 1) {
-                        s_node[w][sp] = c0 + q;
-                        s_mask[w][sp] = open;
+                        s_node[sp] = c0 + q;
+                        s_mask[sp] = open;
                         sp++;
                     }
                 }
             }
         }
     }
-    if (overflow && l == 0) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_STACK);
+    return true;
+}
+
+template <bool KICK, bool GUARD>
+__global__ __launch_bounds__(kTravThreads) void k_traverse(
+    const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
+    const double* __restrict__ mass, double2* __restrict__ vel, double2* __restrict__ acc_out,
+    uint32_t* __restrict__ inter_out, uint32_t n, const StepConsts* __restrict__ k,
+    uint32_t* __restrict__ meta, int count_step) {
+    if (meta[META_FLAGS] & kHaltFlags) return;
+    __shared__ int32_t s_node[kTravWaves][kStackDepth];
+    __shared__ uint64_t s_mask[kTravWaves][kStackDepth];
+    const int w = threadIdx.x >> 6;
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t i = (blk * kTravWaves + w) * 64 + lane_id();
+    const bool valid = i < n;
+    if (count_step && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
+    if (ballot64(valid) == 0) return;
+    const double2 p = valid ? pos[i] : make_double2(0.0, 0.0);
+    Acc a;
+    const bool ok = exact_walk<GUARD>(nodes, pos, mass, p, (int32_t)i, valid, k->theta_sq,
+                                      (float)k->eps2, s_node[w], s_mask[w], a);
+    if (!ok && lane_id() == 0) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_STACK);
     if (!valid) return;
     const double ax = (double)a.ax - (double)a.cx;
     const double ay = (double)a.ay - (double)a.cy;
@@ -142,9 +150,209 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Group traversal: one wavefront per 64 Morton-adjacent bodies. Phase 1 walks
+// the tree 64 cells at a time (one per lane) from a shared LDS stack; a cell
+// is accepted when wsq < theta^2 dmin^2, dmin = distance from its COM to the
+// group's bounding box -- then every member's own test holds too. Accepted
+// cells and single-body leaves go to an LDS interaction list as FP32 offsets
+// from the group centre (the FP64 subtraction is done once per entry).
+// Phase 2: every lane sums the list (broadcast LDS reads), 14 flops per entry,
+// FP32 tile sums added with Kahan compensation. The list is flushed whenever
+// fewer than 64 free entries remain. Taking up to 64 cells per step from the
+// top of the stack needs about 1,500 entries on a 1M-body disk at theta 0.5,
+// hence 2,048; the take is throttled so the stack can never overflow.
+// ---------------------------------------------------------------------------
+constexpr int kGStack = 2048;
+constexpr int kGList = 192;
+constexpr int kGFlush = kGList - 64;
+// A group whose interaction list passes `budget` entries (default 4096; its
+// bodies are spread over a large area, e.g. across a jump of the Morton
+// curve) is re-evaluated by the masked per-body walk instead; on a 1M-body
+// disk about 0.3 % of the groups do.
+constexpr int kGBudget = 4096;
+
+template <bool ZERO_EPS>
+__device__ __forceinline__ void eval_list(const float4* __restrict__ list, int cnt, float bx,
+                                          float by, float eps2f, Acc& a) {
+    float px = 0.f, py = 0.f;
+#pragma unroll 8
+    for (int e = 0; e < cnt; e++) {
+        const float4 d = list[e];
+        const float dx = d.x - bx, dy = d.y - by;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2f));
+        float rs = __builtin_amdgcn_rsqf(r2);
+        if (ZERO_EPS) rs = r2 > 0.f ? rs : 0.f;
+        const float w = d.z * (rs * rs * rs);
+        px = fmaf(dx, w, px);
+        py = fmaf(dy, w, py);
+    }
+    kahan_add(a.ax, a.cx, px);
+    kahan_add(a.ay, a.cy, py);
+}
+
+template <bool KICK, bool ZERO_EPS, bool GUARD>
+__global__ __launch_bounds__(64) void k_traverse_group(
+    const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
+    const double* __restrict__ mass, double2* __restrict__ vel, double2* __restrict__ acc_out,
+    uint32_t* __restrict__ inter_out, uint32_t n, const StepConsts* __restrict__ k,
+    uint32_t* __restrict__ meta, int count_step, int budget) {
+    if (meta[META_FLAGS] & kHaltFlags) return;
+    __shared__ __attribute__((aligned(16))) int32_t s_stack[kGStack];
+    __shared__ float4 s_list[kGList];
+    const int l = lane_id();
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t i = g * 64 + l;
+    const bool valid = i < n;
+    if (count_step && blockIdx.x == 0 && l == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
+    const double2 p = valid ? pos[i] : pos[g * 64];
+    // group bounding box (invalid lanes repeat the first body)
+    double xmin = p.x, xmax = p.x, ymin = p.y, ymax = p.y;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        xmin = fmin(xmin, __shfl_xor(xmin, d, 64));
+        xmax = fmax(xmax, __shfl_xor(xmax, d, 64));
+        ymin = fmin(ymin, __shfl_xor(ymin, d, 64));
+        ymax = fmax(ymax, __shfl_xor(ymax, d, 64));
+    }
+    const double ox = 0.5 * (xmin + xmax), oy = 0.5 * (ymin + ymax);
+    const double hx = 0.5 * (xmax - xmin), hy = 0.5 * (ymax - ymin);
+    const float bx = (float)(p.x - ox), by = (float)(p.y - oy);
+    const double theta_sq = k->theta_sq;
+    const float eps2f = (float)k->eps2;
+    const int32_t me = (int32_t)i;
+    const int32_t g_first = (int32_t)(g * 64), g_last = (int32_t)min(g * 64 + 63, n - 1);
+    Acc a;
+    uint32_t direct = 0;        // multi-body-leaf terms of this lane
+    bool own_in_multi = false;  // this body sits in a multi-body leaf (not listed)
+    int sp = 1;
+    int cnt = 0, listed = 0;
+    if (l == 0) s_stack[0] = 0;
+    bool overflow = false, fallback = false;
+    while (sp > 0) {
+        int take = min(64, sp);
+        take = min(take, (kGStack - sp) / 3);
+        if (take <= 0) {
+            overflow = true;
+            break;
+        }
+        const int base = sp - take;
+        const bool have = l < take;
+        const int32_t node = have ? s_stack[base + l] : 0;
+        sp = base;
+        NodeRec r;
+        if (have) r = nodes[node];  // one 64-byte record: four independent loads
+        const int32_t c0 = have ? r.c0 : -2;
+        const double dxc = have ? r.cx - ox : 0.0, dyc = have ? r.cy - oy : 0.0;
+        bool add = false, open = false;
+        int32_t bs = 0, be = 0;
+        if (have) {
+            if (c0 < 0) {
+                bs = r.bs;
+                be = r.be;
+                add = bs == be;
+            } else {
+                const double ex = fmax(fabs(dxc) - hx, 0.0), ey = fmax(fabs(dyc) - hy, 0.0);
+                const double dmin2 = fma(ex, ex, ey * ey);
+                add = r.wsq < theta_sq * dmin2;
+                if (GUARD) add = add && !(r.bs <= g_last && r.be >= g_first);
+                open = !add;
+            }
+        }
+        // append accepted cells / single-body leaves to the list
+        const uint64_t am = ballot64(add);
+        if (add) {
+            const int at = cnt + (int)__popcll(am & lanemask_lt());
+            s_list[at] = make_float4((float)dxc, (float)dyc, r.mf, 0.f);
+        }
+        cnt += (int)__popcll(am);
+        // push the non-empty children of opened cells
+        const int32_t cm = open ? r.mask : 0;
+        const uint32_t nc = (uint32_t)__popc(cm);
+        const uint32_t incl = wave_incl_scan(nc);
+        const int total = (int)__shfl(incl, 63, 64);
+        if (open) {
+            int at = sp + (int)(incl - nc);
+#pragma unroll
+            for (int q = 3; q >= 0; q--)
+                if ((cm >> q) & 1) s_stack[at++] = c0 + q;
+        }
+        sp += total;
+        // multi-body leaves (level 30): every lane sums their bodies directly
+        uint64_t mm = ballot64(have && c0 < 0 && be > bs);
+        while (mm) {
+            const int src = __ffsll((unsigned long long)mm) - 1;
+            mm &= mm - 1;
+            const int32_t js = __shfl(bs, src, 64), je = __shfl(be, src, 64);
+            own_in_multi |= (me >= js && me <= je);
+            for (int32_t j = js; j <= je; j++) {
+                if (j == me) continue;
+                const double2 q = pos[j];
+                const double ex = q.x - p.x, ey = q.y - p.y;
+                interact(a, ex, ey, fma(ex, ex, ey * ey), (float)mass[j], eps2f);
+            }
+        }
+        if (listed + cnt > budget) {
+            fallback = true;
+            break;
+        }
+        if (cnt > kGFlush) {
+            eval_list<ZERO_EPS>(s_list, cnt, bx, by, eps2f, a);
+            listed += cnt;
+            cnt = 0;
+        }
+    }
+    if (fallback) {
+        // the LDS stack is free again: reuse it for the masked walk
+        a = Acc();
+        overflow = !exact_walk<GUARD>(nodes, pos, mass, p, me, valid, theta_sq, eps2f, s_stack,
+                                      reinterpret_cast<uint64_t*>(s_stack + 2 * kStackDepth),
+                                      a);
+    } else if (cnt > 0) {
+        eval_list<ZERO_EPS>(s_list, cnt, bx, by, eps2f, a);
+        listed += cnt;
+    }
+    if (overflow && l == 0) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_STACK);
+    if (!valid) return;
+    direct = a.n;
+    const double ax = (double)a.ax - (double)a.cx;
+    const double ay = (double)a.ay - (double)a.cy;
+    if (KICK) {
+        const double gdt = k->G_dt;
+        double2 v = vel[i];
+        v.x += ax * gdt;
+        v.y += ay * gdt;
+        vel[i] = v;
+    } else {
+        const double G = k->G;
+        acc_out[i] = make_double2(ax * G, ay * G);
+        // a body's own single-body leaf is in its group's list (a zero term)
+        inter_out[i] = fallback ? direct : (uint32_t)listed - (own_in_multi ? 0u : 1u) + direct;
+    }
+}
+
 void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
     const uint32_t n = (uint32_t)c->n;
     if (n == 0) return;
+    if (c->p.traversal == NB_TRAV_GROUP) {
+        const uint32_t groups = (n + 63) / 64;
+        const bool z = !(c->p.softening > 0.0);
+        const bool gd = c->p.self_guard != 0;
+        const int budget = c->p.group_budget > 0 ? c->p.group_budget : kGBudget;
+#define NB_TRAVG(K, Z, GD)                                                                     \
+    hipLaunchKernelGGL((k_traverse_group<K, Z, GD>), dim3(groups), dim3(64), 0, c->stream,     \
+                       c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src], c->b.acc,     \
+                       c->b.inter, n, c->b.consts, c->b.meta, (int)count_step, budget)
+#define NB_TRAVG2(K, Z) if (gd) NB_TRAVG(K, Z, true); else NB_TRAVG(K, Z, false)
+        if (kick) {
+            if (z) { NB_TRAVG2(true, true); } else { NB_TRAVG2(true, false); }
+        } else {
+            if (z) { NB_TRAVG2(false, true); } else { NB_TRAVG2(false, false); }
+        }
+#undef NB_TRAVG2
+#undef NB_TRAVG
+        return;
+    }
     const uint32_t blocks = (n + kTravThreads - 1) / kTravThreads;
     const bool guard = c->p.self_guard != 0;
 #define NB_TRAV(K, G)                                                                          \

@@ -97,10 +97,9 @@ __global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ pa
         box[3] = ymax;
         const double w = xmax - xmin;
         box[4] = w * w;  // root width_sq (BarnesHutCuda.cu:717)
-        uint32_t flags = 0;
+        // flags stay set for the rest of the batch of steps
         if (!isfinite(xmin) || !isfinite(xmax) || !isfinite(ymin) || !isfinite(ymax))
-            flags |= FLAG_NONFINITE;
-        meta[META_FLAGS] = flags;
+            meta[META_FLAGS] |= (uint32_t)FLAG_NONFINITE;
     }
 }
 
@@ -378,7 +377,7 @@ void launch_scan_u32(nb_ctx* c, uint32_t* data, uint32_t count, uint32_t* total_
     hipLaunchKernelGGL(k_scan_reduce, dim3(g), dim3(kScanThreads), 0, c->stream, data, count,
                        c->b.scan_tmp);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, c->stream, c->b.scan_tmp, g,
-                       total_out);
+                       total_out ? total_out : c->b.scan_tmp + g);
     hipLaunchKernelGGL(k_scan_down, dim3(g), dim3(kScanThreads), 0, c->stream, data, count,
                        c->b.scan_tmp);
 }

@@ -59,13 +59,16 @@ __device__ __forceinline__ void write_empty(NodeRec* r, double wsq) {
 // child ranges (binary searches on the digit, lower_bound_quadrant of
 // BarnesHutCuda.cu:153-163) and the child slots. Leaves get their COM here;
 // multi-body leaves (only possible at level 30) are queued for k_leaf_multi.
-__device__ void build_node(uint32_t p, uint32_t a, int L, const uint64_t* __restrict__ keys,
-                           uint32_t n, const double2* __restrict__ pos,
-                           const double* __restrict__ mass, const int8_t* __restrict__ delta,
-                           const uint32_t* __restrict__ off, double root_wsq,
+__device__ void build_node(uint32_t p, const uint64_t* __restrict__ keys, uint32_t n,
+                           const double2* __restrict__ pos, const double* __restrict__ mass,
+                           const int8_t* __restrict__ delta, const uint32_t* __restrict__ off,
+                           double root_wsq, const uint32_t* __restrict__ node_a,
                            NodeRec* __restrict__ nodes, uint32_t* __restrict__ inode_slot,
-                           uint8_t* __restrict__ inode_level, uint8_t* __restrict__ inode_mask,
-                           uint32_t* __restrict__ multi_list, uint32_t* __restrict__ meta) {
+                           const uint8_t* __restrict__ inode_level,
+                           uint8_t* __restrict__ inode_mask, uint32_t* __restrict__ multi_list,
+                           uint32_t* __restrict__ meta) {
+    const uint32_t a = node_a[p];
+    const int L = inode_level[p];
     uint32_t b;
     if (L == 0) {
         b = n - 1;
@@ -148,7 +151,6 @@ __device__ void build_node(uint32_t p, uint32_t a, int L, const uint64_t* __rest
         }
     }
     inode_mask[p] = (uint8_t)mask;
-    inode_level[p] = (uint8_t)L;
     if (p == 0) {
         NodeRec* r = &nodes[0];
         r->wsq = root_wsq;
@@ -159,20 +161,23 @@ __device__ void build_node(uint32_t p, uint32_t a, int L, const uint64_t* __rest
     }
 }
 
-__global__ __launch_bounds__(256) void k_tree_nodes(
-    const uint64_t* __restrict__ keys, uint32_t n, const double2* __restrict__ pos,
-    const double* __restrict__ mass, const int8_t* __restrict__ delta,
-    const uint32_t* __restrict__ off, const double* __restrict__ box, uint32_t* __restrict__ meta,
-    uint64_t icap, NodeRec* __restrict__ nodes, uint32_t* __restrict__ inode_slot,
-    uint8_t* __restrict__ inode_level, uint8_t* __restrict__ inode_mask,
-    uint32_t* __restrict__ multi_list) {
+// Lists every internal node: pair i owns levels (delta_{i-1}, min(delta_i, 29)],
+// internal node ids off[i] + (L - delta_{i-1} - 1).
+__global__ __launch_bounds__(256) void k_tree_enum(const int8_t* __restrict__ delta,
+                                                   const uint32_t* __restrict__ off, uint32_t n,
+                                                   uint32_t* __restrict__ meta, uint64_t icap,
+                                                   uint32_t* __restrict__ node_a,
+                                                   uint8_t* __restrict__ inode_level,
+                                                   const double2* __restrict__ pos,
+                                                   const double* __restrict__ mass,
+                                                   const double* __restrict__ box,
+                                                   NodeRec* __restrict__ nodes) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const uint32_t I = meta[META_INTERNAL];
     if (I > icap || (meta[META_FLAGS] & FLAG_NONFINITE)) {
-        if (i == 0) meta[META_FLAGS] |= (I > icap ? FLAG_TREE_CAPACITY : 0u);
+        if (i == 0 && I > icap) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_TREE_CAPACITY);
         return;
     }
-    const double root_wsq = box[4];
     if (n == 1) {
         if (i == 0) {
             NodeRec* r = &nodes[0];
@@ -181,7 +186,7 @@ __global__ __launch_bounds__(256) void k_tree_nodes(
             r->cy = x.y;
             r->m = mass[0];
             r->mf = (float)mass[0];
-            r->wsq = root_wsq;
+            r->wsq = box[4];
             r->c0 = -1;
             r->mask = 0;
             r->bs = 0;
@@ -194,11 +199,27 @@ __global__ __launch_bounds__(256) void k_tree_nodes(
     const int prev = i ? (int)delta[i - 1] : -1;
     const int d = delta[i];
     const int hi = d < kMaxLevel - 1 ? d : kMaxLevel - 1;
+    const uint32_t o = off[i];
     for (int L = prev + 1; L <= hi; L++) {
-        const uint32_t p = off[i] + (uint32_t)(L - prev - 1);
-        build_node(p, i, L, keys, n, pos, mass, delta, off, root_wsq, nodes, inode_slot,
-                   inode_level, inode_mask, multi_list, meta);
+        const uint32_t p = o + (uint32_t)(L - prev - 1);
+        node_a[p] = i;
+        inode_level[p] = (uint8_t)L;
     }
+}
+
+__global__ __launch_bounds__(256) void k_tree_nodes(
+    const uint64_t* __restrict__ keys, uint32_t n, const double2* __restrict__ pos,
+    const double* __restrict__ mass, const int8_t* __restrict__ delta,
+    const uint32_t* __restrict__ off, const double* __restrict__ box, uint32_t* __restrict__ meta,
+    uint64_t icap, const uint32_t* __restrict__ node_a, NodeRec* __restrict__ nodes,
+    uint32_t* __restrict__ inode_slot, const uint8_t* __restrict__ inode_level,
+    uint8_t* __restrict__ inode_mask, uint32_t* __restrict__ multi_list) {
+    const uint32_t I = meta[META_INTERNAL];
+    if (I > icap || (meta[META_FLAGS] & FLAG_NONFINITE)) return;
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= I) return;
+    build_node(p, keys, n, pos, mass, delta, off, box[4], node_a, nodes, inode_slot, inode_level,
+               inode_mask, multi_list, meta);
 }
 
 // Multi-body leaf COM, one wave per leaf, in the order of compute_leaf_com
@@ -265,50 +286,125 @@ __global__ __launch_bounds__(256) void k_leaf_multi(const double2* __restrict__ 
     }
 }
 
-// Internal nodes of one level: children in slot order q = 0..3, M += m,
-// R = fma(m, com, R) (propagate_com_up, BarnesHutCuda.cu:315-344, with the
-// multiply-add nvcc contracts), com = R / M; second moments by the parallel
-// axis theorem. HBM traffic: 4 child slots read, 1 slot written per node.
-__global__ __launch_bounds__(256) void k_tree_com(int L, const uint32_t* __restrict__ meta,
-                                                  uint64_t icap,
-                                                  const uint8_t* __restrict__ inode_level,
-                                                  const uint8_t* __restrict__ inode_mask,
-                                                  const uint32_t* __restrict__ inode_slot,
-                                                  NodeRec* __restrict__ nodes) {
+// Mass / COM / second moments of an internal node from its four child slots
+// in slot order q = 0..3: M += m, R = fma(m, com, R) (propagate_com_up,
+// BarnesHutCuda.cu:315-344, with the multiply-add nvcc contracts), com = R/M;
+// moments by the parallel-axis theorem. Empty slots hold m = 0 and com = 0,
+// so summing all four adds exact zeros -- the result equals the reference's
+// sum over the non-empty children -- and the four slot loads are independent.
+struct Com {
+    double m, cx, cy;
+    float ixx, ixy, iyy;
+};
+
+__device__ __forceinline__ Com load_com(const NodeRec* __restrict__ r) {
+    Com c;
+    c.m = r->m;
+    c.cx = r->cx;
+    c.cy = r->cy;
+    c.ixx = r->ixx;
+    c.ixy = r->ixy;
+    c.iyy = r->iyy;
+    return c;
+}
+
+__device__ __forceinline__ Com combine(const Com ch[4]) {
+    double M = 0.0, Rx = 0.0, Ry = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        M += ch[q].m;
+        Rx = fma(ch[q].m, ch[q].cx, Rx);
+        Ry = fma(ch[q].m, ch[q].cy, Ry);
+    }
+    Com o;
+    o.m = M;
+    o.cx = Rx / M;
+    o.cy = Ry / M;
+    double a = 0.0, b = 0.0, c = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double m = ch[q].m, dx = ch[q].cx - o.cx, dy = ch[q].cy - o.cy;
+        a += (double)ch[q].ixx + m * dx * dx;
+        b += (double)ch[q].ixy + m * dx * dy;
+        c += (double)ch[q].iyy + m * dy * dy;
+    }
+    o.ixx = (float)a;
+    o.ixy = (float)b;
+    o.iyy = (float)c;
+    return o;
+}
+
+__device__ __forceinline__ void store_com(NodeRec* __restrict__ r, const Com& o, uint32_t mask) {
+    r->cx = o.cx;
+    r->cy = o.cy;
+    r->m = o.m;
+    r->mf = (float)o.m;
+    r->mask = (int32_t)mask;
+    r->ixx = o.ixx;
+    r->ixy = o.ixy;
+    r->iyy = o.iyy;
+}
+
+__device__ __forceinline__ bool tree_halted(const uint32_t* meta, uint64_t icap) {
+    return meta[META_INTERNAL] > icap ||
+           (meta[META_FLAGS] & (FLAG_TREE_CAPACITY | FLAG_NONFINITE));
+}
+
+// Counting sort of the internal nodes by level, so each level of the
+// bottom-up reduction touches only its own nodes: per-block level histograms
+// (LDS atomics), one exclusive scan over the level-major counts, then a
+// scatter. lvl_cnt has kMaxLevel rows of `nblk` counts.
+__global__ __launch_bounds__(256) void k_level_hist(const uint32_t* __restrict__ meta,
+                                                    uint64_t icap,
+                                                    const uint8_t* __restrict__ inode_level,
+                                                    uint32_t* __restrict__ lvl_cnt) {
+    __shared__ uint32_t h[kMaxLevel];
+    if (threadIdx.x < kMaxLevel) h[threadIdx.x] = 0;
+    __syncthreads();
+    const bool ok = !tree_halted(meta, icap);
     const uint32_t I = meta[META_INTERNAL];
-    if (I > icap || (meta[META_FLAGS] & (FLAG_TREE_CAPACITY | FLAG_NONFINITE))) return;
-    for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < I; p += gridDim.x * 256) {
-        if (inode_level[p] != L) continue;
-        const uint32_t mask = inode_mask[p];
-        const NodeRec* ch = &nodes[1 + 4ull * p];
-        double M = 0.0, Rx = 0.0, Ry = 0.0;
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (ok && p < I) atomicAdd(&h[inode_level[p]], 1u);
+    __syncthreads();
+    if (threadIdx.x < kMaxLevel) lvl_cnt[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_level_scatter(const uint32_t* __restrict__ meta,
+                                                       uint64_t icap,
+                                                       const uint8_t* __restrict__ inode_level,
+                                                       const uint32_t* __restrict__ lvl_off,
+                                                       uint32_t* __restrict__ by_level) {
+    __shared__ uint32_t h[kMaxLevel];
+    if (threadIdx.x < kMaxLevel) h[threadIdx.x] = 0;
+    __syncthreads();
+    if (tree_halted(meta, icap)) return;
+    const uint32_t I = meta[META_INTERNAL];
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= I) return;
+    const uint32_t L = inode_level[p];
+    const uint32_t r = atomicAdd(&h[L], 1u);
+    by_level[lvl_off[L * gridDim.x + blockIdx.x] + r] = p;
+}
+
+// The internal nodes of one level: segment [lvl_off[L * nblk], lvl_off[(L+1) * nblk])
+// of by_level (the last level ends at I).
+__global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __restrict__ meta,
+                                                   uint64_t icap, uint32_t nblk,
+                                                   const uint32_t* __restrict__ lvl_off,
+                                                   const uint32_t* __restrict__ by_level,
+                                                   const uint8_t* __restrict__ inode_mask,
+                                                   const uint32_t* __restrict__ inode_slot,
+                                                   NodeRec* __restrict__ nodes) {
+    if (tree_halted(meta, icap)) return;
+    const uint32_t I = meta[META_INTERNAL];
+    const uint32_t b = lvl_off[(uint64_t)L * nblk];
+    const uint32_t e = L + 1 < kMaxLevel ? lvl_off[(uint64_t)(L + 1) * nblk] : I;
+    for (uint32_t k = b + blockIdx.x * 256 + threadIdx.x; k < e; k += gridDim.x * 256) {
+        const uint32_t p = by_level[k];
+        Com ch[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            if (!((mask >> q) & 1u)) continue;
-            const double m = ch[q].m;
-            M += m;
-            Rx = fma(m, ch[q].cx, Rx);
-            Ry = fma(m, ch[q].cy, Ry);
-        }
-        const double cx = Rx / M, cy = Ry / M;
-        double a = 0.0, b = 0.0, c = 0.0;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            if (!((mask >> q) & 1u)) continue;
-            const double m = ch[q].m, dx = ch[q].cx - cx, dy = ch[q].cy - cy;
-            a += (double)ch[q].ixx + m * dx * dx;
-            b += (double)ch[q].ixy + m * dx * dy;
-            c += (double)ch[q].iyy + m * dy * dy;
-        }
-        NodeRec* r = &nodes[inode_slot[p]];
-        r->cx = cx;
-        r->cy = cy;
-        r->m = M;
-        r->mf = (float)M;
-        r->mask = (int32_t)mask;
-        r->ixx = (float)a;
-        r->ixy = (float)b;
-        r->iyy = (float)c;
+        for (int q = 0; q < 4; q++) ch[q] = load_com(&nodes[1 + 4ull * p + q]);
+        store_com(&nodes[inode_slot[p]], combine(ch), inode_mask[p]);
     }
 }
 
@@ -326,16 +422,25 @@ void launch_tree_build(nb_ctx* c, int src) {
                            c->b.delta, c->b.cnt);
     }
     launch_scan_u32(c, c->b.cnt, pairs, &c->b.meta[META_INTERNAL]);
-    hipLaunchKernelGGL(k_tree_nodes, dim3(pg), dim3(256), 0, c->stream, c->b.keys[0], n,
+    hipLaunchKernelGGL(k_tree_enum, dim3(pg), dim3(256), 0, c->stream, c->b.delta, c->b.cnt, n,
+                       c->b.meta, (uint64_t)c->icap, c->b.node_a, c->b.inode_level,
+                       c->b.pos[src], c->b.mass[src], c->b.box, c->b.nodes);
+    const uint32_t ng = (uint32_t)((c->icap + 255) / 256);
+    hipLaunchKernelGGL(k_tree_nodes, dim3(ng), dim3(256), 0, c->stream, c->b.keys[0], n,
                        c->b.pos[src], c->b.mass[src], c->b.delta, c->b.cnt, c->b.box, c->b.meta,
-                       (uint64_t)c->icap, c->b.nodes, c->b.inode_slot, c->b.inode_level,
-                       c->b.inode_mask, c->b.multi_list);
+                       (uint64_t)c->icap, c->b.node_a, c->b.nodes, c->b.inode_slot,
+                       c->b.inode_level, c->b.inode_mask, c->b.multi_list);
     hipLaunchKernelGGL(k_leaf_multi, dim3(64), dim3(256), 0, c->stream, c->b.pos[src],
                        c->b.mass[src], c->b.multi_list, c->b.meta, c->b.nodes);
-    const uint32_t cg = (uint32_t)((c->icap + 255) / 256 < 2048 ? (c->icap + 255) / 256 : 2048);
+    hipLaunchKernelGGL(k_level_hist, dim3(ng), dim3(256), 0, c->stream, c->b.meta,
+                       (uint64_t)c->icap, c->b.inode_level, c->b.lvl_cnt);
+    launch_scan_u32(c, c->b.lvl_cnt, (uint32_t)(kMaxLevel * ng), nullptr);
+    hipLaunchKernelGGL(k_level_scatter, dim3(ng), dim3(256), 0, c->stream, c->b.meta,
+                       (uint64_t)c->icap, c->b.inode_level, c->b.lvl_cnt, c->b.by_level);
+    const uint32_t cg = ng < 512 ? ng : 512;
     for (int L = kMaxLevel - 1; L >= 0; L--) {
-        hipLaunchKernelGGL(k_tree_com, dim3(cg ? cg : 1), dim3(256), 0, c->stream, L, c->b.meta,
-                           (uint64_t)c->icap, c->b.inode_level, c->b.inode_mask,
+        hipLaunchKernelGGL(k_com_level, dim3(cg), dim3(256), 0, c->stream, L, c->b.meta,
+                           (uint64_t)c->icap, ng, c->b.lvl_cnt, c->b.by_level, c->b.inode_mask,
                            c->b.inode_slot, c->b.nodes);
     }
 }

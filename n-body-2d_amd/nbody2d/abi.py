@@ -27,6 +27,8 @@ INTEG_KD = 0
 INTEG_DK = 1
 FAR_MONOPOLE = 0
 FAR_LEARNED = 1
+TRAV_EXACT = 0
+TRAV_GROUP = 1
 PHASES = ("bbox_morton", "sort", "build", "force", "drift")
 
 # Every symbol include/nbody.h declares (checked by tests/test_abi_symbols.py).
@@ -42,7 +44,8 @@ class Params(C.Structure):
     _fields_ = [
         ("G", C.c_double), ("dt", C.c_double), ("theta", C.c_double), ("softening", C.c_double),
         ("algorithm", C.c_int32), ("integrator", C.c_int32), ("self_guard", C.c_int32),
-        ("far_field", C.c_int32), ("use_graph", C.c_int32), ("reserved", C.c_int32 * 7),
+        ("far_field", C.c_int32), ("use_graph", C.c_int32), ("traversal", C.c_int32),
+        ("group_budget", C.c_int32), ("reserved", C.c_int32 * 5),
     ]
 
 
