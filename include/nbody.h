@@ -62,7 +62,9 @@ enum nb_far_field { NB_FAR_MONOPOLE = 0, NB_FAR_LEARNED = 1 };
  * (cell, lane mask) entries. GROUP: the same wavefront accepts a cell only
  * when the test holds for the whole group's bounding box (a conservative
  * superset of every member's test), collects the accepted cells and leaf
- * bodies in an LDS interaction list, then every lane sums the whole list.
+ * bodies in an LDS interaction list, then every lane sums the whole list; a
+ * group whose list grows past group_budget is split in halves, down to
+ * single bodies, whose group test is exactly their own opening test.
  * GROUP applies more (never fewer) force terms per body than EXACT at the
  * same theta, so its force error is no larger. */
 enum nb_traversal { NB_TRAV_EXACT = 0, NB_TRAV_GROUP = 1 };
@@ -82,7 +84,7 @@ typedef struct nb_params {
     int32_t use_graph;  /* 1: steps replay a captured hipGraph (default) */
     int32_t traversal;  /* enum nb_traversal */
     int32_t group_budget; /* GROUP: interaction-list entries after which a group
-                             is re-evaluated by the EXACT walk (0 = 4096) */
+                             is split in halves and walked again (0 = 8192) */
     int32_t reserved[5];
 } nb_params;
 

@@ -166,11 +166,11 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
 constexpr int kGStack = 2048;
 constexpr int kGList = 192;
 constexpr int kGFlush = kGList - 64;
-// A group whose interaction list passes `budget` entries (default 4096; its
-// bodies are spread over a large area, e.g. across a jump of the Morton
-// curve) is re-evaluated by the masked per-body walk instead; on a 1M-body
-// disk about 0.3 % of the groups do.
-constexpr int kGBudget = 4096;
+// Before its walk a group is split at its largest Morton jump when the two
+// parts are much more compact than the whole. As a safety net, a group whose
+// interaction list still passes `budget` entries (default 8192) is split into
+// halves and walked again, down to single bodies.
+constexpr int kGBudget = 8192;
 
 template <bool ZERO_EPS>
 __device__ __forceinline__ void eval_list(const float4* __restrict__ list, int cnt, float bx,
@@ -196,127 +196,196 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
     const double* __restrict__ mass, double2* __restrict__ vel, double2* __restrict__ acc_out,
     uint32_t* __restrict__ inter_out, uint32_t n, const StepConsts* __restrict__ k,
-    uint32_t* __restrict__ meta, int count_step, int budget) {
+    uint32_t* __restrict__ meta, int count_step, int budget,
+    const uint64_t* __restrict__ keys) {
     if (meta[META_FLAGS] & kHaltFlags) return;
     __shared__ __attribute__((aligned(16))) int32_t s_stack[kGStack];
     __shared__ float4 s_list[kGList];
+    __shared__ int32_t s_todo[16];
     const int l = lane_id();
     const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t i = g * 64 + l;
     const bool valid = i < n;
     if (count_step && blockIdx.x == 0 && l == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
     const double2 p = valid ? pos[i] : pos[g * 64];
-    // group bounding box (invalid lanes repeat the first body)
-    double xmin = p.x, xmax = p.x, ymin = p.y, ymax = p.y;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        xmin = fmin(xmin, __shfl_xor(xmin, d, 64));
-        xmax = fmax(xmax, __shfl_xor(xmax, d, 64));
-        ymin = fmin(ymin, __shfl_xor(ymin, d, 64));
-        ymax = fmax(ymax, __shfl_xor(ymax, d, 64));
-    }
-    const double ox = 0.5 * (xmin + xmax), oy = 0.5 * (ymin + ymax);
-    const double hx = 0.5 * (xmax - xmin), hy = 0.5 * (ymax - ymin);
-    const float bx = (float)(p.x - ox), by = (float)(p.y - oy);
     const double theta_sq = k->theta_sq;
     const float eps2f = (float)k->eps2;
     const int32_t me = (int32_t)i;
-    const int32_t g_first = (int32_t)(g * 64), g_last = (int32_t)min(g * 64 + 63, n - 1);
-    Acc a;
-    uint32_t direct = 0;        // multi-body-leaf terms of this lane
-    bool own_in_multi = false;  // this body sits in a multi-body leaf (not listed)
-    int sp = 1;
-    int cnt = 0, listed = 0;
-    if (l == 0) s_stack[0] = 0;
-    bool overflow = false, fallback = false;
-    while (sp > 0) {
-        int take = min(64, sp);
-        take = min(take, (kGStack - sp) / 3);
-        if (take <= 0) {
-            overflow = true;
-            break;
-        }
-        const int base = sp - take;
-        const bool have = l < take;
-        const int32_t node = have ? s_stack[base + l] : 0;
-        sp = base;
-        NodeRec r;
-        if (have) r = nodes[node];  // one 64-byte record: four independent loads
-        const int32_t c0 = have ? r.c0 : -2;
-        const double dxc = have ? r.cx - ox : 0.0, dyc = have ? r.cy - oy : 0.0;
-        bool add = false, open = false;
-        int32_t bs = 0, be = 0;
-        if (have) {
-            if (c0 < 0) {
-                bs = r.bs;
-                be = r.be;
-                add = bs == be;
-            } else {
-                const double ex = fmax(fabs(dxc) - hx, 0.0), ey = fmax(fabs(dyc) - hy, 0.0);
-                const double dmin2 = fma(ex, ex, ey * ey);
-                add = r.wsq < theta_sq * dmin2;
-                if (GUARD) add = add && !(r.bs <= g_last && r.be >= g_first);
-                open = !add;
-            }
-        }
-        // append accepted cells / single-body leaves to the list
-        const uint64_t am = ballot64(add);
-        if (add) {
-            const int at = cnt + (int)__popcll(am & lanemask_lt());
-            s_list[at] = make_float4((float)dxc, (float)dyc, r.mf, 0.f);
-        }
-        cnt += (int)__popcll(am);
-        // push the non-empty children of opened cells
-        const int32_t cm = open ? r.mask : 0;
-        const uint32_t nc = (uint32_t)__popc(cm);
-        const uint32_t incl = wave_incl_scan(nc);
-        const int total = (int)__shfl(incl, 63, 64);
-        if (open) {
-            int at = sp + (int)(incl - nc);
+    // Morton jump to the next body: digits shared with it (64 = none follows).
+    const uint64_t key = valid ? keys[i] : 0ull;
+    const uint64_t key_next = __shfl_down(key, 1, 64);
+    const int jump = (valid && l < 63 && i + 1 < n) ? __clzll(key ^ key_next) : 64;
+    // This lane's result, filled by the sub-group that finally owns it.
+    Acc fin;
+    uint32_t fin_listed = 0;
+    bool fin_own_multi = false, overflow = false;
+    // Pending sub-groups as (first lane << 8 | lane count); a group whose list
+    // passes the budget is split into halves (a single body is never split:
+    // its group test is its own opening test).
+    int nt = 1;
+    if (l == 0) s_todo[0] = 64;
+    while (nt > 0) {
+        const int32_t e = readfirst(s_todo[--nt]);
+        const int f = e >> 8, cn = e & 255;
+        const bool member = valid && l >= f && l < f + cn;
+        if (ballot64(member) == 0) continue;
+        double xmin = member ? p.x : INFINITY, xmax = member ? p.x : -INFINITY;
+        double ymin = member ? p.y : INFINITY, ymax = member ? p.y : -INFINITY;
 #pragma unroll
-            for (int q = 3; q >= 0; q--)
-                if ((cm >> q) & 1) s_stack[at++] = c0 + q;
+        for (int d = 32; d >= 1; d >>= 1) {
+            xmin = fmin(xmin, __shfl_xor(xmin, d, 64));
+            xmax = fmax(xmax, __shfl_xor(xmax, d, 64));
+            ymin = fmin(ymin, __shfl_xor(ymin, d, 64));
+            ymax = fmax(ymax, __shfl_xor(ymax, d, 64));
         }
-        sp += total;
-        // multi-body leaves (level 30): every lane sums their bodies directly
-        uint64_t mm = ballot64(have && c0 < 0 && be > bs);
-        while (mm) {
-            const int src = __ffsll((unsigned long long)mm) - 1;
-            mm &= mm - 1;
-            const int32_t js = __shfl(bs, src, 64), je = __shfl(be, src, 64);
-            own_in_multi |= (me >= js && me <= je);
-            for (int32_t j = js; j <= je; j++) {
-                if (j == me) continue;
-                const double2 q = pos[j];
-                const double ex = q.x - p.x, ey = q.y - p.y;
-                interact(a, ex, ey, fma(ex, ex, ey * ey), (float)mass[j], eps2f);
+        // Split at the largest Morton jump inside the group when both parts
+        // are far more compact than the whole (the group straddles a jump of
+        // the curve); no walk is wasted on the wide box.
+        if (cn > 1) {
+            const int my = (member && l < f + cn - 1) ? jump : 65;
+            int best = my;
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) best = min(best, __shfl_xor(best, d, 64));
+            const uint64_t at = ballot64(member && my == best);
+            const int cut = __ffsll((unsigned long long)at);  // first lane after the jump
+            if (best < 64 && cut > f && cut < f + cn) {
+                const bool left = l < cut;
+                double ax0 = member && left ? p.x : INFINITY, ax1 = member && left ? p.x : -INFINITY;
+                double ay0 = member && left ? p.y : INFINITY, ay1 = member && left ? p.y : -INFINITY;
+                double bx0 = member && !left ? p.x : INFINITY, bx1 = member && !left ? p.x : -INFINITY;
+                double by0 = member && !left ? p.y : INFINITY, by1 = member && !left ? p.y : -INFINITY;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) {
+                    ax0 = fmin(ax0, __shfl_xor(ax0, d, 64));
+                    ax1 = fmax(ax1, __shfl_xor(ax1, d, 64));
+                    ay0 = fmin(ay0, __shfl_xor(ay0, d, 64));
+                    ay1 = fmax(ay1, __shfl_xor(ay1, d, 64));
+                    bx0 = fmin(bx0, __shfl_xor(bx0, d, 64));
+                    bx1 = fmax(bx1, __shfl_xor(bx1, d, 64));
+                    by0 = fmin(by0, __shfl_xor(by0, d, 64));
+                    by1 = fmax(by1, __shfl_xor(by1, d, 64));
+                }
+                const double ew = fmax(xmax - xmin, ymax - ymin);
+                const double ea = fmax(ax1 - ax0, ay1 - ay0);
+                const double eb = fmax(bx1 - bx0, by1 - by0);
+                if (ew * ew > 4.0 * (ea * ea + eb * eb) && nt + 2 <= 16) {
+                    if (l == 0) {
+                        s_todo[nt] = (cut << 8) | (f + cn - cut);
+                        s_todo[nt + 1] = (f << 8) | (cut - f);
+                    }
+                    nt += 2;
+                    continue;
+                }
             }
         }
-        if (listed + cnt > budget) {
-            fallback = true;
-            break;
+        const double ox = 0.5 * (xmin + xmax), oy = 0.5 * (ymin + ymax);
+        const double hx = 0.5 * (xmax - xmin), hy = 0.5 * (ymax - ymin);
+        const float bx = (float)(p.x - ox), by = (float)(p.y - oy);
+        const int32_t g_first = (int32_t)(g * 64 + f);
+        const int32_t g_last = (int32_t)min(g * 64 + f + cn - 1, n - 1);
+        const bool capped = cn > 1;
+        Acc t;
+        int sp = 1, cnt = 0;
+        uint32_t listed = 0;
+        bool own_multi = false, split = false;
+        if (l == 0) s_stack[0] = 0;
+        while (sp > 0) {
+            int take = min(64, sp);
+            take = min(take, (kGStack - sp) / 3);
+            if (take <= 0) {
+                overflow = true;
+                break;
+            }
+            const int base = sp - take;
+            const bool have = l < take;
+            const int32_t node = have ? s_stack[base + l] : 0;
+            sp = base;
+            NodeRec r;
+            if (have) r = nodes[node];  // one 64-byte record: four independent loads
+            const int32_t c0 = have ? r.c0 : -2;
+            const double dxc = have ? r.cx - ox : 0.0, dyc = have ? r.cy - oy : 0.0;
+            bool add = false, open = false;
+            int32_t bs = 0, be = 0;
+            if (have) {
+                if (c0 < 0) {
+                    bs = r.bs;
+                    be = r.be;
+                    add = bs == be;
+                } else {
+                    const double ex = fmax(fabs(dxc) - hx, 0.0), ey = fmax(fabs(dyc) - hy, 0.0);
+                    const double dmin2 = fma(ex, ex, ey * ey);
+                    add = r.wsq < theta_sq * dmin2;
+                    if (GUARD) add = add && !(r.bs <= g_last && r.be >= g_first);
+                    open = !add;
+                }
+            }
+            // append accepted cells / single-body leaves to the list
+            const uint64_t am = ballot64(add);
+            if (add) {
+                const int at = cnt + (int)__popcll(am & lanemask_lt());
+                s_list[at] = make_float4((float)dxc, (float)dyc, r.mf, 0.f);
+            }
+            cnt += (int)__popcll(am);
+            // push the non-empty children of opened cells
+            const int32_t cm = open ? r.mask : 0;
+            const uint32_t nc = (uint32_t)__popc(cm);
+            const uint32_t incl = wave_incl_scan(nc);
+            const int total = (int)__shfl(incl, 63, 64);
+            if (open) {
+                int at = sp + (int)(incl - nc);
+#pragma unroll
+                for (int q = 3; q >= 0; q--)
+                    if ((cm >> q) & 1) s_stack[at++] = c0 + q;
+            }
+            sp += total;
+            // multi-body leaves (level 30): every lane sums their bodies directly
+            uint64_t mm = ballot64(have && c0 < 0 && be > bs);
+            while (mm) {
+                const int src = __ffsll((unsigned long long)mm) - 1;
+                mm &= mm - 1;
+                const int32_t js = __shfl(bs, src, 64), je = __shfl(be, src, 64);
+                own_multi |= (me >= js && me <= je);
+                for (int32_t j = js; j <= je; j++) {
+                    if (j == me) continue;
+                    const double2 q = pos[j];
+                    const double ex = q.x - p.x, ey = q.y - p.y;
+                    interact(t, ex, ey, fma(ex, ex, ey * ey), (float)mass[j], eps2f);
+                }
+            }
+            if (capped && listed + (uint32_t)cnt > (uint32_t)budget && nt + 2 <= 16) {
+                split = true;
+                break;
+            }
+            if (cnt > kGFlush) {
+                eval_list<ZERO_EPS>(s_list, cnt, bx, by, eps2f, t);
+                listed += cnt;
+                cnt = 0;
+            }
         }
-        if (cnt > kGFlush) {
-            eval_list<ZERO_EPS>(s_list, cnt, bx, by, eps2f, a);
+        if (split) {
+            const int h = cn >> 1;
+            if (l == 0) {
+                s_todo[nt] = ((f + h) << 8) | (cn - h);
+                s_todo[nt + 1] = (f << 8) | h;
+            }
+            nt += 2;
+            continue;
+        }
+        if (overflow) break;
+        if (cnt > 0) {
+            eval_list<ZERO_EPS>(s_list, cnt, bx, by, eps2f, t);
             listed += cnt;
-            cnt = 0;
         }
-    }
-    if (fallback) {
-        // the LDS stack is free again: reuse it for the masked walk
-        a = Acc();
-        overflow = !exact_walk<GUARD>(nodes, pos, mass, p, me, valid, theta_sq, eps2f, s_stack,
-                                      reinterpret_cast<uint64_t*>(s_stack + 2 * kStackDepth),
-                                      a);
-    } else if (cnt > 0) {
-        eval_list<ZERO_EPS>(s_list, cnt, bx, by, eps2f, a);
-        listed += cnt;
+        if (member) {
+            fin = t;
+            fin_listed = listed;
+            fin_own_multi = own_multi;
+        }
     }
     if (overflow && l == 0) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_STACK);
     if (!valid) return;
-    direct = a.n;
-    const double ax = (double)a.ax - (double)a.cx;
-    const double ay = (double)a.ay - (double)a.cy;
+    const double ax = (double)fin.ax - (double)fin.cx;
+    const double ay = (double)fin.ay - (double)fin.cy;
     if (KICK) {
         const double gdt = k->G_dt;
         double2 v = vel[i];
@@ -327,7 +396,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
         const double G = k->G;
         acc_out[i] = make_double2(ax * G, ay * G);
         // a body's own single-body leaf is in its group's list (a zero term)
-        inter_out[i] = fallback ? direct : (uint32_t)listed - (own_in_multi ? 0u : 1u) + direct;
+        inter_out[i] = fin_listed - (fin_own_multi ? 0u : 1u) + fin.n;
     }
 }
 
@@ -342,7 +411,8 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
 #define NB_TRAVG(K, Z, GD)                                                                     \
     hipLaunchKernelGGL((k_traverse_group<K, Z, GD>), dim3(groups), dim3(64), 0, c->stream,     \
                        c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src], c->b.acc,     \
-                       c->b.inter, n, c->b.consts, c->b.meta, (int)count_step, budget)
+                       c->b.inter, n, c->b.consts, c->b.meta, (int)count_step, budget,     \
+                       c->b.keys[0])
 #define NB_TRAVG2(K, Z) if (gd) NB_TRAVG(K, Z, true); else NB_TRAVG(K, Z, false)
         if (kick) {
             if (z) { NB_TRAVG2(true, true); } else { NB_TRAVG2(true, false); }

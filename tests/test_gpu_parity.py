@@ -300,16 +300,19 @@ def test_group_traversal_step_matches_forces():
 
 
 @pytest.mark.parametrize("guard", [0, 1])
-def test_group_fallback_equals_exact(guard):
-    """With a budget of one entry every group falls back to the masked
-    per-body walk: results equal EXACT bit for bit."""
+def test_group_split_to_single_bodies_equals_exact(guard):
+    """With a budget of one entry every group splits down to single bodies,
+    whose group test is the per-body opening test: the interaction sets equal
+    EXACT's body by body and the forces agree to FP32 rounding."""
     m, p, v = CASES["disk4k"]
     kw = dict(theta=0.8, softening=0.01, self_guard=guard)
     with _ctx(m, p, v, traversal=nbody2d.abi.TRAV_GROUP, group_budget=1, **kw) as c:
         ga, gi, gt = c.forces()
     with _ctx(m, p, v, traversal=nbody2d.abi.TRAV_EXACT, **kw) as c:
         ea, ei, et = c.forces()
-    assert np.array_equal(ga, ea) and np.array_equal(gi, ei) and gt == et
+    assert np.array_equal(gi, ei) and gt == et
+    rms, mx = ics.force_errors(ga, ea)
+    assert rms <= AP_RMS_TOL and mx <= AP_MAX_TOL, (rms, mx)
 
 
 def test_group_traversal_large_disk_error_and_guard():
