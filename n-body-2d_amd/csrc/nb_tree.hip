@@ -408,15 +408,10 @@ __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __rest
     }
 }
 
-__global__ void k_tree_reset(uint32_t* __restrict__ meta) {
-    if (threadIdx.x == 0) meta[META_MULTI_LEAVES] = 0;
-}
-
 void launch_tree_build(nb_ctx* c, int src) {
     const uint32_t n = (uint32_t)c->n;
     const uint32_t pairs = n > 1 ? n - 1 : 0;
     const uint32_t pg = pairs ? (pairs + 255) / 256 : 1;
-    hipLaunchKernelGGL(k_tree_reset, dim3(1), dim3(64), 0, c->stream, c->b.meta);
     if (pairs) {
         hipLaunchKernelGGL(k_tree_delta, dim3(pg), dim3(256), 0, c->stream, c->b.keys[0], n,
                            c->b.delta, c->b.cnt);
@@ -437,6 +432,8 @@ void launch_tree_build(nb_ctx* c, int src) {
     launch_scan_u32(c, c->b.lvl_cnt, (uint32_t)(kMaxLevel * ng), nullptr);
     hipLaunchKernelGGL(k_level_scatter, dim3(ng), dim3(256), 0, c->stream, c->b.meta,
                        (uint64_t)c->icap, c->b.inode_level, c->b.lvl_cnt, c->b.by_level);
+    // One launch per level, deepest first (a level's children are all done
+    // when its launch starts).
     const uint32_t cg = ng < 512 ? ng : 512;
     for (int L = kMaxLevel - 1; L >= 0; L--) {
         hipLaunchKernelGGL(k_com_level, dim3(cg), dim3(256), 0, c->stream, L, c->b.meta,
