@@ -52,7 +52,7 @@ enum nb_algorithm { NB_ALGO_BARNES_HUT = 0, NB_ALGO_ALL_PAIRS = 1 };
 enum nb_integrator { NB_INTEG_KD = 0, NB_INTEG_DK = 1 };
 
 /* Far field of an accepted cell: the reference's monopole, or the monopole
- * plus a learned correction (nb_load_mlp). */
+ * plus the learned correction (nb_mlp_*; GROUP traversal only). */
 enum nb_far_field { NB_FAR_MONOPOLE = 0, NB_FAR_LEARNED = 1 };
 
 /* Barnes-Hut traversal. EXACT: every body applies the reference's opening
@@ -177,6 +177,59 @@ int nb_debug_tree(nb_ctx* ctx, nb_node* nodes, uint64_t cap);
  * acc[t] = G * sum_{j != targets[t]} m_j (r_j - r_t) / (|r|^2 + eps^2)^{3/2}. */
 int nb_direct_sum_f64(uint64_t n, const double* pos_xy, const double* mass, double G,
                       double softening, uint64_t m, const int64_t* targets, double* acc_xy);
+
+/* ---------------------------------------------------------------------------
+ * Learned far-field correction. No counterpart in the reference: its force
+ * is the monopole of every accepted cell (BarnesHutCuda.cu:461-471). The
+ * network (8 -> 64 -> 64 -> 2, ReLU, BF16 weights and activations on MFMA,
+ * FP32 accumulation) predicts a_exact - a_monopole of an accepted cell from
+ * its offset, size/distance, mass fraction and second moments; a context with
+ * far_field = NB_FAR_LEARNED adds it for every accepted internal cell.
+ * Parameters are FP32 in the order W1[64][8], b1[64], W2[64][64], b2[64],
+ * W3[2][64], b3[2] (NB_MLP_PARAMS values); a new context holds zeros (no
+ * correction).
+ * ------------------------------------------------------------------------- */
+#define NB_MLP_PARAMS 4866
+#define NB_MLP_INPUTS 8
+
+int nb_mlp_set_weights(nb_ctx* ctx, const float* params, uint64_t count);
+int nb_mlp_get_weights(nb_ctx* ctx, float* params, uint64_t count);
+
+/* He-normal initial weights from `seed`, zero biases, Adam state cleared. */
+int nb_mlp_init(nb_ctx* ctx, uint64_t seed);
+
+/* Training pairs from the current state (Barnes-Hut contexts): builds the
+ * tree, walks it from `sample_bodies` seeded bodies with the per-body opening
+ * test at `theta` (cells holding the body are never accepted), and keeps
+ * every accepted internal cell of at most 8,192 bodies, up to `max_pairs`.
+ * Each pair gets its 8 inputs and its target, the exact FP64 Plummer
+ * acceleration of the cell's bodies minus the cell's monopole, in the
+ * network's output units. *pairs_out = pairs held. */
+int nb_mlp_generate(nb_ctx* ctx, double theta, uint32_t sample_bodies, uint64_t max_pairs,
+                    uint64_t seed, uint64_t* pairs_out);
+
+/* Copies up to `cap` held pairs: features [k][8], targets [k][2], the sample
+ * body (index in the tree's sorted order) and the cell's tree slot. Any
+ * output may be NULL; *count = pairs held. */
+int nb_mlp_get_pairs(nb_ctx* ctx, float* features, float* targets, uint32_t* body,
+                     uint32_t* cell, uint64_t cap, uint64_t* count);
+
+/* Replaces the held pairs with caller data. */
+int nb_mlp_set_pairs(nb_ctx* ctx, const float* features, const float* targets, uint64_t count);
+
+/* `steps` Adam steps (beta 0.9/0.999) on seeded minibatches of `batch` held
+ * pairs, loss = mean over the batch of the squared error summed over the two
+ * outputs. *last_loss = loss of the last minibatch. */
+int nb_mlp_train(nb_ctx* ctx, uint32_t steps, uint32_t batch, float lr, uint64_t seed,
+                 float* last_loss);
+
+/* Gradient of that loss over exactly the given pairs (which become the held
+ * pairs) at the current weights, from the training kernels. */
+int nb_mlp_grad(nb_ctx* ctx, const float* features, const float* targets, uint64_t count,
+                float* grad, float* loss);
+
+/* Network outputs for `count` input rows on MFMA: out [count][2]. */
+int nb_mlp_forward(nb_ctx* ctx, const float* features, uint64_t count, float* out);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,10 @@ namespace {
 
 thread_local std::string g_err;
 
+// Initial accepted-cell region per group of 64 bodies (learned far field);
+// a step that needs more halts, and the region grows to 1.25x the need.
+constexpr uint32_t kInitialCells = 1024;
+
 int fail(nb_ctx* c, int code, const std::string& msg) {
     if (c)
         c->err = msg;
@@ -99,6 +103,7 @@ void destroy_graphs(nb_ctx* c) {
 void free_all(nb_ctx* c) {
     destroy_graphs(c);
     free_tree(c);
+    free_mlp(c);
     Buffers& b = c->b;
     for (int k = 0; k < 2; k++) {
         dfree(b.pos[k]);
@@ -149,8 +154,9 @@ int check_params(const nb_params* p, std::string* why) {
     if (p->group_budget < 0) return *why = "group_budget must be >= 0", NB_ERR_INVALID_ARG;
     if (p->far_field != NB_FAR_MONOPOLE && p->far_field != NB_FAR_LEARNED)
         return *why = "unknown far_field", NB_ERR_INVALID_ARG;
-    if (p->far_field == NB_FAR_LEARNED)
-        return *why = "far_field=LEARNED needs the MLP correction, not built in this library",
+    if (p->far_field == NB_FAR_LEARNED &&
+        (p->algorithm != NB_ALGO_BARNES_HUT || p->traversal != NB_TRAV_GROUP))
+        return *why = "far_field=LEARNED needs algorithm=BARNES_HUT and traversal=GROUP",
                NB_ERR_NOT_SUPPORTED;
     return NB_OK;
 }
@@ -257,10 +263,20 @@ int read_meta(nb_ctx* c) {
     return NB_OK;
 }
 
-int grow_tree(nb_ctx* c, uint64_t needed) {
-    const uint64_t icap = needed + needed / 4 + 1024;
-    int rc = alloc_tree(c, icap);
-    if (rc) return rc;
+// Grows whichever buffer the halted step overflowed.
+int grow_for(nb_ctx* c, uint32_t flags) {
+    if (flags & FLAG_TREE_CAPACITY) {
+        const uint64_t icap = c->last_internal + c->last_internal / 4 + 1024;
+        int rc = alloc_tree(c, icap);
+        if (rc) return rc;
+    }
+    if (flags & FLAG_CELL_CAPACITY) {
+        const uint64_t need = c->host_meta[META_CELLS_NEEDED];
+        const uint64_t cap = need + need / 4 + 64;
+        if (cap > 0xffffffffull) return fail(c, NB_ERR_OUT_OF_MEMORY, "accepted-cell region");
+        int rc = alloc_cells(c, (uint32_t)cap);
+        if (rc) return rc;
+    }
     return build_graphs(c);
 }
 
@@ -290,14 +306,18 @@ int build_and_force(nb_ctx* c, bool forces) {
         c->last_internal = c->host_meta[META_INTERNAL];
         if (flags & FLAG_NONFINITE) return fail(c, NB_ERR_NUMERIC, "non-finite body positions");
         if (flags & FLAG_STACK) return fail(c, NB_ERR_OVERFLOW, "traversal stack overflow");
-        if (!(flags & FLAG_TREE_CAPACITY)) return NB_OK;
-        rc = grow_tree(c, c->last_internal);
+        if (!(flags & (FLAG_TREE_CAPACITY | FLAG_CELL_CAPACITY))) return NB_OK;
+        rc = grow_for(c, flags);
         if (rc) return rc;
     }
     return fail(c, NB_ERR_HIP, "tree capacity did not converge");
 }
 
 }  // namespace
+
+namespace nb {
+int build_tree(nb_ctx* c) { return build_and_force(c, false); }
+}  // namespace nb
 
 extern "C" {
 
@@ -406,6 +426,14 @@ int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* 
 #undef NB_TRY
     rc = upload_consts(c);
     if (rc) return bail(rc);
+    if (c->p.far_field == NB_FAR_LEARNED && n) {
+        // the graphs capture the network's buffers: allocate them (zero
+        // weights, i.e. no correction) before the first launch
+        rc = prepare_mlp(c);
+        if (rc) return bail(rc);
+        rc = alloc_cells(c, kInitialCells);
+        if (rc) return bail(rc);
+    }
     if (bh && n) {
         rc = alloc_tree(c, std::max<uint64_t>(n, 1024));
         if (rc) return bail(rc);
@@ -465,8 +493,8 @@ int nb_step(nb_ctx* c, uint64_t k) {
         remaining -= done;
         if (flags & FLAG_NONFINITE) return fail(c, NB_ERR_NUMERIC, "non-finite body positions");
         if (flags & FLAG_STACK) return fail(c, NB_ERR_OVERFLOW, "traversal stack overflow");
-        if (flags & FLAG_TREE_CAPACITY) {
-            rc = grow_tree(c, c->last_internal);
+        if (flags & (FLAG_TREE_CAPACITY | FLAG_CELL_CAPACITY)) {
+            rc = grow_for(c, flags);
             if (rc) return rc;
             if (is_dk(c) && remaining > 0) {
                 // The halted step drifted but did not kick: finish it.
@@ -476,8 +504,8 @@ int nb_step(nb_ctx* c, uint64_t k) {
                 NB_HIP(c, hipGetLastError());
                 rc = read_meta(c);
                 if (rc) return rc;
-                if (c->host_meta[META_FLAGS] & FLAG_TREE_CAPACITY)
-                    return fail(c, NB_ERR_HIP, "tree capacity did not converge");
+                if (c->host_meta[META_FLAGS] & (FLAG_TREE_CAPACITY | FLAG_CELL_CAPACITY))
+                    return fail(c, NB_ERR_HIP, "buffer capacity did not converge");
                 c->iteration += 1;
                 c->sim_time += c->dt;
                 remaining -= 1;

@@ -43,7 +43,6 @@ __device__ __forceinline__ void interact(Acc& a, double dx, double dy, double d2
 
 constexpr int kTravThreads = 256;
 constexpr int kTravWaves = kTravThreads / 64;
-constexpr uint32_t kHaltFlags = FLAG_TREE_CAPACITY | FLAG_NONFINITE;
 
 // Wavefront-cooperative traversal. The stack entries are (slot, lane mask):
 // the mask holds the lanes that still need the subtree. A popped internal
@@ -191,13 +190,21 @@ __device__ __forceinline__ void eval_list(const float4* __restrict__ list, int c
     kahan_add(a.ay, a.cy, py);
 }
 
-template <bool KICK, bool ZERO_EPS, bool GUARD>
+// LEARN: the walk leaves the far-field network to k_mlp_apply. Every
+// accepted internal cell is also written to the group's region of `cells`
+// (offset from its subgroup's centre, mass, side, second moments, subgroup
+// id = first lane of the subgroup); each member records its subgroup and its
+// offset in units of the subgroup's half-extent, and the raw acceleration
+// (without G) goes to acc_out instead of a kick.
+template <bool KICK, bool ZERO_EPS, bool GUARD, bool LEARN>
 __global__ __launch_bounds__(64) void k_traverse_group(
     const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
     const double* __restrict__ mass, double2* __restrict__ vel, double2* __restrict__ acc_out,
     uint32_t* __restrict__ inter_out, uint32_t n, const StepConsts* __restrict__ k,
     uint32_t* __restrict__ meta, int count_step, int budget,
-    const uint64_t* __restrict__ keys) {
+    const uint64_t* __restrict__ keys, CellRec* __restrict__ cells,
+    uint32_t* __restrict__ cell_cnt, uint32_t cell_cap, float2* __restrict__ sub_h,
+    float4* __restrict__ body_w) {
     if (meta[META_FLAGS] & kHaltFlags) return;
     __shared__ __attribute__((aligned(16))) int32_t s_stack[kGStack];
     __shared__ float4 s_list[kGList];
@@ -223,6 +230,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     // passes the budget is split into halves (a single body is never split:
     // its group test is its own opening test).
     int nt = 1;
+    uint32_t cbase = 0;  // cells of the finished subgroups (LEARN)
     if (l == 0) s_todo[0] = 64;
     while (nt > 0) {
         const int32_t e = readfirst(s_todo[--nt]);
@@ -286,6 +294,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
         const bool capped = cn > 1;
         Acc t;
         int sp = 1, cnt = 0;
+        uint32_t ccnt = 0;  // cells of this subgroup written after cbase
         uint32_t listed = 0;
         bool own_multi = false, split = false;
         if (l == 0) s_stack[0] = 0;
@@ -326,6 +335,20 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                 s_list[at] = make_float4((float)dxc, (float)dyc, r.mf, 0.f);
             }
             cnt += (int)__popcll(am);
+            if (LEARN) {
+                const bool cell = add && c0 >= 0;
+                const uint64_t cmk = ballot64(cell);
+                if (cell) {
+                    const uint32_t at = cbase + ccnt + (uint32_t)__popcll(cmk & lanemask_lt());
+                    if (at < cell_cap) {
+                        CellRec cr;
+                        cr.a = make_float4((float)dxc, (float)dyc, r.mf, (float)sqrt(r.wsq));
+                        cr.b = make_float4(r.ixx, r.ixy, r.iyy, __int_as_float(f));
+                        cells[(uint64_t)g * cell_cap + at] = cr;
+                    }
+                }
+                ccnt += (uint32_t)__popcll(cmk);
+            }
             // push the non-empty children of opened cells
             const int32_t cm = open ? r.mask : 0;
             const uint32_t nc = (uint32_t)__popc(cm);
@@ -376,6 +399,15 @@ __global__ __launch_bounds__(64) void k_traverse_group(
             eval_list<ZERO_EPS>(s_list, cnt, bx, by, eps2f, t);
             listed += cnt;
         }
+        if (LEARN) {
+            cbase += ccnt;
+            if (member) {
+                const float wx = hx > 0.0 ? bx / (float)hx : 0.f;
+                const float wy = hy > 0.0 ? by / (float)hy : 0.f;
+                body_w[i] = make_float4(wx, wy, __int_as_float(f), 0.f);
+            }
+            if (l == 0) sub_h[g * 64 + f] = make_float2((float)hx, (float)hy);
+        }
         if (member) {
             fin = t;
             fin_listed = listed;
@@ -383,10 +415,20 @@ __global__ __launch_bounds__(64) void k_traverse_group(
         }
     }
     if (overflow && l == 0) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_STACK);
+    if (LEARN && l == 0) {
+        cell_cnt[g] = cbase;
+        if (cbase > cell_cap) {
+            atomicMax(&meta[META_CELLS_NEEDED], cbase);
+            atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_CELL_CAPACITY);
+        }
+    }
     if (!valid) return;
     const double ax = (double)fin.ax - (double)fin.cx;
     const double ay = (double)fin.ay - (double)fin.cy;
-    if (KICK) {
+    if (LEARN) {
+        acc_out[i] = make_double2(ax, ay);
+        if (!KICK) inter_out[i] = fin_listed - (fin_own_multi ? 0u : 1u) + fin.n;
+    } else if (KICK) {
         const double gdt = k->G_dt;
         double2 v = vel[i];
         v.x += ax * gdt;
@@ -408,19 +450,24 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
         const bool z = !(c->p.softening > 0.0);
         const bool gd = c->p.self_guard != 0;
         const int budget = c->p.group_budget > 0 ? c->p.group_budget : kGBudget;
-#define NB_TRAVG(K, Z, GD)                                                                     \
-    hipLaunchKernelGGL((k_traverse_group<K, Z, GD>), dim3(groups), dim3(64), 0, c->stream,     \
+        const bool learn = c->p.far_field == NB_FAR_LEARNED;
+#define NB_TRAVG(K, Z, GD, LN)                                                                 \
+    hipLaunchKernelGGL((k_traverse_group<K, Z, GD, LN>), dim3(groups), dim3(64), 0, c->stream, \
                        c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src], c->b.acc,     \
-                       c->b.inter, n, c->b.consts, c->b.meta, (int)count_step, budget,     \
-                       c->b.keys[0])
-#define NB_TRAVG2(K, Z) if (gd) NB_TRAVG(K, Z, true); else NB_TRAVG(K, Z, false)
+                       c->b.inter, n, c->b.consts, c->b.meta, (int)(count_step && !learn),     \
+                       budget, c->b.keys[0], c->b.cells, c->b.cell_cnt, c->cell_cap, c->b.sub_h,       \
+                       c->b.body_w)
+#define NB_TRAVG3(K, Z, GD) if (learn) NB_TRAVG(false, Z, GD, true); else NB_TRAVG(K, Z, GD, false)
+#define NB_TRAVG2(K, Z) if (gd) { NB_TRAVG3(K, Z, true); } else { NB_TRAVG3(K, Z, false); }
         if (kick) {
             if (z) { NB_TRAVG2(true, true); } else { NB_TRAVG2(true, false); }
         } else {
             if (z) { NB_TRAVG2(false, true); } else { NB_TRAVG2(false, false); }
         }
+#undef NB_TRAVG3
 #undef NB_TRAVG2
 #undef NB_TRAVG
+        if (learn) launch_mlp_apply(c, src, kick, count_step);
         return;
     }
     const uint32_t blocks = (n + kTravThreads - 1) / kTravThreads;

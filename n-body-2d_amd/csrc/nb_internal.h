@@ -18,6 +18,7 @@
 #include <string>
 
 #include "../../include/nbody.h"
+#include "nb_mlp.h"
 
 namespace nb {
 
@@ -36,13 +37,17 @@ enum Meta : int {
     META_FLAGS = 1,      // FLAG_* bits of the current step
     META_STEPS_DONE = 2, // completed steps since the host last cleared it
     META_MULTI_LEAVES = 3, // multi-body leaves queued by the current build
+    META_CELLS_NEEDED = 4, // most accepted cells of one group (learned far field)
     META_COUNT = 8
 };
 enum Flag : uint32_t {
     FLAG_TREE_CAPACITY = 1u,  // internal nodes exceed the allocated capacity
     FLAG_STACK = 2u,          // traversal stack overflow
     FLAG_NONFINITE = 4u,      // non-finite bounding box
+    FLAG_CELL_CAPACITY = 8u,  // a group's accepted cells exceed its region
 };
+// Flags that stop the rest of a step (the host grows the buffer and re-runs).
+constexpr uint32_t kHaltFlags = FLAG_TREE_CAPACITY | FLAG_NONFINITE | FLAG_CELL_CAPACITY;
 
 // One tree slot, 64 bytes so one wave-uniform node fetch is a single 64-byte
 // scalar load.
@@ -57,6 +62,13 @@ struct __attribute__((aligned(64))) NodeRec {
     float mf;        // mass as FP32 for the pair arithmetic
 };
 static_assert(sizeof(NodeRec) == 64, "NodeRec must be 64 bytes");
+
+// One accepted cell waiting for the far-field network: a = {dx, dy, M, s}
+// (offset COM - subgroup centre, mass, side), b = {Ixx, Ixy, Iyy, subgroup id
+// as int bits}.
+struct CellRec {
+    float4 a, b;
+};
 
 // Device-side constants of a step, read by the kernels so set_dt needs no
 // graph rebuild.
@@ -95,6 +107,24 @@ struct Buffers {
     double2* ap_part;      // all-pairs partial sums [split][N]
     uint32_t* meta;        // [META_COUNT]
     StepConsts* consts;
+    // far-field network (allocated on first use)
+    float* mlp_w;          // [4866] FP32 master parameters, canonical order
+    float* mlp_m;          // [4866] Adam first moments
+    float* mlp_v;          // [4866] Adam second moments
+    uint2* mlp_tiles;      // [24*64] BF16 A tiles, per-lane MFMA layout
+    float* mlp_bias;       // [144] b1, b2, b3, zero padding
+    float* mlp_gpart;      // [parts][4866] per-workgroup partial gradients
+    float* mlp_loss;       // [parts]
+    uint32_t* pair_body;   // [pair_cap] training pairs: sample body
+    uint32_t* pair_slot;   // [pair_cap] training pairs: cell slot
+    float* pair_x;         // [pair_cap][8] features
+    float* pair_t;         // [pair_cap][2] targets
+    unsigned long long* pair_count;
+    // learned far field in the step: per group of 64 bodies
+    CellRec* cells;        // [groups][cell_cap] accepted cells
+    uint32_t* cell_cnt;    // [groups]
+    float2* sub_h;         // [groups][64] half-extents of the subgroups
+    float4* body_w;        // [N] {bx/hx, by/hy, subgroup id, -}
 };
 
 }  // namespace nb
@@ -122,6 +152,11 @@ struct nb_ctx {
     double dt = 0.0;
     uint64_t last_internal = 0;
     uint32_t* host_meta = nullptr;  // pinned copy of meta
+    uint64_t pair_cap = 0;          // training-pair capacity
+    uint64_t pair_count = 0;        // pairs held
+    uint32_t train_parts = 0;       // partial-gradient rows allocated
+    uint32_t train_step = 0;        // Adam step counter
+    uint32_t cell_cap = 0;          // accepted-cell region per group
     std::string err;
 };
 
@@ -141,6 +176,14 @@ void launch_tree_build(nb_ctx* c, int src);
 void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step);
 void launch_allpairs(nb_ctx* c, int src, bool kick, bool count_step);
 void launch_drift(nb_ctx* c, int src, bool check, bool count_step);
+// ---- far-field network (nb_mlp.hip) ----
+void free_mlp(nb_ctx* c);
+int prepare_mlp(nb_ctx* c);
+int alloc_cells(nb_ctx* c, uint32_t cap);
+void launch_mlp_apply(nb_ctx* c, int src, bool kick, bool count_step);
+// Builds the tree of the current state (nb_capi.hip).
+int build_tree(nb_ctx* c);
+
 hipError_t direct_sum_f64(uint64_t n, const double2* pos, const double* mass, double eps2,
                           uint64_t m, const int64_t* targets, double2* part, uint32_t split,
                           hipStream_t s);

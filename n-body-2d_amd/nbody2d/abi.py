@@ -36,8 +36,12 @@ EXPORTS = (
     "nb_params_default", "nb_create", "nb_step", "nb_get_state", "nb_compute_forces",
     "nb_set_dt", "nb_get_info", "nb_set_timing", "nb_get_timing", "nb_destroy",
     "nb_last_error", "nb_status_string", "nb_debug_build", "nb_debug_tree",
-    "nb_direct_sum_f64",
+    "nb_direct_sum_f64", "nb_mlp_set_weights", "nb_mlp_get_weights", "nb_mlp_init",
+    "nb_mlp_generate", "nb_mlp_get_pairs", "nb_mlp_set_pairs", "nb_mlp_train", "nb_mlp_grad",
+    "nb_mlp_forward",
 )
+MLP_PARAMS = 4866
+MLP_INPUTS = 8
 
 
 class Params(C.Structure):
@@ -93,6 +97,17 @@ def lib() -> C.CDLL:
         L.nb_debug_tree.argtypes = [P, P, C.c_uint64]
         L.nb_direct_sum_f64.argtypes = [C.c_uint64, P, P, C.c_double, C.c_double, C.c_uint64,
                                         P, P]
+        L.nb_mlp_set_weights.argtypes = [P, P, C.c_uint64]
+        L.nb_mlp_get_weights.argtypes = [P, P, C.c_uint64]
+        L.nb_mlp_init.argtypes = [P, C.c_uint64]
+        L.nb_mlp_generate.argtypes = [P, C.c_double, C.c_uint32, C.c_uint64, C.c_uint64,
+                                      C.POINTER(C.c_uint64)]
+        L.nb_mlp_get_pairs.argtypes = [P, P, P, P, P, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.nb_mlp_set_pairs.argtypes = [P, P, P, C.c_uint64]
+        L.nb_mlp_train.argtypes = [P, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64,
+                                   C.POINTER(C.c_float)]
+        L.nb_mlp_grad.argtypes = [P, P, P, C.c_uint64, P, C.POINTER(C.c_float)]
+        L.nb_mlp_forward.argtypes = [P, P, C.c_uint64, P]
         del dp
         _lib = L
     return _lib
@@ -198,6 +213,65 @@ class Context:
         nodes = np.zeros(count, dtype=NODE_DTYPE)
         self._check(lib().nb_debug_tree(self._h, _ptr(nodes), count))
         return nodes
+
+
+    # ---- learned far field ----
+    def mlp_set_weights(self, w) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._check(lib().nb_mlp_set_weights(self._h, _ptr(w), w.size))
+
+    def mlp_weights(self) -> np.ndarray:
+        w = np.empty(MLP_PARAMS, dtype=np.float32)
+        self._check(lib().nb_mlp_get_weights(self._h, _ptr(w), w.size))
+        return w
+
+    def mlp_init(self, seed: int = 0) -> None:
+        self._check(lib().nb_mlp_init(self._h, int(seed)))
+
+    def mlp_generate(self, theta: float, sample_bodies: int, max_pairs: int,
+                     seed: int = 0) -> int:
+        k = C.c_uint64()
+        self._check(lib().nb_mlp_generate(self._h, float(theta), int(sample_bodies),
+                                          int(max_pairs), int(seed), C.byref(k)))
+        return int(k.value)
+
+    def mlp_pairs(self, cap: int | None = None):
+        cnt = C.c_uint64()
+        self._check(lib().nb_mlp_get_pairs(self._h, None, None, None, None, 0, C.byref(cnt)))
+        k = int(cnt.value) if cap is None else min(int(cap), int(cnt.value))
+        x = np.empty((k, MLP_INPUTS), dtype=np.float32)
+        t = np.empty((k, 2), dtype=np.float32)
+        body = np.empty(k, dtype=np.uint32)
+        cell = np.empty(k, dtype=np.uint32)
+        self._check(lib().nb_mlp_get_pairs(self._h, _ptr(x), _ptr(t), _ptr(body), _ptr(cell),
+                                           k, C.byref(cnt)))
+        return x, t, body, cell
+
+    def mlp_set_pairs(self, x, t) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        self._check(lib().nb_mlp_set_pairs(self._h, _ptr(x), _ptr(t), x.shape[0]))
+
+    def mlp_train(self, steps: int, batch: int, lr: float, seed: int = 0) -> float:
+        loss = C.c_float()
+        self._check(lib().nb_mlp_train(self._h, int(steps), int(batch), float(lr), int(seed),
+                                       C.byref(loss)))
+        return float(loss.value)
+
+    def mlp_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(MLP_PARAMS, dtype=np.float32)
+        loss = C.c_float()
+        self._check(lib().nb_mlp_grad(self._h, _ptr(x), _ptr(t), x.shape[0], _ptr(g),
+                                      C.byref(loss)))
+        return g, float(loss.value)
+
+    def mlp_forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty((x.shape[0], 2), dtype=np.float32)
+        self._check(lib().nb_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(y)))
+        return y
 
 
 def direct_sum_f64(pos, mass, G: float, softening: float, targets) -> np.ndarray:

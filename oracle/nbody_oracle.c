@@ -916,3 +916,178 @@ void or_direct_sum(uint64_t n, const double* pos, const double* mass, double eps
     free(tid);
     free(args);
 }
+
+/* =====================================================================
+ * Part D — far-field MLP, FP64 (checker of the GPU forward/backward)
+ * ===================================================================== */
+
+static double bf16_round(double v) {
+    float f = (float)v;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) != 0x7f800000u) { /* finite: round to nearest even */
+        const uint32_t lsb = (u >> 16) & 1u;
+        u += 0x7fffu + lsb;
+    }
+    u &= 0xffff0000u;
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+typedef struct {
+    const double *W1, *b1, *W2, *b2, *W3, *b3;
+} mlp_view;
+
+static mlp_view mlp_split(const double* p) {
+    mlp_view v;
+    v.W1 = p;
+    v.b1 = v.W1 + 64 * 8;
+    v.W2 = v.b1 + 64;
+    v.b2 = v.W2 + 64 * 64;
+    v.W3 = v.b2 + 64;
+    v.b3 = v.W3 + 2 * 64;
+    return v;
+}
+
+/* one sample: returns pre-activations and activations of both hidden layers */
+static void mlp_fwd1(const mlp_view* v, const float* x, int bf16, double* z1, double* h1,
+                     double* z2, double* h2, double* y) {
+    double xin[8];
+    for (int k = 0; k < 8; k++) xin[k] = bf16 ? bf16_round(x[k]) : x[k];
+    for (int j = 0; j < 64; j++) {
+        double s = v->b1[j];
+        for (int k = 0; k < 8; k++) s += (bf16 ? bf16_round(v->W1[j * 8 + k]) : v->W1[j * 8 + k]) * xin[k];
+        z1[j] = s;
+        h1[j] = s > 0 ? s : 0.0;
+        if (bf16) h1[j] = bf16_round(h1[j]);
+    }
+    for (int j = 0; j < 64; j++) {
+        double s = v->b2[j];
+        for (int k = 0; k < 64; k++) s += (bf16 ? bf16_round(v->W2[j * 64 + k]) : v->W2[j * 64 + k]) * h1[k];
+        z2[j] = s;
+        h2[j] = s > 0 ? s : 0.0;
+        if (bf16) h2[j] = bf16_round(h2[j]);
+    }
+    for (int o = 0; o < 2; o++) {
+        double s = v->b3[o];
+        for (int k = 0; k < 64; k++) s += (bf16 ? bf16_round(v->W3[o * 64 + k]) : v->W3[o * 64 + k]) * h2[k];
+        y[o] = s;
+    }
+}
+
+double or_mlp_forward(const double* params, uint64_t count, const float* x, float* y, int bf16) {
+    mlp_view v = mlp_split(params);
+    double z1[64], h1[64], z2[64], h2[64], out[2], sum = 0.0;
+    for (uint64_t s = 0; s < count; s++) {
+        mlp_fwd1(&v, x + 8 * s, bf16, z1, h1, z2, h2, out);
+        y[2 * s] = (float)out[0];
+        y[2 * s + 1] = (float)out[1];
+        sum += out[0] + out[1];
+    }
+    return sum;
+}
+
+/* Gradient of L = (1/count) sum_s sum_o (y_so - t_so)^2 with respect to the
+ * parameters (canonical order), by back-propagation; weights used as rounded
+ * in the forward (straight-through for the bf16 rounding). Returns L. */
+double or_mlp_grad(const double* params, uint64_t count, const float* x, const float* t,
+                   double* grad, int bf16) {
+    mlp_view v = mlp_split(params);
+    memset(grad, 0, sizeof(double) * OR_MLP_PARAMS);
+    double* gW1 = grad;
+    double* gb1 = gW1 + 64 * 8;
+    double* gW2 = gb1 + 64;
+    double* gb2 = gW2 + 64 * 64;
+    double* gW3 = gb2 + 64;
+    double* gb3 = gW3 + 2 * 64;
+    double z1[64], h1[64], z2[64], h2[64], y[2], loss = 0.0;
+    for (uint64_t s = 0; s < count; s++) {
+        const float* xs = x + 8 * s;
+        mlp_fwd1(&v, xs, bf16, z1, h1, z2, h2, y);
+        double dy[2];
+        for (int o = 0; o < 2; o++) {
+            const double e = y[o] - t[2 * s + o];
+            loss += e * e;
+            dy[o] = 2.0 * e / (double)count;
+        }
+        double dh2[64], dh1[64];
+        for (int j = 0; j < 64; j++) {
+            double g = 0;
+            for (int o = 0; o < 2; o++) g += (bf16 ? bf16_round(v.W3[o * 64 + j]) : v.W3[o * 64 + j]) * dy[o];
+            dh2[j] = z2[j] > 0 ? g : 0.0;
+        }
+        for (int k = 0; k < 64; k++) {
+            double g = 0;
+            for (int j = 0; j < 64; j++) g += (bf16 ? bf16_round(v.W2[j * 64 + k]) : v.W2[j * 64 + k]) * dh2[j];
+            dh1[k] = z1[k] > 0 ? g : 0.0;
+        }
+        for (int o = 0; o < 2; o++) {
+            gb3[o] += dy[o];
+            for (int j = 0; j < 64; j++) gW3[o * 64 + j] += dy[o] * h2[j];
+        }
+        for (int j = 0; j < 64; j++) {
+            gb2[j] += dh2[j];
+            for (int k = 0; k < 64; k++) gW2[j * 64 + k] += dh2[j] * h1[k];
+        }
+        for (int j = 0; j < 64; j++) {
+            gb1[j] += dh1[j];
+            for (int k = 0; k < 8; k++) gW1[j * 8 + k] += dh1[j] * (bf16 ? bf16_round(xs[k]) : xs[k]);
+        }
+    }
+    return loss / (double)count;
+}
+
+/* Second-order term of sum_j m_j (d + u_j)/(|d + u_j|^2 + eps^2)^{3/2} about
+ * the COM (sum m_j u_j = 0), with I = sum m_j u_j u_j^T and R^2 = |d|^2 + eps^2:
+ *   a_c = (3/(2 R^5)) [ -tr(I) d_c - 2 (I d)_c + 5 (d.I d) d_c / R^2 ],
+ * d = com - x (the Taylor expansion of the Plummer kernel to second order;
+ * pinned by the order of the residual against direct sums in the tests). */
+void or_quadrupole(double dx, double dy, double ixx, double ixy, double iyy, double eps2,
+                   double* ax, double* ay) {
+    const double R2 = dx * dx + dy * dy + eps2, R = sqrt(R2), R5 = R2 * R2 * R;
+    const double tr = ixx + iyy;
+    const double idx = ixx * dx + ixy * dy, idy = ixy * dx + iyy * dy;
+    const double dId = dx * idx + dy * idy;
+    const double k = 1.5 / R5;
+    *ax = k * (-tr * dx - 2.0 * idx + 5.0 * dId * dx / R2);
+    *ay = k * (-tr * dy - 2.0 * idy + 5.0 * dId * dy / R2);
+}
+
+/* The network inputs of a (cell, point) pair, FP64; returns the output scale
+ * M s^2 r / R^5. d = com - point, ell = eps (or the root side when eps = 0). */
+double or_mlp_features(double dx, double dy, double s, double M, double ixx, double ixy,
+                       double iyy, double eps2, double m_root, double ell, double* f) {
+    const double r2 = dx * dx + dy * dy, r = sqrt(r2), R = sqrt(r2 + eps2);
+    f[0] = dx / r;
+    f[1] = dy / r;
+    f[2] = s / R;
+    f[3] = log2(M / m_root) / 16.0;
+    f[4] = ixx / (M * s * s);
+    f[5] = ixy / (M * s * s);
+    f[6] = iyy / (M * s * s);
+    f[7] = log2(r / ell) / 8.0;
+    return M * s * s * r / (R * R * R * R * R);
+}
+
+/* Exact Plummer acceleration of bodies [bs, be] at (x, y) minus the monopole
+ * of their total mass at their COM (both FP64, G excluded). */
+void or_cell_correction(const double* pos, const double* mass, int64_t bs, int64_t be, double x,
+                        double y, double eps2, double* out) {
+    double ax = 0.0, ay = 0.0, M = 0.0, cx = 0.0, cy = 0.0;
+    for (int64_t j = bs; j <= be; j++) {
+        const double dx = pos[2 * j] - x, dy = pos[2 * j + 1] - y;
+        const double r2 = dx * dx + dy * dy + eps2;
+        const double w = mass[j] / (r2 * sqrt(r2));
+        ax += dx * w;
+        ay += dy * w;
+        M += mass[j];
+        cx += mass[j] * pos[2 * j];
+        cy += mass[j] * pos[2 * j + 1];
+    }
+    cx /= M;
+    cy /= M;
+    const double dx = cx - x, dy = cy - y, r2 = dx * dx + dy * dy + eps2;
+    const double w = M / (r2 * sqrt(r2));
+    out[0] = ax - dx * w;
+    out[1] = ay - dy * w;
+}

@@ -120,4 +120,31 @@ void or_direct_sum(uint64_t n, const double* pos, const double* mass, double eps
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* ---------------- Part D: far-field MLP (8 -> 64 -> 64 -> 2, ReLU) ----------------
+ * Parameters in the canonical order W1[64][8], b1[64], W2[64][64], b2[64],
+ * W3[2][64], b3[2] (4,866 values). Loss = mean over the batch of the squared
+ * error summed over the 2 outputs. bf16 = 1 rounds weights and hidden
+ * activations to bfloat16 (round to nearest even) as the GPU does. */
+#define OR_MLP_PARAMS 4866
+double or_mlp_forward(const double* params, uint64_t count, const float* x, float* y, int bf16);
+double or_mlp_grad(const double* params, uint64_t count, const float* x, const float* t,
+                   double* grad, int bf16);
+/* Softened quadrupole term of a cell (second moments about its COM) at the
+ * point at offset d = com - x: (3/(2R^5))[-tr(I)d - 2Id + 5(d.Id)d/R^2],
+ * R^2 = |d|^2 + eps2. */
+void or_quadrupole(double dx, double dy, double ixx, double ixy, double iyy, double eps2,
+                   double* ax, double* ay);
+/* Network inputs f[8] of a (cell, point) pair; returns the output scale. */
+double or_mlp_features(double dx, double dy, double s, double M, double ixx, double ixy,
+                       double iyy, double eps2, double m_root, double ell, double* f);
+/* Exact minus monopole acceleration of bodies [bs, be] at (x, y). */
+void or_cell_correction(const double* pos, const double* mass, int64_t bs, int64_t be, double x,
+                        double y, double eps2, double* out);
+#ifdef __cplusplus
+}
+#endif
 #endif

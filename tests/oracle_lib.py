@@ -53,6 +53,14 @@ def load(fast: bool = False) -> C.CDLL:
     L.or_tree_forces.argtypes = [u64, _P, _P, _P, d, d, i, _P, _P]
     L.or_model_step.argtypes = [u64, _P, _P, _P, _P, d, d, d, d, i]
     L.or_direct_sum.argtypes = [u64, _P, _P, d, u64, _P, i, _P]
+    L.or_mlp_forward.argtypes = [_P, u64, _P, _P, i]
+    L.or_mlp_forward.restype = d
+    L.or_mlp_grad.argtypes = [_P, u64, _P, _P, _P, i]
+    L.or_mlp_grad.restype = d
+    L.or_quadrupole.argtypes = [d, d, d, d, d, d, C.POINTER(d), C.POINTER(d)]
+    L.or_mlp_features.argtypes = [d, d, d, d, d, d, d, d, d, d, _P]
+    L.or_mlp_features.restype = d
+    L.or_cell_correction.argtypes = [_P, _P, i64, i64, d, d, d, _P]
     _libs[name] = L
     return L
 
@@ -199,3 +207,47 @@ def refcpu_allpairs_run(pos, vel, mass, G, dt, eps2, steps, kernel=FORCE_REF_CPU
     load().or_refcpu_allpairs_run(len(m), ptr(p), ptr(v), ptr(m), float(G), float(dt),
                                   float(eps2), int(kernel), int(steps))
     return p, v
+
+
+# ---- Part D: far-field network ----
+MLP_PARAMS = 4866
+
+
+def mlp_forward(params, x, bf16=True):
+    w = c64(params)
+    xs = c64(x, np.float32).reshape(-1, 8)
+    y = np.empty((len(xs), 2), dtype=np.float32)
+    load().or_mlp_forward(ptr(w), len(xs), ptr(xs), ptr(y), int(bool(bf16)))
+    return y
+
+
+def mlp_grad(params, x, t, bf16=True):
+    w = c64(params)
+    xs = c64(x, np.float32).reshape(-1, 8)
+    ts = c64(t, np.float32).reshape(-1, 2)
+    g = np.empty(MLP_PARAMS)
+    loss = load().or_mlp_grad(ptr(w), len(xs), ptr(xs), ptr(ts), ptr(g), int(bool(bf16)))
+    return g, loss
+
+
+def quadrupole(d, I, eps2=0.0):
+    ax, ay = C.c_double(), C.c_double()
+    load().or_quadrupole(float(d[0]), float(d[1]), float(I[0]), float(I[1]), float(I[2]),
+                         float(eps2), C.byref(ax), C.byref(ay))
+    return np.array([ax.value, ay.value])
+
+
+def mlp_features(d, s, M, I, eps2, m_root, ell):
+    f = np.empty(8)
+    scale = load().or_mlp_features(float(d[0]), float(d[1]), float(s), float(M), float(I[0]),
+                                   float(I[1]), float(I[2]), float(eps2), float(m_root),
+                                   float(ell), ptr(f))
+    return f, scale
+
+
+def cell_correction(pos, mass, bs, be, x, eps2):
+    p, m = c64(pos), c64(mass)
+    out = np.empty(2)
+    load().or_cell_correction(ptr(p), ptr(m), int(bs), int(be), float(x[0]), float(x[1]),
+                              float(eps2), ptr(out))
+    return out

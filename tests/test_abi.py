@@ -52,11 +52,13 @@ def test_create_rejects_invalid_params(field, value):
     assert e.value.status == abi.NB_ERR_INVALID_ARG
 
 
-def test_learned_far_field_not_supported_yet_is_an_error():
+@pytest.mark.parametrize("kw", [dict(traversal=abi.TRAV_EXACT),
+                                dict(algorithm=abi.ALGO_ALL_PAIRS, traversal=abi.TRAV_GROUP)])
+def test_learned_far_field_needs_group_barnes_hut(kw):
     m = np.ones(4)
     x = np.zeros((4, 2))
     with pytest.raises(abi.NBodyError) as e:
-        abi.Context(m, x, x, far_field=abi.FAR_LEARNED)
+        abi.Context(m, x, x, far_field=abi.FAR_LEARNED, **kw)
     assert e.value.status == abi.NB_ERR_NOT_SUPPORTED
 
 

