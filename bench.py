@@ -1,17 +1,27 @@
-"""Benchmark: Barnes-Hut body-steps/s at N = 1,048,576 on one MI355X.
+"""Benchmark: N = 1,048,576 body-steps/s on one MI355X, plain Barnes-Hut at
+theta = 0.5 against Barnes-Hut with the learned far-field correction at
+theta = 0.8, each with its RMS force error against the FP64 direct sum.
 
 Workload: one exponential disk of 1,048,576 bodies (central mass 1e4, radii
 Exp(1) truncated at 5, lognormal masses, circular velocities + 5 %
-dispersion), G = 1, softening 0.01, dt = 1e-3, plain Barnes-Hut at theta = 0.5
-(FP32 pair forces with Kahan sums, FP64 state and leapfrog). One step = bbox,
-Morton keys, radix sort, gather, tree build, traversal + kick, drift, replayed
-as a hipGraph with all bodies resident in HBM.
+dispersion), G = 1, softening 0.01, dt = 1e-3 (FP32 pair forces with Kahan
+sums, FP64 state and leapfrog). One step = bbox, Morton keys, radix sort,
+gather, tree build, group traversal, [far-field network + kick], drift,
+replayed as a hipGraph with all bodies resident in HBM. The network runs once
+per accepted cell at the centre of the traversal subgroup; its variation
+across the subgroup is the softened quadrupole's Jacobian.
 
-Printed JSON line (rank 0): value = N * steps / wall time of the timed steps;
-roofline = the traversal kernel's algorithmic FP32 rate (interactions x 14
-flops / its average duration from hipEvents on the context stream) against
-the 157.3 TFLOP/s FP32 peak; cpu_baseline = the oracle's restatement of the
-reference's threaded CPU Barnes-Hut timed on this host on a bounded sample.
+Learned leg: before the timed region the network is initialised, 2M
+(cell, exact - monopole) training pairs are generated from the initial state
+by the GPU's FP64 direct sums over the cells, and the training kernels run
+6,000 Adam steps; the training time is reported, not timed as steps.
+
+Printed JSON line: value = N * steps / wall time of the timed steps of the
+learned leg (the plain leg's figures sit beside it); roofline = the traversal
+kernel's algorithmic FP32 rate (interactions x 14 flops / its average
+duration from hipEvents on the context stream) against the 157.3 TFLOP/s FP32
+peak; cpu_baseline = the oracle's restatement of the reference's threaded
+CPU Barnes-Hut timed on this host on a bounded sample.
 """
 from __future__ import annotations
 
@@ -29,12 +39,17 @@ sys.path.insert(0, str(REPO / "n-body-2d_amd"))
 import numpy as np  # noqa: E402
 
 N_BODIES = 1 << 20
-THETA = 0.5
+THETA_PLAIN = 0.5
+THETA_LEARNED = 0.8
 EPS = 0.01
 DT = 1e-3
 SEED = 3
 FLOPS_PER_INTERACTION = 14
+MLP_FLOPS_PER_EVAL = 2 * (8 * 64 + 64 * 64 + 64 * 2)
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
+TRAIN_PAIRS = 1 << 22
+TRAIN_SAMPLE_BODIES = 8192
 METRIC = "N=1M body-steps/s: plain BH vs learned far-field BH at equal force error"
 
 
@@ -48,28 +63,115 @@ def traffic_from_profile(kernel: str):
     return None
 
 
-def cpu_baseline(mass, pos, vel, target_s: float = 12.0):
+def cpu_baseline(mass, pos, vel, theta, target_s: float = 12.0):
     """Reference threaded CPU Barnes-Hut (oracle restatement, reference flags)."""
     sys.path.insert(0, str(REPO / "tests"))
     import oracle_lib as O  # the cpu_baseline leg is the only bench use of oracle/
 
     share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     threads = max(1, min(share, os.cpu_count() or 1, 256))
-    rc, p1, v1, t1 = O.refcpu_bh_run(pos, vel, mass, 1.0, DT, THETA, EPS * EPS, threads, 1,
+    rc, p1, v1, t1 = O.refcpu_bh_run(pos, vel, mass, 1.0, DT, theta, EPS * EPS, threads, 1,
                                      fast=True)
     if rc != 0:
         raise RuntimeError(f"cpu baseline failed: {rc}")
     steps = max(1, min(20, int(target_s / max(t1, 1e-3))))
-    rc, _, _, secs = O.refcpu_bh_run(p1, v1, mass, 1.0, DT, THETA, EPS * EPS, threads, steps,
+    rc, _, _, secs = O.refcpu_bh_run(p1, v1, mass, 1.0, DT, theta, EPS * EPS, threads, steps,
                                      fast=True)
     return {
         "value": len(mass) * steps / secs,
         "unit": "body-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"exp_disk N={len(mass)} theta={THETA} (diagonal MAC, reference CPU force "
+        "sample": f"exp_disk N={len(mass)} theta={theta} (diagonal MAC, reference CPU force "
                   f"kernel), {steps} steps after 1 warm-up step, {threads} threads, "
                   f"{secs:.2f} s",
+    }
+
+
+def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
+    abi = nbody2d.abi
+    n = len(mass)
+    guard = 1 if theta > 0.7071 else 0
+    ctx = nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, theta=theta, softening=EPS,
+                          traversal=traversal, self_guard=guard,
+                          far_field=abi.FAR_LEARNED if learned else abi.FAR_MONOPOLE)
+    leg = {"theta": theta, "self_guard": guard,
+           "far_field": "learned" if learned else "monopole",
+           "traversal": "group" if traversal == abi.TRAV_GROUP else "exact"}
+    if learned:
+        t0 = time.perf_counter()
+        ctx.mlp_init(1)
+        pairs = ctx.mlp_generate(theta, TRAIN_SAMPLE_BODIES, TRAIN_PAIRS, seed=7)
+        t1 = time.perf_counter()
+        l1 = ctx.mlp_train(args.train_steps, 4096, 2e-3, seed=1)
+        l2 = ctx.mlp_train(args.train_steps // 2, 4096, 4e-4, seed=2)
+        t2 = time.perf_counter()
+        leg["training"] = {"pairs": pairs, "sample_bodies": TRAIN_SAMPLE_BODIES,
+                           "adam_steps": args.train_steps + args.train_steps // 2,
+                           "batch": 4096, "loss": [l1, l2],
+                           "generate_s": t1 - t0, "train_s": t2 - t1}
+        print(f"[bench] trained on {pairs} pairs: loss {l1:.3e} -> {l2:.3e} "
+              f"({t1 - t0:.2f} s + {t2 - t1:.2f} s)", flush=True)
+    # Force error on the initial state (the same positions for every leg).
+    t = ics.sample_targets(n, args.error_sample, seed=1)
+    ref = nbody2d.direct_sum_f64(pos, mass, 1.0, EPS, t)
+    acc, inter, total = ctx.forces()
+    rms, mx = ics.force_errors(acc[t], ref)
+    leg["force_error"] = {"rms": rms, "max": mx, "sample": int(len(t)), "state": "initial",
+                          "judge": "GPU FP64 direct sum"}
+    info = ctx.info()
+    leg["interactions"] = total
+    leg["interactions_per_body"] = total / n
+    leg["tree_nodes"] = int(info.tree_nodes)
+    if learned:
+        leg["far_field_cells"] = int(info.far_field_cells)
+        leg["far_field_cells_per_group"] = info.far_field_cells / ((n + 63) // 64)
+
+    ctx.step(args.warmup)
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    wall = time.perf_counter() - t0
+    leg["body_steps_per_s"] = n * args.steps / wall
+    leg["ms_per_step"] = wall * 1e3 / args.steps
+    print(f"[bench] {leg['far_field']} theta={theta}: {leg['ms_per_step']:.3f} ms/step",
+          flush=True)
+
+    # Kernel durations: the same steps launched eagerly with hipEvents around
+    # every phase on the context stream.
+    ctx.set_timing(True)
+    ctx.step(args.steps)
+    phase_ms, steps_timed = ctx.timing()
+    ctx.set_timing(False)
+    leg["phase_ms_per_step_eager"] = {k: v / steps_timed for k, v in phase_ms.items()}
+    if learned:
+        leg["far_field_cells_per_step_timed"] = int(ctx.info().far_field_cells)
+
+    # and on the state after the timed steps (each leg's own trajectory)
+    acc2, _, total2 = ctx.forces()
+    p_now, _ = ctx.state()
+    ref2 = nbody2d.direct_sum_f64(p_now, mass, 1.0, EPS, t)
+    rms2, mx2 = ics.force_errors(acc2[t], ref2)
+    ctx.close()
+    leg["force_error_after_steps"] = {"rms": rms2, "max": mx2,
+                                      "steps": 2 * args.steps + args.warmup}
+    leg["interactions_per_body_after_steps"] = total2 / n
+    return leg
+
+
+def roofline_walk(leg, traversal_name):
+    force_ms = leg["phase_ms_per_step_eager"]["force"]
+    achieved = leg["interactions"] * FLOPS_PER_INTERACTION / (force_ms * 1e-3) / 1e12
+    kernel = "k_traverse_group" if traversal_name == "group" else "k_traverse"
+    return {
+        "bound": "mfma",
+        "achieved": achieved,
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / FP32_PEAK_TFLOPS,
+        "traffic": traffic_from_profile(kernel),
+        "kernel": kernel + " (FP32 vector; 14 flops per interaction)",
+        "interactions_per_launch": leg["interactions"],
+        "avg_launch_ms": force_ms,
     }
 
 
@@ -81,8 +183,12 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=N_BODIES)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--error-sample", type=int, default=4096)
-    ap.add_argument("--traversal", choices=("group", "exact"), default="group")
-    ap.add_argument("--theta", type=float, default=THETA)
+    ap.add_argument("--legs", choices=("both", "plain", "learned"), default="both")
+    ap.add_argument("--traversal", choices=("group", "exact"), default="group",
+                    help="traversal of the plain leg (the learned leg needs group)")
+    ap.add_argument("--theta-plain", type=float, default=THETA_PLAIN)
+    ap.add_argument("--theta-learned", type=float, default=THETA_LEARNED)
+    ap.add_argument("--train-steps", type=int, default=4000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -94,81 +200,72 @@ def main() -> int:
 
     n = args.n
     mass, pos, vel = ics.exp_disk(n, seed=SEED)
-    print(f"[bench] n={n} steps={args.steps} warmup={args.warmup}", flush=True)
-    trav = nbody2d.abi.TRAV_GROUP if args.traversal == "group" else nbody2d.abi.TRAV_EXACT
-    theta = args.theta
-    ctx = nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, theta=theta, softening=EPS,
-                          traversal=trav)
-    ctx.step(args.warmup)
-    t0 = time.perf_counter()
-    ctx.step(args.steps)
-    wall = time.perf_counter() - t0
-    value = n * args.steps / wall
-    print(f"[bench] timed {args.steps} graph steps: {wall * 1e3 / args.steps:.3f} ms/step",
+    print(f"[bench] n={n} steps={args.steps} warmup={args.warmup} legs={args.legs}",
           flush=True)
+    abi = nbody2d.abi
+    legs = {}
+    if args.legs in ("both", "plain"):
+        trav = abi.TRAV_GROUP if args.traversal == "group" else abi.TRAV_EXACT
+        legs["plain"] = run_leg(nbody2d, ics, mass, pos, vel, args.theta_plain, False, trav,
+                                args)
+    if args.legs in ("both", "learned"):
+        legs["learned"] = run_leg(nbody2d, ics, mass, pos, vel, args.theta_learned, True,
+                                  abi.TRAV_GROUP, args)
 
-    # Kernel durations: the same steps launched eagerly with hipEvents around
-    # every phase on the context stream.
-    ctx.set_timing(True)
-    ctx.step(args.steps)
-    phase_ms, steps_timed = ctx.timing()
-    ctx.set_timing(False)
-    per_phase = {k: v / steps_timed for k, v in phase_ms.items()}
-
-    acc, inter, total = ctx.forces()
-    info = ctx.info()
-    p_now, _ = ctx.state()
-    t = ics.sample_targets(n, args.error_sample, seed=1)
-    ref = nbody2d.direct_sum_f64(p_now, mass, 1.0, EPS, t)
-    rms, mx = ics.force_errors(acc[t], ref)
-    ctx.close()
-
-    force_ms = per_phase["force"]
-    achieved = total * FLOPS_PER_INTERACTION / (force_ms * 1e-3) / 1e12
-    roofline = {
-        "bound": "mfma",
-        "achieved": achieved,
-        "peak": FP32_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": achieved / FP32_PEAK_TFLOPS,
-        "traffic": traffic_from_profile("k_traverse_group" if args.traversal == "group"
-                                        else "k_traverse"),
-        "kernel": ("k_traverse_group" if args.traversal == "group" else "k_traverse")
-                  + " (FP32 vector; 14 flops per interaction)",
-        "interactions_per_launch": total,
-        "avg_launch_ms": force_ms,
-    }
+    main_leg = legs.get("learned") or legs["plain"]
+    main_name = "learned" if "learned" in legs else "plain"
+    workload = (f"exp_disk N={n}, G=1, eps={EPS}, dt={DT}; "
+                + (f"learned far-field BH theta={args.theta_learned} (BF16 MFMA network "
+                   f"trained on the GPU from direct-sum pairs)" if "learned" in legs else "")
+                + (" vs " if len(legs) == 2 else "")
+                + (f"plain BH theta={args.theta_plain}" if "plain" in legs else "")
+                + "; FP32 Kahan pair forces, FP64 state/leapfrog, hipGraph steps")
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": main_leg["body_steps_per_s"],
         "unit": "body-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": main_leg["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"exp_disk N={n} plain Barnes-Hut theta={theta} eps={EPS} dt={DT} "
-                        "(FP32 Kahan pair forces, FP64 state/leapfrog, hipGraph steps)",
-            "n_bodies": n, "theta": theta, "softening": EPS, "dt": DT, "seed": SEED,
-            "far_field": "monopole", "traversal": args.traversal,
+            "workload": workload,
+            "n_bodies": n, "softening": EPS, "dt": DT, "seed": SEED,
+            "value_leg": main_name, "theta": main_leg["theta"],
+            "far_field": main_leg["far_field"], "traversal": main_leg["traversal"],
         },
-        "force_error": {"rms": rms, "max": mx, "sample": int(len(t)),
-                        "judge": "GPU FP64 direct sum"},
-        "interactions_per_body": total / n,
-        "tree_nodes": int(info.tree_nodes),
-        "phase_ms_per_step_eager": per_phase,
-        "roofline": roofline,
+        "force_error": main_leg["force_error"],
+        "legs": legs,
+        "roofline": roofline_walk(main_leg, main_leg["traversal"]),
     }
+    if len(legs) == 2:
+        out["learned_over_plain"] = (legs["learned"]["body_steps_per_s"]
+                                     / legs["plain"]["body_steps_per_s"])
+        out["error_learned_over_plain"] = (legs["learned"]["force_error"]["rms"]
+                                           / legs["plain"]["force_error"]["rms"])
+    if "learned" in legs:
+        lg = legs["learned"]
+        ff_ms = lg["phase_ms_per_step_eager"].get("far_field", 0.0)
+        if ff_ms > 0:
+            flops = lg["far_field_cells_per_step_timed"] * MLP_FLOPS_PER_EVAL
+            out["roofline_far_field"] = {
+                "bound": "mfma", "achieved": flops / (ff_ms * 1e-3) / 1e12,
+                "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": flops / (ff_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
+                "kernel": "k_mlp_apply (BF16 MFMA; 9,472 flops per accepted cell)",
+                "avg_launch_ms": ff_ms,
+            }
     if not args.no_cpu:
         print("[bench] cpu baseline ...", flush=True)
-        cb = cpu_baseline(mass, pos, vel)
+        cb = cpu_baseline(mass, pos, vel, args.theta_plain)
         out["cpu_baseline"] = cb
-        out["gpu_over_cpu"] = value / cb["value"]
+        if "plain" in legs:
+            out["plain_gpu_over_cpu"] = legs["plain"]["body_steps_per_s"] / cb["value"]
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

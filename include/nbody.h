@@ -85,7 +85,10 @@ typedef struct nb_params {
     int32_t traversal;  /* enum nb_traversal */
     int32_t group_budget; /* GROUP: interaction-list entries after which a group
                              is split in halves and walked again (0 = 8192) */
-    int32_t reserved[5];
+    int32_t cell_capacity; /* LEARNED: initial accepted-cell entries per group of
+                              64 bodies (0 = 4096); a step that needs more
+                              halts, the region grows and the step runs again */
+    int32_t reserved[4];
 } nb_params;
 
 typedef struct nb_info {
@@ -96,6 +99,10 @@ typedef struct nb_info {
     uint64_t internal_nodes;  /* internal nodes of the last tree */
     uint64_t node_capacity;   /* internal-node capacity currently allocated */
     uint64_t device_bytes;    /* device memory held by the context */
+    uint64_t far_field_cells; /* accepted cells the network corrected per force
+                                 evaluation (far_field = LEARNED): the last
+                                 nb_compute_forces, or the mean over the last
+                                 nb_step call; each is evaluated at 4 points */
 } nb_info;
 
 /* Timing phases reported by nb_get_timing (milliseconds, summed over the
@@ -106,7 +113,8 @@ enum nb_phase {
     NB_PHASE_BUILD = 2,       /* quadtree topology + mass/COM */
     NB_PHASE_FORCE = 3,       /* traversal (or all-pairs) + kick */
     NB_PHASE_DRIFT = 4,       /* drift */
-    NB_PHASE_COUNT = 5
+    NB_PHASE_FAR_FIELD = 5,   /* learned far-field network + kick (LEARNED only) */
+    NB_PHASE_COUNT = 6
 };
 
 /* Exported tree slot for parity checks (canonical topology + COM). */
@@ -230,6 +238,17 @@ int nb_mlp_grad(nb_ctx* ctx, const float* features, const float* targets, uint64
 
 /* Network outputs for `count` input rows on MFMA: out [count][2]. */
 int nb_mlp_forward(nb_ctx* ctx, const float* features, uint64_t count, float* out);
+
+/* Bookkeeping of the learned far field in the last force evaluation, for
+ * tests. Groups are 64 consecutive bodies of the tree's sorted order
+ * (groups = ceil(n/64)). Per group: nsub[g] finished subgroups; per subgroup
+ * slot (64 per group): its box centre sub_centre[2k..2k+1] and its range
+ * {start, count} in the group's region; cell_slots: the regions' tree slots
+ * (groups x *cell_cap_out entries); per sorted body: its subgroup and its
+ * offset from that subgroup's centre. Any output may be NULL. */
+int nb_debug_far_field(nb_ctx* ctx, uint32_t* nsub, double* sub_centre, uint32_t* sub_range,
+                       uint32_t* cell_slots, int32_t* body_sub, float* body_off,
+                       uint32_t* cell_cap_out);
 
 #ifdef __cplusplus
 }

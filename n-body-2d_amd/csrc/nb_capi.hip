@@ -22,9 +22,9 @@ namespace {
 
 thread_local std::string g_err;
 
-// Initial accepted-cell region per group of 64 bodies (learned far field);
-// a step that needs more halts, and the region grows to 1.25x the need.
-constexpr uint32_t kInitialCells = 1024;
+// Default accepted-cell region per group of 64 bodies (learned far field);
+// a step that needs more halts, and the region grows to twice the need.
+constexpr uint32_t kInitialCells = 4096;
 
 int fail(nb_ctx* c, int code, const std::string& msg) {
     if (c)
@@ -152,6 +152,7 @@ int check_params(const nb_params* p, std::string* why) {
     if (p->traversal != NB_TRAV_EXACT && p->traversal != NB_TRAV_GROUP)
         return *why = "unknown traversal", NB_ERR_INVALID_ARG;
     if (p->group_budget < 0) return *why = "group_budget must be >= 0", NB_ERR_INVALID_ARG;
+    if (p->cell_capacity < 0) return *why = "cell_capacity must be >= 0", NB_ERR_INVALID_ARG;
     if (p->far_field != NB_FAR_MONOPOLE && p->far_field != NB_FAR_LEARNED)
         return *why = "unknown far_field", NB_ERR_INVALID_ARG;
     if (p->far_field == NB_FAR_LEARNED &&
@@ -177,7 +178,9 @@ int upload_consts(nb_ctx* c) {
 struct Marks {
     nb_ctx* c;
     void begin(int ph) {
-        if (c->timing) (void)hipEventRecord(c->ev[2 * ph], c->stream);
+        if (!c->timing) return;
+        (void)hipEventRecord(c->ev[2 * ph], c->stream);
+        c->marked |= 1u << ph;
     }
     void end(int ph) {
         if (c->timing) (void)hipEventRecord(c->ev[2 * ph + 1], c->stream);
@@ -205,6 +208,11 @@ int enqueue_force(nb_ctx* c, int src, bool kick, bool count_step) {
         mk.begin(NB_PHASE_FORCE);
         launch_traverse(c, s, kick, count_step);
         mk.end(NB_PHASE_FORCE);
+        if (c->p.far_field == NB_FAR_LEARNED) {
+            mk.begin(NB_PHASE_FAR_FIELD);
+            launch_mlp_apply(c, s, kick, count_step);
+            mk.end(NB_PHASE_FAR_FIELD);
+        }
     } else {
         mk.begin(NB_PHASE_FORCE);
         launch_allpairs(c, s, kick, count_step);
@@ -272,7 +280,7 @@ int grow_for(nb_ctx* c, uint32_t flags) {
     }
     if (flags & FLAG_CELL_CAPACITY) {
         const uint64_t need = c->host_meta[META_CELLS_NEEDED];
-        const uint64_t cap = need + need / 4 + 64;
+        const uint64_t cap = 2 * need + 64;
         if (cap > 0xffffffffull) return fail(c, NB_ERR_OUT_OF_MEMORY, "accepted-cell region");
         int rc = alloc_cells(c, (uint32_t)cap);
         if (rc) return rc;
@@ -304,6 +312,7 @@ int build_and_force(nb_ctx* c, bool forces) {
         if (rc) return rc;
         const uint32_t flags = c->host_meta[META_FLAGS];
         c->last_internal = c->host_meta[META_INTERNAL];
+        if (forces) c->last_cells = c->host_meta[META_CELLS_TOTAL];
         if (flags & FLAG_NONFINITE) return fail(c, NB_ERR_NUMERIC, "non-finite body positions");
         if (flags & FLAG_STACK) return fail(c, NB_ERR_OVERFLOW, "traversal stack overflow");
         if (!(flags & (FLAG_TREE_CAPACITY | FLAG_CELL_CAPACITY))) return NB_OK;
@@ -431,7 +440,8 @@ int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* 
         // weights, i.e. no correction) before the first launch
         rc = prepare_mlp(c);
         if (rc) return bail(rc);
-        rc = alloc_cells(c, kInitialCells);
+        rc = alloc_cells(c, c->p.cell_capacity > 0 ? (uint32_t)c->p.cell_capacity
+                                                     : kInitialCells);
         if (rc) return bail(rc);
     }
     if (bh && n) {
@@ -463,13 +473,15 @@ int nb_step(nb_ctx* c, uint64_t k) {
         if (rc) return rc;
         for (uint64_t s = 0; s < remaining; s++) {
             if (c->timing) {
+                c->marked = 0;
                 const int dst = enqueue_step(c, c->cur);
                 NB_HIP(c, hipGetLastError());
                 NB_HIP(c, hipStreamSynchronize(c->stream));
                 for (int ph = 0; ph < NB_PHASE_COUNT; ph++) {
+                    if (!((c->marked >> ph) & 1u)) continue;  // phase absent from this step
                     float ms = 0.f;
-                    if (hipEventElapsedTime(&ms, c->ev[2 * ph], c->ev[2 * ph + 1]) == hipSuccess)
-                        c->phase_ms[ph] += ms;
+                    NB_HIP(c, hipEventElapsedTime(&ms, c->ev[2 * ph], c->ev[2 * ph + 1]));
+                    c->phase_ms[ph] += ms;
                 }
                 c->steps_timed++;
                 c->cur = dst;
@@ -480,14 +492,13 @@ int nb_step(nb_ctx* c, uint64_t k) {
                 c->cur = enqueue_step(c, c->cur);
                 NB_HIP(c, hipGetLastError());
             }
-            // Reset the phase events so a phase absent from this variant
-            // contributes nothing next time.
         }
         rc = read_meta(c);
         if (rc) return rc;
         const uint32_t flags = c->host_meta[META_FLAGS];
         const uint64_t done = std::min<uint64_t>(c->host_meta[META_STEPS_DONE], remaining);
         c->last_internal = c->host_meta[META_INTERNAL];
+        if (done) c->last_cells = c->host_meta[META_CELLS_TOTAL] / done;
         c->iteration += done;
         c->sim_time += (double)done * c->dt;
         remaining -= done;
@@ -592,6 +603,7 @@ int nb_get_info(nb_ctx* c, nb_info* info) {
     info->tree_nodes = c->n ? 1 + 4 * c->last_internal : 0;
     info->node_capacity = c->icap;
     info->device_bytes = c->device_bytes;
+    info->far_field_cells = c->last_cells;
     return NB_OK;
 }
 

@@ -190,20 +190,19 @@ __device__ __forceinline__ void eval_list(const float4* __restrict__ list, int c
     kahan_add(a.ay, a.cy, py);
 }
 
-// LEARN: the walk leaves the far-field network to k_mlp_apply. Every
-// accepted internal cell is also written to the group's region of `cells`
-// (offset from its subgroup's centre, mass, side, second moments, subgroup
-// id = first lane of the subgroup); each member records its subgroup and its
-// offset in units of the subgroup's half-extent, and the raw acceleration
-// (without G) goes to acc_out instead of a kick.
+// LEARN: the walk leaves the far-field network to k_mlp_apply. The tree slot
+// of every accepted internal cell is also appended to the group's region of
+// `cell_slot`; each finished subgroup k records its cell range and its box
+// centre, each member its subgroup and its offset from that centre, and the
+// raw acceleration (without G) goes to acc_out instead of a kick.
 template <bool KICK, bool ZERO_EPS, bool GUARD, bool LEARN>
 __global__ __launch_bounds__(64) void k_traverse_group(
     const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
     const double* __restrict__ mass, double2* __restrict__ vel, double2* __restrict__ acc_out,
     uint32_t* __restrict__ inter_out, uint32_t n, const StepConsts* __restrict__ k,
     uint32_t* __restrict__ meta, int count_step, int budget,
-    const uint64_t* __restrict__ keys, CellRec* __restrict__ cells,
-    uint32_t* __restrict__ cell_cnt, uint32_t cell_cap, float2* __restrict__ sub_h,
+    const uint64_t* __restrict__ keys, uint32_t* __restrict__ cell_slot,
+    uint32_t* __restrict__ cell_cnt, uint32_t cell_cap, SubRec* __restrict__ subs,
     float4* __restrict__ body_w) {
     if (meta[META_FLAGS] & kHaltFlags) return;
     __shared__ __attribute__((aligned(16))) int32_t s_stack[kGStack];
@@ -231,6 +230,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     // its group test is its own opening test).
     int nt = 1;
     uint32_t cbase = 0;  // cells of the finished subgroups (LEARN)
+    int nsub = 0;        // finished subgroups (LEARN)
     if (l == 0) s_todo[0] = 64;
     while (nt > 0) {
         const int32_t e = readfirst(s_todo[--nt]);
@@ -340,12 +340,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                 const uint64_t cmk = ballot64(cell);
                 if (cell) {
                     const uint32_t at = cbase + ccnt + (uint32_t)__popcll(cmk & lanemask_lt());
-                    if (at < cell_cap) {
-                        CellRec cr;
-                        cr.a = make_float4((float)dxc, (float)dyc, r.mf, (float)sqrt(r.wsq));
-                        cr.b = make_float4(r.ixx, r.ixy, r.iyy, __int_as_float(f));
-                        cells[(uint64_t)g * cell_cap + at] = cr;
-                    }
+                    if (at < cell_cap) cell_slot[(uint64_t)g * cell_cap + at] = (uint32_t)node;
                 }
                 ccnt += (uint32_t)__popcll(cmk);
             }
@@ -400,13 +395,17 @@ __global__ __launch_bounds__(64) void k_traverse_group(
             listed += cnt;
         }
         if (LEARN) {
-            cbase += ccnt;
-            if (member) {
-                const float wx = hx > 0.0 ? bx / (float)hx : 0.f;
-                const float wy = hy > 0.0 ? by / (float)hy : 0.f;
-                body_w[i] = make_float4(wx, wy, __int_as_float(f), 0.f);
+            if (member) body_w[i] = make_float4(bx, by, __int_as_float(nsub), 0.f);
+            if (l == 0) {
+                SubRec sr;
+                sr.ox = ox;
+                sr.oy = oy;
+                sr.start = cbase;
+                sr.count = ccnt;
+                subs[(uint64_t)g * 64 + nsub] = sr;
             }
-            if (l == 0) sub_h[g * 64 + f] = make_float2((float)hx, (float)hy);
+            cbase += ccnt;
+            nsub++;
         }
         if (member) {
             fin = t;
@@ -416,7 +415,8 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     }
     if (overflow && l == 0) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_STACK);
     if (LEARN && l == 0) {
-        cell_cnt[g] = cbase;
+        cell_cnt[g] = (uint32_t)nsub;
+        atomicAdd(&meta[META_CELLS_TOTAL], cbase);
         if (cbase > cell_cap) {
             atomicMax(&meta[META_CELLS_NEEDED], cbase);
             atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_CELL_CAPACITY);
@@ -455,8 +455,8 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
     hipLaunchKernelGGL((k_traverse_group<K, Z, GD, LN>), dim3(groups), dim3(64), 0, c->stream, \
                        c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src], c->b.acc,     \
                        c->b.inter, n, c->b.consts, c->b.meta, (int)(count_step && !learn),     \
-                       budget, c->b.keys[0], c->b.cells, c->b.cell_cnt, c->cell_cap, c->b.sub_h,       \
-                       c->b.body_w)
+                       budget, c->b.keys[0], c->b.cell_slot, c->b.cell_cnt, c->cell_cap,       \
+                       c->b.subs, c->b.body_w)
 #define NB_TRAVG3(K, Z, GD) if (learn) NB_TRAVG(false, Z, GD, true); else NB_TRAVG(K, Z, GD, false)
 #define NB_TRAVG2(K, Z) if (gd) { NB_TRAVG3(K, Z, true); } else { NB_TRAVG3(K, Z, false); }
         if (kick) {
@@ -467,7 +467,6 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
 #undef NB_TRAVG3
 #undef NB_TRAVG2
 #undef NB_TRAVG
-        if (learn) launch_mlp_apply(c, src, kick, count_step);
         return;
     }
     const uint32_t blocks = (n + kTravThreads - 1) / kTravThreads;

@@ -38,6 +38,7 @@ enum Meta : int {
     META_STEPS_DONE = 2, // completed steps since the host last cleared it
     META_MULTI_LEAVES = 3, // multi-body leaves queued by the current build
     META_CELLS_NEEDED = 4, // most accepted cells of one group (learned far field)
+    META_CELLS_TOTAL = 5,  // accepted cells of all groups (learned far field)
     META_COUNT = 8
 };
 enum Flag : uint32_t {
@@ -63,11 +64,11 @@ struct __attribute__((aligned(64))) NodeRec {
 };
 static_assert(sizeof(NodeRec) == 64, "NodeRec must be 64 bytes");
 
-// One accepted cell waiting for the far-field network: a = {dx, dy, M, s}
-// (offset COM - subgroup centre, mass, side), b = {Ixx, Ixy, Iyy, subgroup id
-// as int bits}.
-struct CellRec {
-    float4 a, b;
+// A finished subgroup of a traversal group (learned far field): its box
+// centre and its range of accepted cells in the group's region.
+struct SubRec {
+    double ox, oy;
+    uint32_t start, count;
 };
 
 // Device-side constants of a step, read by the kernels so set_dt needs no
@@ -111,7 +112,7 @@ struct Buffers {
     float* mlp_w;          // [4866] FP32 master parameters, canonical order
     float* mlp_m;          // [4866] Adam first moments
     float* mlp_v;          // [4866] Adam second moments
-    uint2* mlp_tiles;      // [24*64] BF16 A tiles, per-lane MFMA layout
+    void* mlp_tiles;       // BF16 A tiles in the per-lane MFMA layouts (kMlpTileBytes)
     float* mlp_bias;       // [144] b1, b2, b3, zero padding
     float* mlp_gpart;      // [parts][4866] per-workgroup partial gradients
     float* mlp_loss;       // [parts]
@@ -119,12 +120,12 @@ struct Buffers {
     uint32_t* pair_slot;   // [pair_cap] training pairs: cell slot
     float* pair_x;         // [pair_cap][8] features
     float* pair_t;         // [pair_cap][2] targets
-    unsigned long long* pair_count;
+    uint32_t* sample_off;  // [N + 2] per-sample pair counts, then offsets; [N + 1] total
     // learned far field in the step: per group of 64 bodies
-    CellRec* cells;        // [groups][cell_cap] accepted cells
-    uint32_t* cell_cnt;    // [groups]
-    float2* sub_h;         // [groups][64] half-extents of the subgroups
-    float4* body_w;        // [N] {bx/hx, by/hy, subgroup id, -}
+    uint32_t* cell_slot;   // [groups][cell_cap] tree slots of the accepted cells
+    uint32_t* cell_cnt;    // [groups] finished subgroups
+    SubRec* subs;          // [groups][64]
+    float4* body_w;        // [N] {offset from the subgroup centre (x, y), subgroup, -}
 };
 
 }  // namespace nb
@@ -145,6 +146,7 @@ struct nb_ctx {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     bool timing = false;
     hipEvent_t ev[2 * NB_PHASE_COUNT] = {};  // start/stop per phase
+    uint32_t marked = 0;                     // phases recorded in the current step
     double phase_ms[NB_PHASE_COUNT] = {};
     uint64_t steps_timed = 0;
     uint64_t iteration = 0;
@@ -157,6 +159,7 @@ struct nb_ctx {
     uint32_t train_parts = 0;       // partial-gradient rows allocated
     uint32_t train_step = 0;        // Adam step counter
     uint32_t cell_cap = 0;          // accepted-cell region per group
+    uint64_t last_cells = 0;        // accepted cells of the last nb_compute_forces
     std::string err;
 };
 

@@ -4,24 +4,33 @@
 // accumulation. Parameters in the canonical order W1[64][8], b1[64],
 // W2[64][64], b2[64], W3[2][64], b3[2] (4,866 values).
 //
-// MFMA formulation (v_mfma_f32_16x16x16bf16_1k, D = A B + C, wave64):
-//   lane l holds A[m = l%16][k = 4(l/16)+v], B[k = 4(l/16)+v][n = l%16] and
-//   D[m = 4(l/16)+v][n = l%16], v = 0..3.
-// The weights are A (rows = output neurons, cols = inputs) and the batch is
-// B (16 evaluation points per tile, one per column), so a layer's output
-// tile t (neurons 16t..16t+15) is, lane for lane, the B operand of k-block t
-// of the next layer: no shuffle or LDS round trip between layers. Biases are
-// the accumulator's initial value.
+// MFMA formulation (D = A B + C, wave64). The weights are A (rows = output
+// neurons), the batch is B (16 evaluation points per tile, one per column);
+// biases are the accumulator's initial value.
+//   layer 1, v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)+v],
+//     B[4(l/16)+v][l%16] (v = 0..3): inputs f0..f3 in lanes 0-15, f4..f7 in
+//     lanes 16-31, zero padding (k = 8..15) in lanes 32-63.
+//   layers 2, 3, v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)+v],
+//     B[8(l/16)+v][l%16] (v = 0..7).
+//   every layer's D: lane l holds D[4(l/16)+v][l%16] (v = 0..3).
+// Output tile t of a layer (neurons 16t..16t+15) gives lane l the neurons
+// 16t + 4(l/16) + v. The K = 32 block kb of the next layer takes tiles 2kb
+// and 2kb+1 as they are in the registers: its k index 8g + v (g = l/16)
+// stands for neuron 32kb + 4g + v (v < 4) or 32kb + 16 + 4g + v - 4
+// (v >= 4), and the packed weights use the same order (mlp_k_neuron), so
+// no shuffle or LDS round trip happens between layers.
 //
-// Inputs of a (cell, point) pair, d = COM - point, r = |d|, R^2 = r^2 + eps^2,
-// s = cell side, M = cell mass, I = second moments about the COM:
-//   f0,f1 = d/r   f2 = s/R   f3 = log2(M/M_root)/16
-//   f4,f5,f6 = (Ixx, Ixy, Iyy) / (M s^2)   f7 = log2(r/ell)/8
-// with ell = eps (or the root cell side when eps = 0). Output y (2 values):
-// the correction a_exact - a_monopole of the Plummer kernel in units of
-// M s^2 r / R^5, the scale of the softened quadrupole term, whose value in
-// these units is (3/2)[-tr(I') u - 2 I' u + 5 (r/R)^2 (u.I'u) u] with
-// u = (f0, f1), I' = (f4..f6): an O(1) function of the inputs.
+// Inputs of a (cell, point) pair, d = COM - point, r = |d|, u = d/r,
+// w = (-u_y, u_x), R^2 = r^2 + eps^2, s = cell side, M = cell mass, I =
+// second moments about the COM, taken in the frame (u, w):
+//   f0,f1 = u   f2 = s/R   f3 = log2(M/M_root)/16
+//   f4,f5,f6 = (u.Iu, u.Iw, w.Iw) / (M s^2)   f7 = log2(r/ell)/8
+// with ell = eps (or the root cell side when eps = 0). Output (y0, y1): the
+// correction a_exact - a_monopole of the Plummer kernel along (u, w), in units
+// of M s^2 r / R^5 (the scale of the softened quadrupole term). In these
+// units the quadrupole term is (3/2)(-f4 - f6 - 2 f4 + 5 rho f4, -2 f5) with
+// rho = r^2/R^2 = 1/(1 + 2^(-16 f7)) when ell = eps: linear in the moment
+// inputs, so the network mostly learns the higher orders.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,13 +43,27 @@ constexpr int kMlpHidden = 64;
 constexpr int kMlpOut = 2;
 constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden + kMlpHidden * kMlpHidden +
                            kMlpHidden + kMlpOut * kMlpHidden + kMlpOut;  // 4866
-constexpr int kMlpTiles = 24;   // A tiles: W1 0..3, W2 4..19 (4 n x 4 k), W3 20..23
+// Packed A tiles: layer 1 as 4 x 64 uint2 (4 BF16 per lane), then layer 2
+// (4 output tiles x 2 K blocks) and layer 3 (2 K blocks) as 10 x 64 uint4
+// (8 BF16 per lane).
+constexpr int kMlpTiles1 = 4;
+constexpr int kMlpTiles23 = 10;
+constexpr int kMlpTileBytes = kMlpTiles1 * 64 * 8 + kMlpTiles23 * 64 * 16;
+// Bias vector in LDS: b1[64], b2[64], b3[2], zero padding.
+constexpr int kMlpBiasLen = 144;
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Round to nearest even, as bfloat16 (the same rounding the host packer and
-// the training kernel use).
+// Neuron that k index j (0..31) of K block kb stands for (see above).
+__host__ __device__ constexpr int mlp_k_neuron(int kb, int j) {
+    return 32 * kb + ((j & 7) < 4 ? 4 * (j >> 3) + (j & 7) : 16 + 4 * (j >> 3) + (j & 7) - 4);
+}
+
+// Round to nearest even, as bfloat16 (host packing and the training kernel;
+// the inference path uses the hardware conversion, which rounds the same way).
 __host__ __device__ inline uint16_t f32_to_bf16(float f) {
     union {
         float f;
@@ -63,13 +86,16 @@ __host__ __device__ inline float bf16_to_f32(uint16_t h) {
 
 __host__ __device__ inline float bf16_round(float f) { return bf16_to_f32(f32_to_bf16(f)); }
 
-__device__ inline bf16x4 pack_bf16(f32x4 v) {
-    bf16x4 r;
-    r.x = (short)f32_to_bf16(v.x);
-    r.y = (short)f32_to_bf16(v.y);
-    r.z = (short)f32_to_bf16(v.z);
-    r.w = (short)f32_to_bf16(v.w);
-    return r;
+// Two floats to two BF16 in one register (v_cvt_pk_bf16_f32).
+__device__ inline uint32_t pk2(float a, float b) {
+    bf16x2 v;
+    v.x = (__bf16)a;
+    v.y = (__bf16)b;
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+__device__ inline s16x4 pack4(f32x4 v) {
+    return __builtin_bit_cast(s16x4, make_uint2(pk2(v.x, v.y), pk2(v.z, v.w)));
 }
 
 __device__ inline f32x4 relu4(f32x4 v) {
@@ -81,26 +107,67 @@ __device__ inline f32x4 relu4(f32x4 v) {
     return r;
 }
 
-// The 8 inputs of one (cell, point) pair and the output scale.
+// Two output tiles of a layer (ReLU applied) as the B operand of one K block.
+__device__ inline bf16x8 pack8_relu(f32x4 a, f32x4 b) {
+    a = relu4(a);
+    b = relu4(b);
+    return __builtin_bit_cast(bf16x8, make_uint4(pk2(a.x, a.y), pk2(a.z, a.w), pk2(b.x, b.y),
+                                                 pk2(b.z, b.w)));
+}
+
+// The 8 inputs of one (cell, point) pair; returns the output scale and the
+// unit offset (ux, uy) that rotates the output back.
 __device__ inline float mlp_features(float dx, float dy, float s, float M, float ixx, float ixy,
                                      float iyy, float eps2, float inv_mroot, float inv_ell,
-                                     float f[8]) {
+                                     float f[8], float& ux, float& uy) {
     const float r2 = fmaf(dx, dx, dy * dy);
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * inv_r;
     const float inv_R = __builtin_amdgcn_rsqf(r2 + eps2);
     const float ms2 = M * s * s;
-    const float k = 1.0f / ms2;
-    f[0] = dx * inv_r;
-    f[1] = dy * inv_r;
+    const float k = __builtin_amdgcn_rcpf(ms2);
+    ux = dx * inv_r;
+    uy = dy * inv_r;
+    // moments in the frame (u, w), w = (-uy, ux)
+    const float iu_x = fmaf(ixx, ux, ixy * uy), iu_y = fmaf(ixy, ux, iyy * uy);
+    const float uiu = fmaf(ux, iu_x, uy * iu_y);
+    const float uiw = fmaf(-uy, iu_x, ux * iu_y);
+    const float wiw = (ixx + iyy) - uiu;
+    f[0] = ux;
+    f[1] = uy;
     f[2] = s * inv_R;
     f[3] = __log2f(M * inv_mroot) * 0.0625f;
-    f[4] = ixx * k;
-    f[5] = ixy * k;
-    f[6] = iyy * k;
+    f[4] = uiu * k;
+    f[5] = uiw * k;
+    f[6] = wiw * k;
     f[7] = __log2f(r * inv_ell) * 0.125f;
     const float iR2 = inv_R * inv_R;
     return ms2 * r * (iR2 * iR2 * inv_R);
+}
+
+// Jacobian dQ/dd of the softened quadrupole term Q(d) = (3/(2R^5))
+// [-tr(I) d - 2 I d + 5 (d.Id) d / R^2] (d = COM - point, R^2 = |d|^2 + eps^2),
+// FP32: J = {dQx/ddx, dQx/ddy, dQy/ddx, dQy/ddy}. Moving the point by b
+// changes d by -b, so Q(point + b) ~ Q - J b.
+__device__ inline void quad_jacobian(float dx, float dy, float ixx, float ixy, float iyy,
+                                     float eps2, float J[4]) {
+    const float R2 = fmaf(dx, dx, fmaf(dy, dy, eps2));
+    const float iR = __builtin_amdgcn_rsqf(R2);
+    const float iR2 = iR * iR;
+    const float iR5 = iR2 * iR2 * iR;
+    const float t = ixx + iyy;
+    const float idx = fmaf(ixx, dx, ixy * dy), idy = fmaf(ixy, dx, iyy * dy);
+    const float S = fmaf(dx, idx, dy * idy);
+    const float px = fmaf(5.f * S * iR2, dx, -fmaf(t, dx, 2.f * idx));
+    const float py = fmaf(5.f * S * iR2, dy, -fmaf(t, dy, 2.f * idy));
+    const float k = 1.5f * iR5;
+    const float diag = 5.f * S * iR2 - t;
+    const float a = 5.f * iR2, b = 10.f * iR2, c = 10.f * S * iR2 * iR2;
+    // J_ij = k [ -a d_j P_i - t d_ij - 2 I_ij + b (Id)_j d_i + 5 S R^-2 d_ij - c d_i d_j ]
+    J[0] = k * (-a * dx * px + diag - 2.f * ixx + b * idx * dx - c * dx * dx);
+    J[1] = k * (-a * dy * px - 2.f * ixy + b * idy * dx - c * dx * dy);
+    J[2] = k * (-a * dx * py - 2.f * ixy + b * idx * dy - c * dy * dx);
+    J[3] = k * (-a * dy * py + diag - 2.f * iyy + b * idy * dy - c * dy * dy);
 }
 
 // 1 / ell for the distance input: eps, or the root cell side when eps = 0.
@@ -108,93 +175,70 @@ __device__ inline float mlp_inv_ell(float eps2f, double root_wsq) {
     return eps2f > 0.f ? __builtin_amdgcn_rsqf(eps2f) : (float)(1.0 / sqrt(root_wsq));
 }
 
-// Bias vector in LDS: b1[64], b2[64], b3[2], zero padding to kMlpBiasLen.
-constexpr int kMlpBiasLen = 144;
-
-// The 24 A tiles of this lane, loaded once per batch of tiles.
+// This lane's A tiles, loaded once per batch of tiles (48 VGPRs).
 struct MlpWeights {
-    bf16x4 a[kMlpTiles];
+    s16x4 a1[kMlpTiles1];
+    bf16x8 a23[kMlpTiles23];
 };
 
-__device__ inline void mlp_load_weights(MlpWeights& w, const uint2* __restrict__ wt, int lane) {
+__device__ inline void mlp_load_weights(MlpWeights& w, const void* __restrict__ tiles,
+                                        int lane) {
+    const uint2* t1 = reinterpret_cast<const uint2*>(tiles);
+    const uint4* t23 = reinterpret_cast<const uint4*>(t1 + kMlpTiles1 * 64);
 #pragma unroll
-    for (int t = 0; t < kMlpTiles; t++) {
-        const uint2 u = wt[t * 64 + lane];
-        w.a[t].x = (short)(u.x & 0xffff);
-        w.a[t].y = (short)(u.x >> 16);
-        w.a[t].z = (short)(u.y & 0xffff);
-        w.a[t].w = (short)(u.y >> 16);
-    }
+    for (int t = 0; t < kMlpTiles1; t++)
+        w.a1[t] = __builtin_bit_cast(s16x4, t1[t * 64 + lane]);
+#pragma unroll
+    for (int t = 0; t < kMlpTiles23; t++)
+        w.a23[t] = __builtin_bit_cast(bf16x8, t23[t * 64 + lane]);
 }
 
 __device__ inline f32x4 mlp_bias4(const float* sb, int off, int lane) {
     return *reinterpret_cast<const f32x4*>(sb + off + 4 * (lane >> 4));
 }
 
-// Forward pass of one 16-column tile. `x` holds this lane's four inputs
-// (lanes 0-15: f0..f3 of column l, lanes 16-31: f4..f7 of column l-16,
-// lanes 32-63: zero). Returns this lane's D rows of the last layer: lanes
-// 0-15 hold (y0, y1, 0, 0) of column l.
-__device__ inline f32x4 mlp_tile_forward(f32x4 x, const MlpWeights& w, const float* sb,
-                                         int lane) {
-    const bf16x4 b0 = pack_bf16(x);
-    bf16x4 h1[4];
+// Forward pass of T independent 16-column tiles (interleaved for the MFMA
+// pipeline). x[t] holds this lane's four inputs of tile t (lanes 0-15: f0..f3
+// of column l, lanes 16-31: f4..f7 of column l-16, lanes 32-63: zero).
+// y[t]: lanes 0-15 hold (y0, y1, 0, 0) of column l.
+template <int T>
+__device__ inline void mlp_forward_tiles(const f32x4* x, const MlpWeights& w, const float* sb,
+                                         int lane, f32x4* y) {
+    s16x4 b0[T];
 #pragma unroll
-    for (int t = 0; t < 4; t++)
-        h1[t] = pack_bf16(relu4(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-            w.a[t], b0, mlp_bias4(sb, 16 * t, lane), 0, 0, 0)));
-    bf16x4 h2[4];
+    for (int t = 0; t < T; t++) b0[t] = pack4(x[t]);
+    f32x4 h1[T][4];
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
-        f32x4 c = mlp_bias4(sb, 64 + 16 * nt, lane);
+    for (int n = 0; n < 4; n++) {
+        const f32x4 c = mlp_bias4(sb, 16 * n, lane);
 #pragma unroll
-        for (int kt = 0; kt < 4; kt++)
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.a[4 + 4 * nt + kt], h1[kt], c, 0, 0,
-                                                          0);
-        h2[nt] = pack_bf16(relu4(c));
+        for (int t = 0; t < T; t++)
+            h1[t][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.a1[n], b0[t], c, 0, 0, 0);
     }
-    f32x4 y = mlp_bias4(sb, 128, lane);
+    bf16x8 k1[T][2];
 #pragma unroll
-    for (int kt = 0; kt < 4; kt++)
-        y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.a[20 + kt], h2[kt], y, 0, 0, 0);
-    return y;
-}
-
-// Corrections of `count` accepted cells (LDS: ca = {dx, dy, M, s} with
-// d = COM - group centre, cb = {Ixx, Ixy, Iyy, -}) at three points: the group
-// centre, centre + (hx, 0) and centre + (0, hy). Lanes 0-15 add the sums over
-// cells to u[2q], u[2q+1] for point q; a member at offset (bx, by) from the
-// centre then takes u0 + (bx/hx)(u1 - u0) + (by/hy)(u2 - u0).
-__device__ inline void mlp_eval_cells(const float4* ca, const float4* cb, int count, float hx,
-                                      float hy, float eps2f, float inv_mroot, float inv_ell,
-                                      const MlpWeights& w, const float* sb, int lane,
-                                      float u[6]) {
-    const int col = lane & 15, part = lane >> 4;
-    const int tiles = (count + 15) >> 4;
-#pragma nounroll
-    for (int q = 0; q < 3; q++) {
-        const float ox = q == 1 ? hx : 0.f, oy = q == 2 ? hy : 0.f;
-        for (int t = 0; t < tiles; t++) {
-            const int cell = 16 * t + col;
-            const bool ok = cell < count && part < 2;
-            f32x4 x = {0.f, 0.f, 0.f, 0.f};
-            float scale = 0.f;
-            if (ok) {
-                const float4 a = ca[cell], b = cb[cell];
-                float f[8];
-                scale = mlp_features(a.x - ox, a.y - oy, a.w, a.z, b.x, b.y, b.z, eps2f,
-                                     inv_mroot, inv_ell, f);
-                x.x = part ? f[4] : f[0];
-                x.y = part ? f[5] : f[1];
-                x.z = part ? f[6] : f[2];
-                x.w = part ? f[7] : f[3];
-            }
-            const f32x4 y = mlp_tile_forward(x, w, sb, lane);
-            if (ok && part == 0) {
-                u[2 * q] = fmaf(y.x, scale, u[2 * q]);
-                u[2 * q + 1] = fmaf(y.y, scale, u[2 * q + 1]);
-            }
+    for (int t = 0; t < T; t++) {
+        k1[t][0] = pack8_relu(h1[t][0], h1[t][1]);
+        k1[t][1] = pack8_relu(h1[t][2], h1[t][3]);
+    }
+    f32x4 h2[T][4];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const f32x4 c = mlp_bias4(sb, 64 + 16 * n, lane);
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            f32x4 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.a23[2 * n], k1[t][0], c, 0, 0, 0);
+            h2[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.a23[2 * n + 1], k1[t][1], a, 0,
+                                                               0, 0);
         }
+    }
+    const f32x4 c3 = mlp_bias4(sb, 128, lane);
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        const bf16x8 k20 = pack8_relu(h2[t][0], h2[t][1]);
+        const bf16x8 k21 = pack8_relu(h2[t][2], h2[t][3]);
+        const f32x4 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.a23[8], k20, c3, 0, 0, 0);
+        y[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.a23[9], k21, a, 0, 0, 0);
     }
 }
 

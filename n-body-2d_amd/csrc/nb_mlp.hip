@@ -34,15 +34,19 @@ __device__ __forceinline__ uint32_t hash_u32(uint64_t a) {
 }
 
 // ---------------------------------------------------------------------------
-// Pair collection: wave-cooperative masked walk (as the EXACT traversal).
+// Pair collection: wave-cooperative masked walk (as the EXACT traversal), in
+// two passes so the pair order is deterministic: COUNT stores each sample's
+// number of pairs, an exclusive scan turns them into offsets, WRITE stores
+// the pairs of sample k from offset[k] on (up to cap pairs in all).
 // ---------------------------------------------------------------------------
+template <bool WRITE>
 __global__ __launch_bounds__(64) void k_mlp_collect(const NodeRec* __restrict__ nodes,
                                                     const double2* __restrict__ pos, uint32_t n,
                                                     uint32_t samples, uint64_t seed,
-                                                    double theta_sq, uint64_t cap,
+                                                    double theta_sq, uint32_t* __restrict__ counts,
+                                                    uint32_t cap,
                                                     uint32_t* __restrict__ pair_body,
-                                                    uint32_t* __restrict__ pair_slot,
-                                                    unsigned long long* __restrict__ count) {
+                                                    uint32_t* __restrict__ pair_slot) {
     __shared__ int32_t s_node[kStackDepth];
     __shared__ uint64_t s_mask[kStackDepth];
     const int l = lane_id();
@@ -50,6 +54,7 @@ __global__ __launch_bounds__(64) void k_mlp_collect(const NodeRec* __restrict__ 
     const bool valid = k < samples && n > 1;
     const int32_t me = valid ? (int32_t)(hash_u32(seed * 0x9e3779b97f4a7c15ull + k) % n) : -1;
     const double2 p = valid ? pos[me] : make_double2(0.0, 0.0);
+    uint32_t at = (WRITE && valid) ? counts[k] : 0u;
     const uint64_t act = ballot64(valid);
     if (act == 0) return;
     int sp = 1;
@@ -67,18 +72,12 @@ __global__ __launch_bounds__(64) void k_mlp_collect(const NodeRec* __restrict__ 
         const double d2 = fma(dx, dx, dy * dy);
         const bool inside = me >= bs && me <= be;
         const bool accept = mine && !inside && (r->wsq < theta_sq * d2);
-        const bool take = accept && (uint32_t)(be - bs + 1) <= kPairMaxBodies;
-        const uint64_t tm = ballot64(take);
-        if (tm) {
-            unsigned long long base = 0;
-            if (l == __ffsll((unsigned long long)tm) - 1)
-                base = atomicAdd(count, (unsigned long long)__popcll(tm));
-            base = __shfl(base, __ffsll((unsigned long long)tm) - 1, 64);
-            const uint64_t at = base + __popcll(tm & lanemask_lt());
-            if (take && at < cap) {
+        if (accept && (uint32_t)(be - bs + 1) <= kPairMaxBodies) {
+            if (WRITE && at < cap) {
                 pair_body[at] = (uint32_t)me;
                 pair_slot[at] = (uint32_t)node;
             }
+            at++;
         }
         const uint64_t open = ballot64(mine && !accept);
         if (open && sp + 4 <= kStackDepth) {
@@ -92,6 +91,7 @@ __global__ __launch_bounds__(64) void k_mlp_collect(const NodeRec* __restrict__ 
                 }
         }
     }
+    if (!WRITE && valid) counts[k] = at;
 }
 
 // Features and exact-minus-monopole targets, one wave per pair.
@@ -99,9 +99,9 @@ __global__ __launch_bounds__(256) void k_mlp_targets(
     const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
     const double* __restrict__ mass, const double* __restrict__ box, double eps2,
     const uint32_t* __restrict__ pair_body, const uint32_t* __restrict__ pair_slot,
-    const unsigned long long* __restrict__ count, uint64_t cap, float* __restrict__ X,
+    const uint32_t* __restrict__ total, uint64_t cap, float* __restrict__ X,
     float* __restrict__ T) {
-    const uint64_t npairs = *count < cap ? *count : cap;
+    const uint64_t npairs = *total < cap ? *total : cap;
     const int l = lane_id();
     const float inv_mroot = (float)(1.0 / nodes[0].m);
     const float eps2f = (float)eps2;
@@ -128,13 +128,16 @@ __global__ __launch_bounds__(256) void k_mlp_targets(
             const double dx = r->cx - x.x, dy = r->cy - x.y;
             const double r2 = dx * dx + dy * dy + eps2;
             const double w = r->m / (r2 * sqrt(r2));
-            float f[8];
+            float f[8], ux, uy;
             const float scale = mlp_features((float)dx, (float)dy, (float)sqrt(r->wsq), r->mf,
-                                             r->ixx, r->ixy, r->iyy, eps2f, inv_mroot, inv_ell, f);
+                                             r->ixx, r->ixy, r->iyy, eps2f, inv_mroot, inv_ell, f,
+                                             ux, uy);
 #pragma unroll
             for (int i = 0; i < 8; i++) X[q * 8 + i] = f[i];
-            T[q * 2 + 0] = (float)((ax - dx * w) / (double)scale);
-            T[q * 2 + 1] = (float)((ay - dy * w) / (double)scale);
+            // exact - monopole along (u, w), w = (-uy, ux)
+            const double cx = ax - dx * w, cy = ay - dy * w;
+            T[q * 2 + 0] = (float)((cx * ux + cy * uy) / (double)scale);
+            T[q * 2 + 1] = (float)((cy * ux - cx * uy) / (double)scale);
         }
     }
 }
@@ -289,10 +292,10 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ w, float* 
     w[i] -= lr * mh / (sqrtf(vh) + eps);
 }
 
-// Batch inference on MFMA: each wave loads the weights once and runs 16-row
-// tiles of X.
+// Batch inference on MFMA: each wave loads the weights once and runs 64-row
+// batches of X as four interleaved tiles.
 __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X, uint64_t count,
-                                                     const uint2* __restrict__ wt,
+                                                     const void* __restrict__ wt,
                                                      const float* __restrict__ bias,
                                                      float* __restrict__ Y) {
     __shared__ __attribute__((aligned(16))) float sb[kMlpBiasLen];
@@ -301,49 +304,62 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X
     const int l = lane_id();
     MlpWeights w;
     mlp_load_weights(w, wt, l);
-    const uint64_t tiles = (count + 15) / 16;
+    const uint64_t batches = (count + 63) / 64;
     const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint64_t waves = (uint64_t)gridDim.x * 4;
     const int part = l >> 4;  // 0: f0..f3, 1: f4..f7, 2-3: zero padding
-    for (uint64_t tile = wave; tile < tiles; tile += waves) {
-        const uint64_t col = tile * 16 + (l & 15);
-        f32x4 x = {0.f, 0.f, 0.f, 0.f};
-        if (col < count && part < 2) {
-            x.x = X[col * 8 + part * 4 + 0];
-            x.y = X[col * 8 + part * 4 + 1];
-            x.z = X[col * 8 + part * 4 + 2];
-            x.w = X[col * 8 + part * 4 + 3];
+    for (uint64_t bt = wave; bt < batches; bt += waves) {
+        f32x4 x[4], y[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint64_t row = bt * 64 + 16 * q + (l & 15);
+            x[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+            if (row < count && part < 2) {
+                x[q].x = X[row * 8 + part * 4 + 0];
+                x[q].y = X[row * 8 + part * 4 + 1];
+                x[q].z = X[row * 8 + part * 4 + 2];
+                x[q].w = X[row * 8 + part * 4 + 3];
+            }
         }
-        const f32x4 y = mlp_tile_forward(x, w, sb, l);
-        if (l < 16 && col < count) {
-            Y[col * 2 + 0] = y.x;
-            Y[col * 2 + 1] = y.y;
+        mlp_forward_tiles<2>(x, w, sb, l, y);
+        mlp_forward_tiles<2>(x + 2, w, sb, l, y + 2);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint64_t row = bt * 64 + 16 * q + l;
+            if (l < 16 && row < count) {
+                Y[row * 2 + 0] = y[q].x;
+                Y[row * 2 + 1] = y[q].y;
+            }
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Far-field network in the step: one wavefront per group of 64 bodies. The
-// group's accepted cells (written by the walk) run through the network in
-// 16-cell tiles at the three points of their subgroup; the corrections are
-// summed per subgroup in LDS in a fixed order, then every body adds the
-// first-order interpolation at its own offset to the walk's acceleration and
-// kicks (or stores G a).
+// Far-field network in the step: one wavefront per group of 64 bodies. For
+// each finished subgroup the accepted cells (tree slots written by the walk)
+// go through the network 64 at a time at the subgroup's box centre (lane l
+// prepares cell l; tile q of the MFMA pass is cells 16q..16q+15). Each lane
+// also adds its cell's softened-quadrupole Jacobian, so a subgroup ends with
+// C = sum of the corrections at the centre and K = sum of the Jacobians; a
+// member at offset b from the centre adds C - K b (first order across the
+// subgroup) to the walk's acceleration and kicks (or stores G a). The sums
+// run in a fixed order.
 // ---------------------------------------------------------------------------
 constexpr int kApplyWaves = 4;
 
 template <bool KICK>
 __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
-    const CellRec* __restrict__ cells, const uint32_t* __restrict__ cell_cnt, uint32_t cell_cap,
-    const float2* __restrict__ sub_h, const float4* __restrict__ body_w,
+    const uint32_t* __restrict__ cell_slot, const uint32_t* __restrict__ cell_cnt,
+    uint32_t cell_cap, const SubRec* __restrict__ subs, const float4* __restrict__ body_w,
     double2* __restrict__ acc, double2* __restrict__ vel, uint32_t n,
     const StepConsts* __restrict__ k, uint32_t* __restrict__ meta,
     const NodeRec* __restrict__ nodes, const double* __restrict__ box,
-    const uint2* __restrict__ tiles, const float* __restrict__ bias, int count_step) {
+    const void* __restrict__ tiles, const float* __restrict__ bias, int count_step) {
     if (meta[META_FLAGS] & kHaltFlags) return;
     __shared__ __attribute__((aligned(16))) float sb[kMlpBiasLen];
-    __shared__ float su[kApplyWaves][64 * 6];
-    __shared__ float2 sh[kApplyWaves][64];
+    __shared__ float su[kApplyWaves][6 * 64];        // [C_x, C_y, K00, K01, K10, K11][subgroup]
+    __shared__ __attribute__((aligned(16))) float sf[kApplyWaves][64 * 8];  // cell inputs
+    __shared__ float2 sr[kApplyWaves][64];           // cell scale * u
     for (int i = threadIdx.x; i < kMlpBiasLen; i += 64 * kApplyWaves) sb[i] = bias[i];
     if (count_step && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
     __syncthreads();
@@ -352,85 +368,98 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     const uint32_t g = blockIdx.x * kApplyWaves + wv;
     if (g >= groups) return;
     float* u = su[wv];
-#pragma unroll
-    for (int q = 0; q < 6; q++) u[q * 64 + l] = 0.f;
-    sh[wv][l] = sub_h[(uint64_t)g * 64 + l];
-    __builtin_amdgcn_wave_barrier();
+    float* f_all = sf[wv];
     const float eps2f = (float)k->eps2;
     const float inv_mroot = (float)(1.0 / nodes[0].m);
     const float inv_ell = mlp_inv_ell(eps2f, box[4]);
     MlpWeights w;
     mlp_load_weights(w, tiles, l);
-    const uint32_t cnt = min(cell_cnt[g], cell_cap);
-    const CellRec* cg = cells + (uint64_t)g * cell_cap;
-    const int col = l & 15, part = l >> 4;
-    for (uint32_t t0 = 0; t0 < cnt; t0 += 16) {
-        const uint32_t cell = t0 + col;
-        const bool ok = cell < cnt;
-        CellRec cr;
-        cr.a = make_float4(1.f, 0.f, 1.f, 1.f);
-        cr.b = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (ok) cr = cg[cell];
-        const int sub = ok ? (__float_as_int(cr.b.w) & 63) : 0;
-        const float2 h = sh[wv][sub];
-        float cx[3], cy[3];
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const float ox = q == 1 ? h.x : 0.f, oy = q == 2 ? h.y : 0.f;
-            float f[8];
-            const float scale = mlp_features(cr.a.x - ox, cr.a.y - oy, cr.a.w, cr.a.z, cr.b.x,
-                                             cr.b.y, cr.b.z, eps2f, inv_mroot, inv_ell, f);
-            f32x4 x = {0.f, 0.f, 0.f, 0.f};
-            if (ok && part < 2) {
-                x.x = part ? f[4] : f[0];
-                x.y = part ? f[5] : f[1];
-                x.z = part ? f[6] : f[2];
-                x.w = part ? f[7] : f[3];
+    const int nsub = (int)min(cell_cnt[g], 64u);
+    const uint32_t* cg = cell_slot + (uint64_t)g * cell_cap;
+    const int col = l & 15;
+    for (int ks = 0; ks < nsub; ks++) {
+        const SubRec sub = subs[(uint64_t)g * 64 + ks];
+        const uint32_t end = min(sub.start + sub.count, cell_cap);
+        float cx = 0.f, cy = 0.f, k00 = 0.f, k01 = 0.f, k10 = 0.f, k11 = 0.f;
+        for (uint32_t b0 = sub.start; b0 < end; b0 += 64) {
+            const uint32_t cell = b0 + l;
+            const bool ok = cell < end;
+            float dx = 1.f, dy = 0.f, s = 1.f, M = 1.f, ixx = 0.f, ixy = 0.f, iyy = 0.f;
+            if (ok) {
+                const NodeRec* r = nodes + cg[cell];
+                dx = (float)(r->cx - sub.ox);
+                dy = (float)(r->cy - sub.oy);
+                s = (float)sqrt(r->wsq);
+                M = r->mf;
+                ixx = r->ixx;
+                ixy = r->ixy;
+                iyy = r->iyy;
             }
-            const f32x4 y = mlp_tile_forward(x, w, sb, l);
-            const bool use = ok && part == 0;
-            cx[q] = use ? y.x * scale : 0.f;
-            cy[q] = use ? y.y * scale : 0.f;
+            float f[8], ux, uy, J[4];
+            float scale = mlp_features(dx, dy, s, M, ixx, ixy, iyy, eps2f, inv_mroot, inv_ell, f,
+                                       ux, uy);
+            quad_jacobian(dx, dy, ixx, ixy, iyy, eps2f, J);
+            if (ok) {
+                k00 += J[0];
+                k01 += J[1];
+                k10 += J[2];
+                k11 += J[3];
+            } else {
+                scale = 0.f;
+            }
+            *reinterpret_cast<float4*>(f_all + 8 * l) = make_float4(f[0], f[1], f[2], f[3]);
+            *reinterpret_cast<float4*>(f_all + 8 * l + 4) = make_float4(f[4], f[5], f[6], f[7]);
+            sr[wv][l] = make_float2(scale * ux, scale * uy);
+            __builtin_amdgcn_wave_barrier();
+            f32x4 x[4], y[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                x[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+                if (l < 32) {
+                    const float4 v = *reinterpret_cast<const float4*>(
+                        f_all + 8 * (16 * q + col) + 4 * (l >> 4));
+                    x[q] = f32x4{v.x, v.y, v.z, v.w};
+                }
+            }
+            mlp_forward_tiles<2>(x, w, sb, l, y);
+            mlp_forward_tiles<2>(x + 2, w, sb, l, y + 2);
+            // lanes 0-15: cells 16q + l, rotated back to (x, y)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float2 su_ = sr[wv][(16 * q + l) & 63];
+                if (l < 16) {
+                    cx += y[q].x * su_.x - y[q].y * su_.y;
+                    cy += y[q].x * su_.y + y[q].y * su_.x;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        // Add the 16 columns to their subgroups' sums, in column order.
-        const int s0 = __shfl(sub, 0, 64);
-        if (ballot64(l < 16 && ok && sub != s0) == 0) {
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-#pragma unroll
-                for (int d = 1; d < 16; d <<= 1) {
-                    cx[q] += __shfl_xor(cx[q], d, 64);
-                    cy[q] += __shfl_xor(cy[q], d, 64);
-                }
-            }
-            if (l == 0) {
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    u[(2 * q) * 64 + s0] += cx[q];
-                    u[(2 * q + 1) * 64 + s0] += cy[q];
-                }
-            }
-        } else {
-            for (int j = 0; j < 16; j++) {
-                if (l == j && ok) {
-#pragma unroll
-                    for (int q = 0; q < 3; q++) {
-                        u[(2 * q) * 64 + sub] += cx[q];
-                        u[(2 * q + 1) * 64 + sub] += cy[q];
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-            }
+        for (int d = 1; d < 64; d <<= 1) {
+            cx += __shfl_xor(cx, d, 64);
+            cy += __shfl_xor(cy, d, 64);
+            k00 += __shfl_xor(k00, d, 64);
+            k01 += __shfl_xor(k01, d, 64);
+            k10 += __shfl_xor(k10, d, 64);
+            k11 += __shfl_xor(k11, d, 64);
         }
+        if (l == 0) {
+            u[0 * 64 + ks] = cx;
+            u[1 * 64 + ks] = cy;
+            u[2 * 64 + ks] = k00;
+            u[3 * 64 + ks] = k01;
+            u[4 * 64 + ks] = k10;
+            u[5 * 64 + ks] = k11;
+        }
+        __builtin_amdgcn_wave_barrier();
     }
-    __builtin_amdgcn_wave_barrier();
     const uint32_t i = g * 64 + l;
     if (i >= n) return;
     const float4 bw = body_w[i];
-    const int sub = __float_as_int(bw.z) & 63;
-    const float u0 = u[0 * 64 + sub], u1 = u[1 * 64 + sub];
-    const float ax = u0 + bw.x * (u[2 * 64 + sub] - u0) + bw.y * (u[4 * 64 + sub] - u0);
-    const float ay = u1 + bw.x * (u[3 * 64 + sub] - u1) + bw.y * (u[5 * 64 + sub] - u1);
+    const int ks = __float_as_int(bw.z) & 63;
+    // correction at offset b: C - K b
+    const float ax = u[0 * 64 + ks] - (u[2 * 64 + ks] * bw.x + u[3 * 64 + ks] * bw.y);
+    const float ay = u[1 * 64 + ks] - (u[4 * 64 + ks] * bw.x + u[5 * 64 + ks] * bw.y);
     const double2 a0 = acc[i];
     const double2 a = make_double2(a0.x + (double)ax, a0.y + (double)ay);
     if (KICK) {
@@ -451,7 +480,7 @@ void launch_mlp_apply(nb_ctx* c, int src, bool kick, bool count_step) {
     const int count = count_step ? 1 : 0;
 #define NB_APPLY(K)                                                                            \
     hipLaunchKernelGGL((k_mlp_apply<K>), dim3(blocks), dim3(64 * kApplyWaves), 0, c->stream,   \
-                       c->b.cells, c->b.cell_cnt, c->cell_cap, c->b.sub_h, c->b.body_w,        \
+                       c->b.cell_slot, c->b.cell_cnt, c->cell_cap, c->b.subs, c->b.body_w,     \
                        c->b.acc, c->b.vel[src], n, c->b.consts, c->b.meta, c->b.nodes,         \
                        c->b.box, c->b.mlp_tiles, c->b.mlp_bias, count)
     if (kick) NB_APPLY(true); else NB_APPLY(false);
@@ -484,32 +513,34 @@ int mlp_fail(nb_ctx* c, int code, const std::string& m) {
                             std::string(#expr) + ": " + hipGetErrorString(e_));           \
     } while (0)
 
-// Packs the FP32 canonical parameters into the MFMA A tiles (BF16, per-lane
-// layout: lane l of tile T holds A[l%16][4(l/16) + v], v = 0..3) and the
-// compact bias vector.
-void pack_tiles(const float* p, std::vector<uint2>& tiles, std::vector<float>& bias) {
+// Packs the FP32 canonical parameters into the MFMA A tiles (BF16, round to
+// nearest even) in the lane layouts of mlp_forward_tiles, and the compact
+// bias vector.
+void pack_tiles(const float* p, std::vector<uint8_t>& tiles, std::vector<float>& bias) {
     using O = MlpOffsets;
-    tiles.assign(kMlpTiles * 64, uint2{0, 0});
+    tiles.assign(kMlpTileBytes, 0);
     bias.assign(kMlpBiasLen, 0.f);
-    auto W = [&](int T, int row, int col) -> float {
-        if (T < 4) {  // W1[64][8], k >= 8 zero
-            const int n = 16 * T + row;
-            return col < 8 ? p[O::W1 + n * 8 + col] : 0.f;
-        }
-        if (T < 20) {
-            const int nt = (T - 4) / 4, kt = (T - 4) % 4;
-            return p[O::W2 + (16 * nt + row) * 64 + 16 * kt + col];
-        }
-        const int kt = T - 20;
-        return row < 2 ? p[O::W3 + row * 64 + 16 * kt + col] : 0.f;
-    };
-    for (int T = 0; T < kMlpTiles; T++)
-        for (int l = 0; l < 64; l++) {
-            uint16_t h[4];
-            for (int v = 0; v < 4; v++) h[v] = f32_to_bf16(W(T, l % 16, 4 * (l / 16) + v));
-            tiles[T * 64 + l] = uint2{(uint32_t)h[0] | ((uint32_t)h[1] << 16),
-                                      (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
-        }
+    uint16_t* t1 = reinterpret_cast<uint16_t*>(tiles.data());                 // [4][64][4]
+    uint16_t* t23 = t1 + kMlpTiles1 * 64 * 4;                                   // [10][64][8]
+    for (int t = 0; t < kMlpTiles1; t++)
+        for (int l = 0; l < 64; l++)
+            for (int v = 0; v < 4; v++) {
+                const int m = l % 16, kk = 4 * (l / 16) + v;
+                const float x = kk < 8 ? p[O::W1 + (16 * t + m) * 8 + kk] : 0.f;
+                t1[(t * 64 + l) * 4 + v] = f32_to_bf16(x);
+            }
+    for (int t = 0; t < kMlpTiles23; t++)
+        for (int l = 0; l < 64; l++)
+            for (int v = 0; v < 8; v++) {
+                const int m = l % 16, j = 8 * (l / 16) + v;
+                float x;
+                if (t < 8) {  // layer 2: output tile t/2, K block t%2
+                    x = p[O::W2 + (16 * (t / 2) + m) * 64 + mlp_k_neuron(t % 2, j)];
+                } else {      // layer 3: K block t-8
+                    x = m < 2 ? p[O::W3 + m * 64 + mlp_k_neuron(t - 8, j)] : 0.f;
+                }
+                t23[(t * 64 + l) * 8 + v] = f32_to_bf16(x);
+            }
     for (int i = 0; i < 64; i++) bias[i] = p[O::b1 + i];
     for (int i = 0; i < 64; i++) bias[64 + i] = p[O::b2 + i];
     bias[128] = p[O::b3];
@@ -522,23 +553,23 @@ int ensure_mlp(nb_ctx* c) {
     MLP_HIP(c, hipMalloc((void**)&b.mlp_w, kMlpParams * sizeof(float)));
     MLP_HIP(c, hipMalloc((void**)&b.mlp_m, kMlpParams * sizeof(float)));
     MLP_HIP(c, hipMalloc((void**)&b.mlp_v, kMlpParams * sizeof(float)));
-    MLP_HIP(c, hipMalloc((void**)&b.mlp_tiles, kMlpTiles * 64 * sizeof(uint2)));
+    MLP_HIP(c, hipMalloc((void**)&b.mlp_tiles, kMlpTileBytes));
     MLP_HIP(c, hipMalloc((void**)&b.mlp_bias, kMlpBiasLen * sizeof(float)));
     MLP_HIP(c, hipMemsetAsync(b.mlp_w, 0, kMlpParams * sizeof(float), c->stream));
-    MLP_HIP(c, hipMemsetAsync(b.mlp_tiles, 0, kMlpTiles * 64 * sizeof(uint2), c->stream));
+    MLP_HIP(c, hipMemsetAsync(b.mlp_tiles, 0, kMlpTileBytes, c->stream));
     MLP_HIP(c, hipMemsetAsync(b.mlp_bias, 0, kMlpBiasLen * sizeof(float), c->stream));
     MLP_HIP(c, hipStreamSynchronize(c->stream));
-    c->device_bytes += kMlpParams * 3 * sizeof(float) + kMlpTiles * 64 * sizeof(uint2) +
+    c->device_bytes += kMlpParams * 3 * sizeof(float) + kMlpTileBytes +
                        kMlpBiasLen * sizeof(float);
     return NB_OK;
 }
 
 int upload_weights(nb_ctx* c, const float* p) {
-    std::vector<uint2> tiles;
+    std::vector<uint8_t> tiles;
     std::vector<float> bias;
     pack_tiles(p, tiles, bias);
     MLP_HIP(c, scopy(c, c->b.mlp_w, p, kMlpParams * sizeof(float), hipMemcpyHostToDevice));
-    MLP_HIP(c, scopy(c, c->b.mlp_tiles, tiles.data(), tiles.size() * sizeof(uint2),
+    MLP_HIP(c, scopy(c, c->b.mlp_tiles, tiles.data(), tiles.size(),
                          hipMemcpyHostToDevice));
     MLP_HIP(c, scopy(c, c->b.mlp_bias, bias.data(), bias.size() * sizeof(float),
                          hipMemcpyHostToDevice));
@@ -557,7 +588,6 @@ int ensure_pairs(nb_ctx* c, uint64_t cap) {
     MLP_HIP(c, hipMalloc((void**)&b.pair_slot, cap * sizeof(uint32_t)));
     MLP_HIP(c, hipMalloc((void**)&b.pair_x, cap * 8 * sizeof(float)));
     MLP_HIP(c, hipMalloc((void**)&b.pair_t, cap * 2 * sizeof(float)));
-    if (!b.pair_count) MLP_HIP(c, hipMalloc((void**)&b.pair_count, sizeof(unsigned long long)));
     c->pair_cap = cap;
     return NB_OK;
 }
@@ -582,19 +612,19 @@ void free_mlp(nb_ctx* c) {
     Buffers& b = c->b;
     for (void* p : {(void*)b.mlp_w, (void*)b.mlp_m, (void*)b.mlp_v, (void*)b.mlp_tiles,
                     (void*)b.mlp_bias, (void*)b.mlp_gpart, (void*)b.mlp_loss, (void*)b.pair_body,
-                    (void*)b.pair_slot, (void*)b.pair_x, (void*)b.pair_t, (void*)b.pair_count,
-                    (void*)b.cells, (void*)b.cell_cnt, (void*)b.sub_h, (void*)b.body_w})
+                    (void*)b.pair_slot, (void*)b.pair_x, (void*)b.pair_t, (void*)b.sample_off,
+                    (void*)b.cell_slot, (void*)b.cell_cnt, (void*)b.subs, (void*)b.body_w})
         if (p) (void)hipFree(p);
-    b.cells = nullptr;
+    b.cell_slot = nullptr;
     b.cell_cnt = nullptr;
-    b.sub_h = nullptr;
+    b.subs = nullptr;
     b.body_w = nullptr;
     c->cell_cap = 0;
     b.mlp_w = b.mlp_m = b.mlp_v = b.mlp_gpart = b.mlp_loss = b.pair_x = b.pair_t = nullptr;
     b.mlp_tiles = nullptr;
     b.mlp_bias = nullptr;
     b.pair_body = b.pair_slot = nullptr;
-    b.pair_count = nullptr;
+    b.sample_off = nullptr;
 }
 
 int prepare_mlp(nb_ctx* c) { return ensure_mlp(c); }
@@ -602,18 +632,18 @@ int prepare_mlp(nb_ctx* c) { return ensure_mlp(c); }
 int alloc_cells(nb_ctx* c, uint32_t cap) {
     Buffers& b = c->b;
     const uint64_t groups = (c->n + 63) / 64;
-    if (b.cells) (void)hipFree(b.cells);
-    b.cells = nullptr;
-    if (c->cell_cap) c->device_bytes -= groups * c->cell_cap * sizeof(CellRec);
+    if (b.cell_slot) (void)hipFree(b.cell_slot);
+    b.cell_slot = nullptr;
+    if (c->cell_cap) c->device_bytes -= groups * c->cell_cap * sizeof(uint32_t);
     c->cell_cap = 0;
-    MLP_HIP(c, hipMalloc((void**)&b.cells, groups * cap * sizeof(CellRec)));
+    MLP_HIP(c, hipMalloc((void**)&b.cell_slot, groups * cap * sizeof(uint32_t)));
     if (!b.cell_cnt) {
         MLP_HIP(c, hipMalloc((void**)&b.cell_cnt, groups * sizeof(uint32_t)));
-        MLP_HIP(c, hipMalloc((void**)&b.sub_h, groups * 64 * sizeof(float2)));
+        MLP_HIP(c, hipMalloc((void**)&b.subs, groups * 64 * sizeof(SubRec)));
         MLP_HIP(c, hipMalloc((void**)&b.body_w, c->n * sizeof(float4)));
-        c->device_bytes += groups * (sizeof(uint32_t) + 64 * sizeof(float2)) + c->n * sizeof(float4);
+        c->device_bytes += groups * (sizeof(uint32_t) + 64 * sizeof(SubRec)) + c->n * sizeof(float4);
     }
-    c->device_bytes += groups * cap * sizeof(CellRec);
+    c->device_bytes += groups * cap * sizeof(uint32_t);
     c->cell_cap = cap;
     return NB_OK;
 }
@@ -664,25 +694,38 @@ int nb_mlp_generate(nb_ctx* c, double theta, uint32_t sample_bodies, uint64_t ma
     if (c->p.algorithm != NB_ALGO_BARNES_HUT)
         return mlp_fail(c, NB_ERR_NOT_SUPPORTED, "pairs need a Barnes-Hut context");
     if (c->n < 2) return mlp_fail(c, NB_ERR_INVALID_ARG, "need at least two bodies");
+    if (max_pairs == 0 || max_pairs > 0xffffffffull)
+        return mlp_fail(c, NB_ERR_INVALID_ARG, "max_pairs must be in [1, 2^32)");
+    // The scan of per-sample counts uses the context's scan scratch, sized for n.
+    const uint32_t samples = (uint32_t)std::min<uint64_t>(sample_bodies, c->n);
     int rc = build_tree(c);
     if (rc) return rc;
-    rc = ensure_pairs(c, max_pairs ? max_pairs : 1);
+    rc = ensure_pairs(c, max_pairs);
     if (rc) return rc;
     Buffers& b = c->b;
-    MLP_HIP(c, hipMemsetAsync(b.pair_count, 0, sizeof(unsigned long long), c->stream));
-    const uint32_t waves = (sample_bodies + 63) / 64;
-    hipLaunchKernelGGL(k_mlp_collect, dim3(waves ? waves : 1), dim3(64), 0, c->stream, b.nodes,
-                       b.pos[c->cur], (uint32_t)c->n, sample_bodies, seed, theta * theta,
-                       (uint64_t)c->pair_cap, b.pair_body, b.pair_slot, b.pair_count);
+    if (!b.sample_off) {
+        MLP_HIP(c, hipMalloc((void**)&b.sample_off, (c->n + 2) * sizeof(uint32_t)));
+        c->device_bytes += (c->n + 2) * sizeof(uint32_t);
+    }
+    uint32_t* total = b.sample_off + c->n + 1;
+    MLP_HIP(c, hipMemsetAsync(total, 0, sizeof(uint32_t), c->stream));
+    const uint32_t waves = std::max<uint32_t>(1, (samples + 63) / 64);
+    const uint32_t cap = (uint32_t)c->pair_cap;
+    hipLaunchKernelGGL(k_mlp_collect<false>, dim3(waves), dim3(64), 0, c->stream, b.nodes,
+                       b.pos[c->cur], (uint32_t)c->n, samples, seed, theta * theta,
+                       b.sample_off, cap, b.pair_body, b.pair_slot);
+    if (samples) launch_scan_u32(c, b.sample_off, samples, total);
+    hipLaunchKernelGGL(k_mlp_collect<true>, dim3(waves), dim3(64), 0, c->stream, b.nodes,
+                       b.pos[c->cur], (uint32_t)c->n, samples, seed, theta * theta,
+                       b.sample_off, cap, b.pair_body, b.pair_slot);
     hipLaunchKernelGGL(k_mlp_targets, dim3(2048), dim3(256), 0, c->stream, b.nodes,
                        b.pos[c->cur], b.mass[c->cur], b.box,
-                       c->p.softening * c->p.softening, b.pair_body, b.pair_slot, b.pair_count,
+                       c->p.softening * c->p.softening, b.pair_body, b.pair_slot, total,
                        (uint64_t)c->pair_cap, b.pair_x, b.pair_t);
     MLP_HIP(c, hipGetLastError());
-    unsigned long long cnt = 0;
-    MLP_HIP(c, hipMemcpyAsync(&cnt, b.pair_count, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
-    MLP_HIP(c, hipStreamSynchronize(c->stream));
-    c->pair_count = cnt < c->pair_cap ? cnt : c->pair_cap;
+    uint32_t cnt = 0;
+    MLP_HIP(c, scopy(c, &cnt, total, sizeof(cnt), hipMemcpyDeviceToHost));
+    c->pair_count = std::min<uint64_t>(cnt, c->pair_cap);
     if (pairs_out) *pairs_out = c->pair_count;
     return NB_OK;
 }
@@ -793,6 +836,47 @@ int nb_mlp_grad(nb_ctx* c, const float* features, const float* targets, uint64_t
     return NB_OK;
 }
 
+int nb_debug_far_field(nb_ctx* c, uint32_t* nsub, double* sub_centre, uint32_t* sub_range,
+                       uint32_t* cell_slots, int32_t* body_sub, float* body_off,
+                       uint32_t* cell_cap_out) {
+    if (!c) return NB_ERR_INVALID_ARG;
+    if (!c->b.cell_slot)
+        return mlp_fail(c, NB_ERR_NOT_SUPPORTED, "no learned far field in this context");
+    const uint64_t n = c->n, groups = (n + 63) / 64;
+    if (cell_cap_out) *cell_cap_out = c->cell_cap;
+    if (nsub) MLP_HIP(c, scopy(c, nsub, c->b.cell_cnt, groups * 4, hipMemcpyDeviceToHost));
+    if (sub_centre || sub_range) {
+        std::vector<SubRec> h(groups * 64);
+        MLP_HIP(c, scopy(c, h.data(), c->b.subs, h.size() * sizeof(SubRec),
+                         hipMemcpyDeviceToHost));
+        for (uint64_t k = 0; k < h.size(); k++) {
+            if (sub_centre) {
+                sub_centre[2 * k] = h[k].ox;
+                sub_centre[2 * k + 1] = h[k].oy;
+            }
+            if (sub_range) {
+                sub_range[2 * k] = h[k].start;
+                sub_range[2 * k + 1] = h[k].count;
+            }
+        }
+    }
+    if (cell_slots)
+        MLP_HIP(c, scopy(c, cell_slots, c->b.cell_slot, groups * c->cell_cap * 4,
+                         hipMemcpyDeviceToHost));
+    if (body_sub || body_off) {
+        std::vector<float4> h(n);
+        MLP_HIP(c, scopy(c, h.data(), c->b.body_w, n * sizeof(float4), hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < n; i++) {
+            if (body_sub) std::memcpy(&body_sub[i], &h[i].z, sizeof(int32_t));
+            if (body_off) {
+                body_off[2 * i] = h[i].x;
+                body_off[2 * i + 1] = h[i].y;
+            }
+        }
+    }
+    return NB_OK;
+}
+
 int nb_mlp_forward(nb_ctx* c, const float* features, uint64_t count, float* out) {
     if (!c || (count && (!features || !out))) return NB_ERR_INVALID_ARG;
     int rc = ensure_mlp(c);
@@ -803,8 +887,8 @@ int nb_mlp_forward(nb_ctx* c, const float* features, uint64_t count, float* out)
     if (e == hipSuccess)
         e = scopy(c, dx, features, count * 8 * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        const uint64_t tiles = (count + 15) / 16;
-        const uint64_t blocks = std::min<uint64_t>((tiles + 3) / 4, 2048);
+        const uint64_t batches = (count + 63) / 64;
+        const uint64_t blocks = std::min<uint64_t>((batches + 3) / 4, 2048);
         hipLaunchKernelGGL(k_mlp_forward, dim3((uint32_t)blocks), dim3(256), 0,
                            c->stream, dx, count, c->b.mlp_tiles, c->b.mlp_bias, dy);
         e = hipGetLastError();

@@ -29,7 +29,7 @@ FAR_MONOPOLE = 0
 FAR_LEARNED = 1
 TRAV_EXACT = 0
 TRAV_GROUP = 1
-PHASES = ("bbox_morton", "sort", "build", "force", "drift")
+PHASES = ("bbox_morton", "sort", "build", "force", "drift", "far_field")
 
 # Every symbol include/nbody.h declares (checked by tests/test_abi_symbols.py).
 EXPORTS = (
@@ -38,7 +38,7 @@ EXPORTS = (
     "nb_last_error", "nb_status_string", "nb_debug_build", "nb_debug_tree",
     "nb_direct_sum_f64", "nb_mlp_set_weights", "nb_mlp_get_weights", "nb_mlp_init",
     "nb_mlp_generate", "nb_mlp_get_pairs", "nb_mlp_set_pairs", "nb_mlp_train", "nb_mlp_grad",
-    "nb_mlp_forward",
+    "nb_mlp_forward", "nb_debug_far_field",
 )
 MLP_PARAMS = 4866
 MLP_INPUTS = 8
@@ -49,7 +49,8 @@ class Params(C.Structure):
         ("G", C.c_double), ("dt", C.c_double), ("theta", C.c_double), ("softening", C.c_double),
         ("algorithm", C.c_int32), ("integrator", C.c_int32), ("self_guard", C.c_int32),
         ("far_field", C.c_int32), ("use_graph", C.c_int32), ("traversal", C.c_int32),
-        ("group_budget", C.c_int32), ("reserved", C.c_int32 * 5),
+        ("group_budget", C.c_int32), ("cell_capacity", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -58,6 +59,7 @@ class Info(C.Structure):
         ("n", C.c_uint64), ("iteration", C.c_uint64), ("simulated_time", C.c_double),
         ("tree_nodes", C.c_uint64), ("internal_nodes", C.c_uint64),
         ("node_capacity", C.c_uint64), ("device_bytes", C.c_uint64),
+        ("far_field_cells", C.c_uint64),
     ]
 
 
@@ -108,6 +110,7 @@ def lib() -> C.CDLL:
                                    C.POINTER(C.c_float)]
         L.nb_mlp_grad.argtypes = [P, P, P, C.c_uint64, P, C.POINTER(C.c_float)]
         L.nb_mlp_forward.argtypes = [P, P, C.c_uint64, P]
+        L.nb_debug_far_field.argtypes = [P, P, P, P, P, P, P, C.POINTER(C.c_uint32)]
         del dp
         _lib = L
     return _lib
@@ -272,6 +275,22 @@ class Context:
         y = np.empty((x.shape[0], 2), dtype=np.float32)
         self._check(lib().nb_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(y)))
         return y
+
+
+    def debug_far_field(self):
+        cap = C.c_uint32()
+        self._check(lib().nb_debug_far_field(self._h, None, None, None, None, None, None,
+                                             C.byref(cap)))
+        groups = (self.n + 63) // 64
+        nsub = np.empty(groups, dtype=np.uint32)
+        centre = np.empty((groups, 64, 2))
+        rng = np.empty((groups, 64, 2), dtype=np.uint32)
+        slots = np.empty((groups, cap.value), dtype=np.uint32)
+        bsub = np.empty(self.n, dtype=np.int32)
+        boff = np.empty((self.n, 2), dtype=np.float32)
+        self._check(lib().nb_debug_far_field(self._h, _ptr(nsub), _ptr(centre), _ptr(rng),
+                                             _ptr(slots), _ptr(bsub), _ptr(boff), C.byref(cap)))
+        return nsub, centre, rng, slots, bsub, boff
 
 
 def direct_sum_f64(pos, mass, G: float, softening: float, targets) -> np.ndarray:

@@ -1053,20 +1053,48 @@ void or_quadrupole(double dx, double dy, double ixx, double ixy, double iyy, dou
     *ay = k * (-tr * dy - 2.0 * idy + 5.0 * dId * dy / R2);
 }
 
+/* Jacobian J[i][j] = dQ_i/dd_j of or_quadrupole, written out from
+ * Q_i = (3/2) R^-5 P_i, P = -tr(I) d - 2 I d + 5 S d R^-2, S = d.Id:
+ *   J_ij = (3/2) R^-5 [ -5 R^-2 d_j P_i - tr(I) d_ij - 2 I_ij
+ *                       + 10 R^-2 (Id)_j d_i + 5 S R^-2 d_ij - 10 S R^-4 d_i d_j ]
+ * (pinned by central differences of or_quadrupole in the tests). */
+void or_quadrupole_jacobian(double dx, double dy, double ixx, double ixy, double iyy,
+                            double eps2, double* J) {
+    const double R2 = dx * dx + dy * dy + eps2, iR2 = 1.0 / R2, iR5 = iR2 * iR2 / sqrt(R2);
+    const double t = ixx + iyy;
+    const double d[2] = {dx, dy};
+    const double Id[2] = {ixx * dx + ixy * dy, ixy * dx + iyy * dy};
+    const double I[2][2] = {{ixx, ixy}, {ixy, iyy}};
+    const double S = dx * Id[0] + dy * Id[1];
+    double P[2];
+    for (int i = 0; i < 2; i++) P[i] = -t * d[i] - 2.0 * Id[i] + 5.0 * S * d[i] * iR2;
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+            const double dij = i == j ? 1.0 : 0.0;
+            J[2 * i + j] = 1.5 * iR5 * (-5.0 * iR2 * d[j] * P[i] - t * dij - 2.0 * I[i][j] +
+                                        10.0 * iR2 * Id[j] * d[i] + 5.0 * S * iR2 * dij -
+                                        10.0 * S * iR2 * iR2 * d[i] * d[j]);
+        }
+}
+
 /* The network inputs of a (cell, point) pair, FP64; returns the output scale
- * M s^2 r / R^5. d = com - point, ell = eps (or the root side when eps = 0). */
+ * M s^2 r / R^5. d = com - point, u = d/r, w = (-u_y, u_x); the second
+ * moments enter in the frame (u, w); ell = eps (or the root side when
+ * eps = 0). Outputs and targets are components along (u, w). */
 double or_mlp_features(double dx, double dy, double s, double M, double ixx, double ixy,
                        double iyy, double eps2, double m_root, double ell, double* f) {
     const double r2 = dx * dx + dy * dy, r = sqrt(r2), R = sqrt(r2 + eps2);
-    f[0] = dx / r;
-    f[1] = dy / r;
+    const double ux = dx / r, uy = dy / r, wx = -uy, wy = ux;
+    const double ms2 = M * s * s;
+    f[0] = ux;
+    f[1] = uy;
     f[2] = s / R;
     f[3] = log2(M / m_root) / 16.0;
-    f[4] = ixx / (M * s * s);
-    f[5] = ixy / (M * s * s);
-    f[6] = iyy / (M * s * s);
+    f[4] = (ux * (ixx * ux + ixy * uy) + uy * (ixy * ux + iyy * uy)) / ms2;
+    f[5] = (wx * (ixx * ux + ixy * uy) + wy * (ixy * ux + iyy * uy)) / ms2;
+    f[6] = (wx * (ixx * wx + ixy * wy) + wy * (ixy * wx + iyy * wy)) / ms2;
     f[7] = log2(r / ell) / 8.0;
-    return M * s * s * r / (R * R * R * R * R);
+    return ms2 * r / (R * R * R * R * R);
 }
 
 /* Exact Plummer acceleration of bodies [bs, be] at (x, y) minus the monopole

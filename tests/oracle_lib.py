@@ -61,6 +61,7 @@ def load(fast: bool = False) -> C.CDLL:
     L.or_mlp_features.argtypes = [d, d, d, d, d, d, d, d, d, d, _P]
     L.or_mlp_features.restype = d
     L.or_cell_correction.argtypes = [_P, _P, i64, i64, d, d, d, _P]
+    L.or_quadrupole_jacobian.argtypes = [d, d, d, d, d, d, _P]
     _libs[name] = L
     return L
 
@@ -251,3 +252,10 @@ def cell_correction(pos, mass, bs, be, x, eps2):
     load().or_cell_correction(ptr(p), ptr(m), int(bs), int(be), float(x[0]), float(x[1]),
                               float(eps2), ptr(out))
     return out
+
+
+def quadrupole_jacobian(d, I, eps2=0.0):
+    J = np.empty(4)
+    load().or_quadrupole_jacobian(float(d[0]), float(d[1]), float(I[0]), float(I[1]),
+                                  float(I[2]), float(eps2), ptr(J))
+    return J.reshape(2, 2)

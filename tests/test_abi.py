@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     assert C.sizeof(abi.Params) == 4 * 8 + 12 * 4
-    assert C.sizeof(abi.Info) == 7 * 8
+    assert C.sizeof(abi.Info) == 8 * 8
     assert abi.NODE_DTYPE.itemsize == 4 * 4 + 4 * 8 + 4 * 4
 
 

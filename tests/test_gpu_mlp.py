@@ -10,9 +10,10 @@
   pairs     generated (cell, body) pairs: opening test, exclusion of the
             body's own cells, inputs and exact-minus-monopole targets against
             the oracle's FP64 restatement on the same tree
-  step      zero weights reproduce the monopole path bit for bit (including
-            a step that grows the accepted-cell regions); graph and eager
-            steps agree
+  step      per body, learned minus monopole equals the oracle's correction
+            (network at the subgroup centre, quadrupole Jacobian across the
+            subgroup) on the walk's own cell lists; growing the accepted-cell
+            regions mid-step changes nothing; graph and eager steps agree
   gain      a network trained on one disk lowers the force error of another
             disk at theta = 0.8 against the FP64 direct sum, well below the
             monopole error at the same theta
@@ -125,29 +126,29 @@ def test_training_pairs_match_oracle():
                                   EPS)
         np.testing.assert_allclose(x[q], f, rtol=2e-5, atol=2e-6)
         corr = O.cell_correction(sp, sm, n["bstart"], n["bend"], sp[body[q]], EPS * EPS)
-        np.testing.assert_allclose(t[q], corr / scale, rtol=1e-4,
-                                   atol=1e-5 * max(1.0, np.abs(corr / scale).max()))
+        u = np.array([f[0], f[1]])
+        want = np.array([corr @ u, corr @ np.array([-u[1], u[0]])]) / scale
+        np.testing.assert_allclose(t[q], want, rtol=1e-4,
+                                   atol=1e-5 * max(1.0, np.abs(want).max()))
 
 
-@pytest.mark.parametrize("theta", [0.5, 0.25])
-def test_zero_network_reproduces_monopole(theta):
-    # theta 0.25 needs more than the initial 1,024 accepted cells per group:
-    # the step halts, grows the regions and runs again
+@pytest.mark.parametrize("theta", [0.5, 0.8])
+def test_learned_walk_matches_monopole_walk(theta):
+    # the learned context walks the same tree with the same opening test:
+    # identical interaction counts, and the correction is a small part of
+    # the acceleration
     m, p, v = ics.exp_disk(16384, seed=6)
-    kw = dict(theta=theta, softening=EPS, traversal=TRAV_GROUP)
+    kw = dict(theta=theta, softening=EPS, traversal=TRAV_GROUP, self_guard=int(theta > 0.71))
+    w = (np.random.default_rng(5).normal(size=4866) * 0.05).astype(np.float32)
     with _ctx(m, p, v, far_field=FAR_MONOPOLE, **kw) as a, \
             _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as b:
-        bytes0 = b.info().device_bytes
-        fa, ia, _ = a.forces()
-        fb, ib, _ = b.forces()
-        assert np.array_equal(fa, fb) and np.array_equal(ia, ib)
-        a.step(3)
-        b.step(3)
-        pa, va = a.state()
-        pb, vb = b.state()
-        assert np.array_equal(pa, pb) and np.array_equal(va, vb)
-        if theta < 0.3:
-            assert b.info().device_bytes > bytes0
+        b.mlp_set_weights(w)
+        fa, ia, ta = a.forces()
+        fb, ib, tb = b.forces()
+        assert np.array_equal(ia, ib) and ta == tb
+        assert b.info().far_field_cells > 0
+    rel = np.linalg.norm(fb - fa, axis=1) / np.linalg.norm(fa, axis=1)
+    assert np.median(rel) < 0.1
 
 
 def _train(m, p, v, theta, steps=4000):
@@ -191,6 +192,80 @@ def test_learned_graph_and_eager_steps_agree():
         b.mlp_set_weights(w)
         a.step(4)
         b.step(4)
+        pa, va = a.state()
+        pb, vb = b.state()
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+
+
+def test_far_field_correction_matches_oracle():
+    """Per body, learned minus monopole equals the oracle's sum over the
+    subgroup's accepted cells of the network's correction at the subgroup
+    centre, minus the summed softened-quadrupole Jacobians times the body's
+    offset from that centre."""
+    m, p, v = ics.exp_disk(4096, seed=12)
+    w = (np.random.default_rng(3).normal(size=4866) * 0.1).astype(np.float32)
+    kw = dict(theta=0.8, softening=EPS, traversal=TRAV_GROUP, self_guard=1)
+    with _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as b, \
+            _ctx(m, p, v, far_field=FAR_MONOPOLE, **kw) as a:
+        b.mlp_set_weights(w)
+        fb, _, _ = b.forces()
+        nsub, centre, rng_, slots, bsub, boff = b.debug_far_field()
+        box, keys, orig, cnt = b.debug_build()
+        nodes = b.debug_tree(cnt)
+        fa, _, _ = a.forces()
+    got = (fb - fa)[orig]                       # sorted order, G = 1
+    eps2 = float(np.float32(EPS * EPS))
+    m_root = nodes[0]["mass"]
+    want = np.zeros_like(got)
+    scale_ref = np.zeros(len(got))
+    checked = 0
+    for g in range(len(nsub)):
+        for k in range(int(nsub[g])):
+            st, ct = rng_[g, k]
+            nd = nodes[slots[g, st:st + ct]]
+            assert np.all(nd["c0"] >= 1)
+            X, S, U = [], [], []
+            K = np.zeros((2, 2))
+            for c in nd:
+                d = np.array([c["cx"], c["cy"]]) - centre[g, k]
+                I = (c["ixx"], c["ixy"], c["iyy"])
+                f, sc = O.mlp_features(d, np.sqrt(c["wsq"]), np.float32(c["mass"]), I, eps2,
+                                       m_root, EPS)
+                X.append(f)
+                S.append(sc)
+                U.append(d / np.linalg.norm(d))
+                K += O.quadrupole_jacobian(d, I, eps2)
+            C = np.zeros(2)
+            if ct:
+                y = O.mlp_forward(w.astype(np.float64), np.array(X, dtype=np.float32))
+                for yy, sc, u in zip(y.astype(np.float64), S, U):
+                    C += sc * (yy[0] * u + yy[1] * np.array([-u[1], u[0]]))
+            members = np.nonzero((bsub == k) & (np.arange(len(bsub)) // 64 == g))[0]
+            for i in members:
+                want[i] = C - K @ boff[i].astype(np.float64)
+                scale_ref[i] = np.abs(C).sum() + np.abs(K @ boff[i]).sum()
+                checked += 1
+    assert checked == len(m)
+    err = np.abs(got - want).max(1) / np.maximum(scale_ref, 1e-30)
+    print(f"[gpu] far-field correction vs oracle: rel err median {np.median(err):.2e} "
+          f"max {err.max():.2e}")
+    assert np.median(err) < 1e-4
+    assert err.max() < 2e-2
+
+
+def test_region_growth_keeps_learned_forces():
+    m, p, v = ics.exp_disk(8192, seed=13)
+    w = (np.random.default_rng(4).normal(size=4866) * 0.1).astype(np.float32)
+    kw = dict(theta=0.8, softening=EPS, traversal=TRAV_GROUP, self_guard=1,
+              far_field=FAR_LEARNED)
+    with _ctx(m, p, v, **kw) as a, _ctx(m, p, v, cell_capacity=8, **kw) as b:
+        a.mlp_set_weights(w)
+        b.mlp_set_weights(w)
+        fa, _, _ = a.forces()
+        fb, _, _ = b.forces()
+        assert np.array_equal(fa, fb)
+        a.step(3)
+        b.step(3)
         pa, va = a.state()
         pb, vb = b.state()
     assert np.array_equal(pa, pb) and np.array_equal(va, vb)

@@ -2,6 +2,7 @@
 
 The reference has no learned correction, no second moments and no network, so
 nothing here has a reference vector: each function is pinned by mathematics.
+  or_quadrupole_jacobian  central differences of or_quadrupole
   or_quadrupole       order of the residual exact - monopole - quadrupole
                       against brute-force sums (a wrong sign or coefficient
                       leaves a second-order residual), with and without eps
@@ -71,11 +72,33 @@ def test_cell_correction_matches_brute_force():
 
 
 def test_mlp_features_hand_values():
-    # d = (3, 4): r = 5, R = sqrt(25 + 11) = 6; s = 2, M = 0.5, m_root = 8, ell = 0.25
+    # d = (3, 4): r = 5, R = sqrt(25 + 11) = 6, u = (0.6, 0.8), w = (-0.8, 0.6);
+    # s = 2, M = 0.5 (M s^2 = 2), m_root = 8, ell = 0.25;
+    # I = (0.2, -0.1, 0.4): Iu = (0.04, 0.26), u.Iu = 0.232, w.Iu = 0.124,
+    # w.Iw = tr I - u.Iu = 0.368
     f, scale = O.mlp_features((3.0, 4.0), 2.0, 0.5, (0.2, -0.1, 0.4), 11.0, 8.0, 0.25)
-    want = [0.6, 0.8, 2.0 / 6.0, -4.0 / 16.0, 0.1, -0.05, 0.2, np.log2(20.0) / 8.0]
-    np.testing.assert_allclose(f, want, rtol=1e-14)
+    want = [0.6, 0.8, 2.0 / 6.0, -4.0 / 16.0, 0.116, 0.062, 0.184, np.log2(20.0) / 8.0]
+    np.testing.assert_allclose(f, want, rtol=1e-13)
     assert scale == pytest.approx(0.5 * 4.0 * 5.0 / 6.0 ** 5, rel=1e-14)
+
+
+@pytest.mark.parametrize("eps2", [0.0, 0.3])
+def test_quadrupole_in_network_units_is_linear_in_moment_inputs(eps2):
+    """In the network's units the quadrupole term is
+    (3/2)(-f4 - f6 - 2 f4 + 5 rho f4, -2 f5) along (u, w), rho = r^2 / R^2."""
+    rng = np.random.default_rng(6)
+    for _ in range(20):
+        d = rng.normal(size=2) * 3
+        s, M = rng.uniform(0.1, 1.0), rng.uniform(0.5, 5.0)
+        a, b = rng.normal(size=2) * 0.3
+        I = np.array([a * a + 0.1, a * b, b * b + 0.05]) * M * s * s
+        f, scale = O.mlp_features(d, s, M, I, eps2, 10.0, 0.1)
+        q = O.quadrupole(d, I, eps2)
+        u = d / np.linalg.norm(d)
+        w = np.array([-u[1], u[0]])
+        rho = (d @ d) / (d @ d + eps2)
+        want = 1.5 * np.array([-f[4] - f[6] - 2 * f[4] + 5 * rho * f[4], -2 * f[5]])
+        np.testing.assert_allclose([q @ u / scale, q @ w / scale], want, rtol=1e-10, atol=1e-12)
 
 
 def _linear_net(A, b):
@@ -154,3 +177,18 @@ def test_mlp_grad_matches_finite_differences():
         assert fd == pytest.approx(g[i], rel=1e-5, abs=1e-8), i
     y = O.mlp_forward(p, x, bf16=False).astype(np.float64)
     assert loss == pytest.approx(((y - t) ** 2).sum(1).mean(), rel=1e-6)
+
+
+@pytest.mark.parametrize("eps2", [0.0, 0.3])
+def test_quadrupole_jacobian_matches_central_differences(eps2):
+    rng = np.random.default_rng(9)
+    for _ in range(20):
+        d = rng.normal(size=2) * 2
+        I = np.array([rng.uniform(0.1, 1), rng.uniform(-0.3, 0.3), rng.uniform(0.1, 1)])
+        J = O.quadrupole_jacobian(d, I, eps2)
+        h = 1e-6 * np.linalg.norm(d)
+        for j in range(2):
+            e = np.zeros(2)
+            e[j] = h
+            fd = (O.quadrupole(d + e, I, eps2) - O.quadrupole(d - e, I, eps2)) / (2 * h)
+            np.testing.assert_allclose(J[:, j], fd, rtol=1e-6, atol=1e-9 * np.abs(J).max())
