@@ -175,6 +175,117 @@ def roofline_walk(leg, traversal_name):
     }
 
 
+def copy_probe(nbytes: int) -> None:
+    """A plain device-to-device copy of known size (nbytes read, nbytes
+    written) whose HBM counters calibrate the traffic of the other kernels."""
+    import torch
+
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"[bench] copy probe: {nbytes} bytes", flush=True)
+
+
+def run_allpairs(nbody2d, ics, args) -> dict:
+    """All-pairs direct sum, N = 262,144 uniform disk (radius 10), masses
+    U[0.5, 1.5], at rest: FP32 LDS-tiled kernel with Kahan sums against the
+    FP64 direct sum on the GPU."""
+    n = args.n if args.n != N_BODIES else 1 << 18
+    mass, pos, vel = ics.uniform_disk(n, seed=SEED)
+    abi = nbody2d.abi
+    ctx = nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, softening=EPS,
+                          algorithm=abi.ALGO_ALL_PAIRS, integrator=abi.INTEG_DK)
+    t = ics.sample_targets(n, args.error_sample, seed=1)
+    ref = nbody2d.direct_sum_f64(pos, mass, 1.0, EPS, t)
+    acc, _, total = ctx.forces()
+    rms, mx = ics.force_errors(acc[t], ref)
+    ctx.step(args.warmup)
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    wall = time.perf_counter() - t0
+    ctx.set_timing(True)
+    ctx.step(args.steps)
+    phase_ms, steps_timed = ctx.timing()
+    ctx.close()
+    force_ms = phase_ms["force"] / steps_timed
+    pairs = float(n) * (n - 1)
+    achieved = pairs * FLOPS_PER_INTERACTION / (force_ms * 1e-3) / 1e12
+    return {
+        "metric": "all-pairs interactions/s (N=262,144 uniform disk)",
+        "value": pairs * args.steps / wall,
+        "unit": "interactions/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"uniform_disk N={n} radius 10, all-pairs direct sum, eps={EPS}, "
+                               f"drift-then-kick, FP32 LDS tiles + Kahan",
+                   "n_bodies": n},
+        "force_error": {"rms": rms, "max": mx, "sample": int(len(t)),
+                        "judge": "GPU FP64 direct sum"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": traffic_from_profile("k_allpairs"),
+                     "kernel": "k_allpairs (FP32 vector; 14 flops per interaction)",
+                     "avg_launch_ms": force_ms},
+        "cpu_baseline": None,
+    }
+
+
+def run_two_disks(nbody2d, ics, args) -> dict:
+    """Plain Barnes-Hut at N = 4,194,304: two exponential disks at x = -8 and
+    +8 approaching at relative speed 0.5, one rotated by 30 degrees; theta 0.5;
+    force error on a seeded 4,096-body sample against the FP64 direct sum at
+    the start and after the steps."""
+    n = args.n if args.n != N_BODIES else 1 << 22
+    mass, pos, vel = ics.two_disks(n, seed=SEED)
+    abi = nbody2d.abi
+    ctx = nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, theta=args.theta_plain, softening=EPS,
+                          traversal=abi.TRAV_GROUP)
+    t = ics.sample_targets(n, args.error_sample, seed=1)
+    ref = nbody2d.direct_sum_f64(pos, mass, 1.0, EPS, t)
+    acc, _, total = ctx.forces()
+    e0 = ics.force_errors(acc[t], ref)
+    ctx.step(args.warmup)
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    wall = time.perf_counter() - t0
+    acc2, _, total2 = ctx.forces()
+    p_now, _ = ctx.state()
+    ref2 = nbody2d.direct_sum_f64(p_now, mass, 1.0, EPS, t)
+    e1 = ics.force_errors(acc2[t], ref2)
+    ctx.set_timing(True)
+    ctx.step(min(args.steps, 10))
+    phase_ms, steps_timed = ctx.timing()
+    ctx.close()
+    per = {k: v / steps_timed for k, v in phase_ms.items()}
+    achieved = total2 * FLOPS_PER_INTERACTION / (per["force"] * 1e-3) / 1e12
+    return {
+        "metric": "N=4M two-disk body-steps/s (plain BH)",
+        "value": n * args.steps / wall, "unit": "body-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"two_disks N={n}, theta={args.theta_plain}, eps={EPS}, "
+                               f"dt={DT}, plain group Barnes-Hut", "n_bodies": n},
+        "force_error": {"rms": e0[0], "max": e0[1], "sample": int(len(t)), "state": "initial",
+                        "judge": "GPU FP64 direct sum"},
+        "force_error_after_steps": {"rms": e1[0], "max": e1[1],
+                                    "steps": args.warmup + args.steps},
+        "interactions_per_body": total / n,
+        "phase_ms_per_step_eager": per,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": None,
+                     "kernel": "k_traverse_group (FP32 vector; 14 flops per interaction)",
+                     "avg_launch_ms": per["force"]},
+        "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +300,11 @@ def main() -> int:
     ap.add_argument("--theta-plain", type=float, default=THETA_PLAIN)
     ap.add_argument("--theta-learned", type=float, default=THETA_LEARNED)
     ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--workload", choices=("disk", "allpairs", "two_disks"), default="disk",
+                    help="disk: the N=1M plain-vs-learned headline; allpairs: N=262,144 "
+                         "direct sum; two_disks: N=4,194,304 plain Barnes-Hut")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="run a device copy of this many bytes first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -197,6 +313,14 @@ def main() -> int:
 
     import nbody2d
     from nbody2d import ics
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+    if args.workload != "disk":
+        run = run_allpairs if args.workload == "allpairs" else run_two_disks
+        print(f"[bench] workload {args.workload}", flush=True)
+        print(json.dumps(run(nbody2d, ics, args)), flush=True)
+        return 0
 
     n = args.n
     mass, pos, vel = ics.exp_disk(n, seed=SEED)

@@ -243,9 +243,9 @@ int nb_mlp_forward(nb_ctx* ctx, const float* features, uint64_t count, float* ou
  * tests. Groups are 64 consecutive bodies of the tree's sorted order
  * (groups = ceil(n/64)). Per group: nsub[g] finished subgroups; per subgroup
  * slot (64 per group): its box centre sub_centre[2k..2k+1] and its range
- * {start, count} in the group's region; cell_slots: the regions' tree slots
- * (groups x *cell_cap_out entries); per sorted body: its subgroup and its
- * offset from that subgroup's centre. Any output may be NULL. */
+ * {start, count} in cell_slots, the tree slots of all regions (groups x
+ * *cell_cap_out entries, indexed from 0); per sorted body: its subgroup and
+ * its offset from that subgroup's centre. Any output may be NULL. */
 int nb_debug_far_field(nb_ctx* ctx, uint32_t* nsub, double* sub_centre, uint32_t* sub_range,
                        uint32_t* cell_slots, int32_t* body_sub, float* body_off,
                        uint32_t* cell_cap_out);

@@ -121,6 +121,7 @@ void free_all(nb_ctx* c) {
     dfree(b.cnt);
     dfree(b.scan_tmp);
     dfree(b.multi_list);
+    dfree(b.queue);
     dfree(b.acc);
     dfree(b.inter);
     dfree(b.ap_part);
@@ -407,6 +408,12 @@ int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* 
     NB_TRY(dalloc(c, &b.cnt, bh ? n : 0));
     NB_TRY(dalloc(c, &b.scan_tmp, scan_blocks));
     NB_TRY(dalloc(c, &b.multi_list, bh ? n / 2 + 1 : 0));
+    if (bh) {
+        // work queue of the group traversal, zero between kernels
+        c->queue_cap = (uint32_t)std::min<uint64_t>(4 * ((n + 63) / 64) + 64, 0xffffffffull);
+        NB_TRY(dalloc(c, &b.queue, c->queue_cap));
+        NB_TRY(hipMemsetAsync(b.queue, 0, c->queue_cap * sizeof(uint64_t), c->stream));
+    }
     NB_TRY(dalloc(c, &b.acc, n));
     NB_TRY(dalloc(c, &b.inter, n));
     NB_TRY(dalloc(c, &b.meta, META_COUNT));

@@ -168,7 +168,8 @@ constexpr int kGFlush = kGList - 64;
 // Before its walk a group is split at its largest Morton jump when the two
 // parts are much more compact than the whole. As a safety net, a group whose
 // interaction list still passes `budget` entries (default 8192) is split into
-// halves and walked again, down to single bodies.
+// pieces of 8 bodies (halves below that), shared with other wavefronts
+// through the work queue, and walked again, down to single bodies.
 constexpr int kGBudget = 8192;
 
 template <bool ZERO_EPS>
@@ -190,11 +191,65 @@ __device__ __forceinline__ void eval_list(const float4* __restrict__ list, int c
     kahan_add(a.ay, a.cy, py);
 }
 
+// Work sharing: a subgroup whose list passes the budget is split into pieces
+// of 8 bodies (halves below that); all pieces but one go to a queue in device memory (meta[META_QHEAD], meta[META_QTAIL]
+// count pushes and pops; an entry is {group, first lane << 8 | lanes} + 1 in
+// the high word so that 0 means "not yet written"), and a wavefront that has
+// finished its own group takes queued subgroups before it exits. Every
+// wavefront that pushes also drains the queue, so each entry is taken once
+// and the queue is empty (all zero) when the kernel ends. Each subgroup's
+// result depends only on its own walk, so the outcome does not depend on
+// which wavefront takes it.
+__device__ __forceinline__ bool queue_push(uint32_t* meta, uint64_t* queue, uint32_t cap,
+                                           uint32_t g, uint32_t sub) {
+    bool ok = false;
+    if (lane_id() == 0) {
+        const uint32_t at = atomicAdd(&meta[META_QHEAD], 1u);
+        if (at < cap) {
+            __threadfence();
+            __hip_atomic_store(&queue[at], ((uint64_t)(sub + 1u) << 32) | g, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            ok = true;
+        }
+    }
+    return __shfl((int)ok, 0, 64) != 0;
+}
+
+__device__ __forceinline__ bool queue_pop(uint32_t* meta, uint64_t* queue, uint32_t cap,
+                                          uint32_t& g, uint32_t& sub) {
+    uint64_t v = 0;
+    if (lane_id() == 0) {
+        for (;;) {
+            const uint32_t h = __hip_atomic_load(&meta[META_QHEAD], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            if (h == 0) break;  // nothing was queued in this launch (the common case)
+            const uint32_t t = __hip_atomic_load(&meta[META_QTAIL], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            if (t >= min(h, cap)) break;
+            if (atomicCAS(&meta[META_QTAIL], t, t + 1u) == t) {
+                // the pusher of entry t has counted it and is about to store it
+                do {
+                    v = __hip_atomic_load(&queue[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                } while ((v >> 32) == 0);
+                __hip_atomic_store(&queue[t], (uint64_t)0, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+    }
+    const uint32_t lo = __shfl((uint32_t)v, 0, 64), hi = __shfl((uint32_t)(v >> 32), 0, 64);
+    g = lo;
+    sub = hi - 1u;
+    return hi != 0;
+}
+
 // LEARN: the walk leaves the far-field network to k_mlp_apply. The tree slot
-// of every accepted internal cell is also appended to the group's region of
-// `cell_slot`; each finished subgroup k records its cell range and its box
-// centre, each member its subgroup and its offset from that centre, and the
-// raw acceleration (without G) goes to acc_out instead of a kick.
+// of every accepted internal cell is also appended to `cell_slot`, in the
+// region of the group the wavefront was launched for; each finished subgroup
+// k of group g gets subs[g][k] (its box centre and its cell range) from a
+// per-group counter, each member its subgroup and its offset from that
+// centre, and the raw acceleration (without G) goes to acc_out instead of a
+// kick.
 template <bool KICK, bool ZERO_EPS, bool GUARD, bool LEARN>
 __global__ __launch_bounds__(64) void k_traverse_group(
     const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
@@ -203,242 +258,259 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     uint32_t* __restrict__ meta, int count_step, int budget,
     const uint64_t* __restrict__ keys, uint32_t* __restrict__ cell_slot,
     uint32_t* __restrict__ cell_cnt, uint32_t cell_cap, SubRec* __restrict__ subs,
-    float4* __restrict__ body_w) {
+    float4* __restrict__ body_w, uint64_t* __restrict__ queue, uint32_t queue_cap) {
     if (meta[META_FLAGS] & kHaltFlags) return;
     __shared__ __attribute__((aligned(16))) int32_t s_stack[kGStack];
     __shared__ float4 s_list[kGList];
     __shared__ int32_t s_todo[16];
     const int l = lane_id();
-    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t i = g * 64 + l;
-    const bool valid = i < n;
+    const uint32_t home = xcd_remap(blockIdx.x, gridDim.x);
     if (count_step && blockIdx.x == 0 && l == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
-    const double2 p = valid ? pos[i] : pos[g * 64];
     const double theta_sq = k->theta_sq;
     const float eps2f = (float)k->eps2;
-    const int32_t me = (int32_t)i;
-    // Morton jump to the next body: digits shared with it (64 = none follows).
-    const uint64_t key = valid ? keys[i] : 0ull;
-    const uint64_t key_next = __shfl_down(key, 1, 64);
-    const int jump = (valid && l < 63 && i + 1 < n) ? __clzll(key ^ key_next) : 64;
-    // This lane's result, filled by the sub-group that finally owns it.
-    Acc fin;
-    uint32_t fin_listed = 0;
-    bool fin_own_multi = false, overflow = false;
-    // Pending sub-groups as (first lane << 8 | lane count); a group whose list
-    // passes the budget is split into halves (a single body is never split:
-    // its group test is its own opening test).
+    bool overflow = false;
+    uint32_t cbase = 0;  // cells written to the home region (LEARN)
+    const uint64_t region = (uint64_t)home * cell_cap;
+    if (LEARN && l == 0)
+        __hip_atomic_store(&cell_cnt[home], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t g = home;
+    // Pending subgroups of group g as (first lane << 8 | lane count); a
+    // single body is never split (its group test is its own opening test).
     int nt = 1;
-    uint32_t cbase = 0;  // cells of the finished subgroups (LEARN)
-    int nsub = 0;        // finished subgroups (LEARN)
     if (l == 0) s_todo[0] = 64;
-    while (nt > 0) {
-        const int32_t e = readfirst(s_todo[--nt]);
-        const int f = e >> 8, cn = e & 255;
-        const bool member = valid && l >= f && l < f + cn;
-        if (ballot64(member) == 0) continue;
-        double xmin = member ? p.x : INFINITY, xmax = member ? p.x : -INFINITY;
-        double ymin = member ? p.y : INFINITY, ymax = member ? p.y : -INFINITY;
+    for (;;) {
+        const uint32_t i = g * 64 + l;
+        const bool valid = i < n;
+        const double2 p = valid ? pos[i] : pos[g * 64];
+        const int32_t me = (int32_t)i;
+        // Morton jump to the next body: digits shared with it (64 = none follows).
+        const uint64_t key = valid ? keys[i] : 0ull;
+        const uint64_t key_next = __shfl_down(key, 1, 64);
+        const int jump = (valid && l < 63 && i + 1 < n) ? __clzll(key ^ key_next) : 64;
+        while (nt > 0 && !overflow) {
+            const int32_t e = readfirst(s_todo[--nt]);
+            const int f = e >> 8, cn = e & 255;
+            const bool member = valid && l >= f && l < f + cn;
+            if (ballot64(member) == 0) continue;
+            double xmin = member ? p.x : INFINITY, xmax = member ? p.x : -INFINITY;
+            double ymin = member ? p.y : INFINITY, ymax = member ? p.y : -INFINITY;
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            xmin = fmin(xmin, __shfl_xor(xmin, d, 64));
-            xmax = fmax(xmax, __shfl_xor(xmax, d, 64));
-            ymin = fmin(ymin, __shfl_xor(ymin, d, 64));
-            ymax = fmax(ymax, __shfl_xor(ymax, d, 64));
-        }
-        // Split at the largest Morton jump inside the group when both parts
-        // are far more compact than the whole (the group straddles a jump of
-        // the curve); no walk is wasted on the wide box.
-        if (cn > 1) {
-            const int my = (member && l < f + cn - 1) ? jump : 65;
-            int best = my;
+            for (int d = 32; d >= 1; d >>= 1) {
+                xmin = fmin(xmin, __shfl_xor(xmin, d, 64));
+                xmax = fmax(xmax, __shfl_xor(xmax, d, 64));
+                ymin = fmin(ymin, __shfl_xor(ymin, d, 64));
+                ymax = fmax(ymax, __shfl_xor(ymax, d, 64));
+            }
+            // Split at the largest Morton jump inside the group when both
+            // parts are far more compact than the whole (the group straddles
+            // a jump of the curve); no walk is wasted on the wide box.
+            if (cn > 1) {
+                const int my = (member && l < f + cn - 1) ? jump : 65;
+                int best = my;
 #pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) best = min(best, __shfl_xor(best, d, 64));
-            const uint64_t at = ballot64(member && my == best);
-            const int cut = __ffsll((unsigned long long)at);  // first lane after the jump
-            if (best < 64 && cut > f && cut < f + cn) {
-                const bool left = l < cut;
-                double ax0 = member && left ? p.x : INFINITY, ax1 = member && left ? p.x : -INFINITY;
-                double ay0 = member && left ? p.y : INFINITY, ay1 = member && left ? p.y : -INFINITY;
-                double bx0 = member && !left ? p.x : INFINITY, bx1 = member && !left ? p.x : -INFINITY;
-                double by0 = member && !left ? p.y : INFINITY, by1 = member && !left ? p.y : -INFINITY;
+                for (int d = 32; d >= 1; d >>= 1) best = min(best, __shfl_xor(best, d, 64));
+                const uint64_t at = ballot64(member && my == best);
+                const int cut = __ffsll((unsigned long long)at);  // first lane after the jump
+                if (best < 64 && cut > f && cut < f + cn) {
+                    const bool left = l < cut;
+                    double ax0 = member && left ? p.x : INFINITY, ax1 = member && left ? p.x : -INFINITY;
+                    double ay0 = member && left ? p.y : INFINITY, ay1 = member && left ? p.y : -INFINITY;
+                    double bx0 = member && !left ? p.x : INFINITY, bx1 = member && !left ? p.x : -INFINITY;
+                    double by0 = member && !left ? p.y : INFINITY, by1 = member && !left ? p.y : -INFINITY;
 #pragma unroll
-                for (int d = 32; d >= 1; d >>= 1) {
-                    ax0 = fmin(ax0, __shfl_xor(ax0, d, 64));
-                    ax1 = fmax(ax1, __shfl_xor(ax1, d, 64));
-                    ay0 = fmin(ay0, __shfl_xor(ay0, d, 64));
-                    ay1 = fmax(ay1, __shfl_xor(ay1, d, 64));
-                    bx0 = fmin(bx0, __shfl_xor(bx0, d, 64));
-                    bx1 = fmax(bx1, __shfl_xor(bx1, d, 64));
-                    by0 = fmin(by0, __shfl_xor(by0, d, 64));
-                    by1 = fmax(by1, __shfl_xor(by1, d, 64));
-                }
-                const double ew = fmax(xmax - xmin, ymax - ymin);
-                const double ea = fmax(ax1 - ax0, ay1 - ay0);
-                const double eb = fmax(bx1 - bx0, by1 - by0);
-                if (ew * ew > 4.0 * (ea * ea + eb * eb) && nt + 2 <= 16) {
-                    if (l == 0) {
-                        s_todo[nt] = (cut << 8) | (f + cn - cut);
-                        s_todo[nt + 1] = (f << 8) | (cut - f);
+                    for (int d = 32; d >= 1; d >>= 1) {
+                        ax0 = fmin(ax0, __shfl_xor(ax0, d, 64));
+                        ax1 = fmax(ax1, __shfl_xor(ax1, d, 64));
+                        ay0 = fmin(ay0, __shfl_xor(ay0, d, 64));
+                        ay1 = fmax(ay1, __shfl_xor(ay1, d, 64));
+                        bx0 = fmin(bx0, __shfl_xor(bx0, d, 64));
+                        bx1 = fmax(bx1, __shfl_xor(bx1, d, 64));
+                        by0 = fmin(by0, __shfl_xor(by0, d, 64));
+                        by1 = fmax(by1, __shfl_xor(by1, d, 64));
                     }
-                    nt += 2;
-                    continue;
+                    const double ew = fmax(xmax - xmin, ymax - ymin);
+                    const double ea = fmax(ax1 - ax0, ay1 - ay0);
+                    const double eb = fmax(bx1 - bx0, by1 - by0);
+                    if (ew * ew > 4.0 * (ea * ea + eb * eb) && nt + 2 <= 16) {
+                        if (l == 0) {
+                            s_todo[nt] = (cut << 8) | (f + cn - cut);
+                            s_todo[nt + 1] = (f << 8) | (cut - f);
+                        }
+                        nt += 2;
+                        continue;
+                    }
                 }
             }
-        }
-        const double ox = 0.5 * (xmin + xmax), oy = 0.5 * (ymin + ymax);
-        const double hx = 0.5 * (xmax - xmin), hy = 0.5 * (ymax - ymin);
-        const float bx = (float)(p.x - ox), by = (float)(p.y - oy);
-        const int32_t g_first = (int32_t)(g * 64 + f);
-        const int32_t g_last = (int32_t)min(g * 64 + f + cn - 1, n - 1);
-        const bool capped = cn > 1;
-        Acc t;
-        int sp = 1, cnt = 0;
-        uint32_t ccnt = 0;  // cells of this subgroup written after cbase
-        uint32_t listed = 0;
-        bool own_multi = false, split = false;
-        if (l == 0) s_stack[0] = 0;
-        while (sp > 0) {
-            int take = min(64, sp);
-            take = min(take, (kGStack - sp) / 3);
-            if (take <= 0) {
-                overflow = true;
-                break;
-            }
-            const int base = sp - take;
-            const bool have = l < take;
-            const int32_t node = have ? s_stack[base + l] : 0;
-            sp = base;
-            NodeRec r;
-            if (have) r = nodes[node];  // one 64-byte record: four independent loads
-            const int32_t c0 = have ? r.c0 : -2;
-            const double dxc = have ? r.cx - ox : 0.0, dyc = have ? r.cy - oy : 0.0;
-            bool add = false, open = false;
-            int32_t bs = 0, be = 0;
-            if (have) {
-                if (c0 < 0) {
-                    bs = r.bs;
-                    be = r.be;
-                    add = bs == be;
-                } else {
-                    const double ex = fmax(fabs(dxc) - hx, 0.0), ey = fmax(fabs(dyc) - hy, 0.0);
-                    const double dmin2 = fma(ex, ex, ey * ey);
-                    add = r.wsq < theta_sq * dmin2;
-                    if (GUARD) add = add && !(r.bs <= g_last && r.be >= g_first);
-                    open = !add;
+            const double ox = 0.5 * (xmin + xmax), oy = 0.5 * (ymin + ymax);
+            const double hx = 0.5 * (xmax - xmin), hy = 0.5 * (ymax - ymin);
+            const float bx = (float)(p.x - ox), by = (float)(p.y - oy);
+            const int32_t g_first = (int32_t)(g * 64 + f);
+            const int32_t g_last = (int32_t)min(g * 64 + f + cn - 1, n - 1);
+            const bool capped = cn > 1;
+            Acc t;
+            int sp = 1, cnt = 0;
+            uint32_t ccnt = 0;  // cells of this subgroup written after cbase
+            uint32_t listed = 0;
+            bool own_multi = false, split = false;
+            if (l == 0) s_stack[0] = 0;
+            while (sp > 0) {
+                int take = min(64, sp);
+                take = min(take, (kGStack - sp) / 3);
+                if (take <= 0) {
+                    overflow = true;
+                    break;
                 }
-            }
-            // append accepted cells / single-body leaves to the list
-            const uint64_t am = ballot64(add);
-            if (add) {
-                const int at = cnt + (int)__popcll(am & lanemask_lt());
-                s_list[at] = make_float4((float)dxc, (float)dyc, r.mf, 0.f);
-            }
-            cnt += (int)__popcll(am);
-            if (LEARN) {
-                const bool cell = add && c0 >= 0;
-                const uint64_t cmk = ballot64(cell);
-                if (cell) {
-                    const uint32_t at = cbase + ccnt + (uint32_t)__popcll(cmk & lanemask_lt());
-                    if (at < cell_cap) cell_slot[(uint64_t)g * cell_cap + at] = (uint32_t)node;
+                const int base = sp - take;
+                const bool have = l < take;
+                const int32_t node = have ? s_stack[base + l] : 0;
+                sp = base;
+                NodeRec r;
+                if (have) r = nodes[node];  // one 64-byte record: four independent loads
+                const int32_t c0 = have ? r.c0 : -2;
+                const double dxc = have ? r.cx - ox : 0.0, dyc = have ? r.cy - oy : 0.0;
+                bool add = false, open = false;
+                int32_t bs = 0, be = 0;
+                if (have) {
+                    if (c0 < 0) {
+                        bs = r.bs;
+                        be = r.be;
+                        add = bs == be;
+                    } else {
+                        const double ex = fmax(fabs(dxc) - hx, 0.0), ey = fmax(fabs(dyc) - hy, 0.0);
+                        const double dmin2 = fma(ex, ex, ey * ey);
+                        add = r.wsq < theta_sq * dmin2;
+                        if (GUARD) add = add && !(r.bs <= g_last && r.be >= g_first);
+                        open = !add;
+                    }
                 }
-                ccnt += (uint32_t)__popcll(cmk);
-            }
-            // push the non-empty children of opened cells
-            const int32_t cm = open ? r.mask : 0;
-            const uint32_t nc = (uint32_t)__popc(cm);
-            const uint32_t incl = wave_incl_scan(nc);
-            const int total = (int)__shfl(incl, 63, 64);
-            if (open) {
-                int at = sp + (int)(incl - nc);
+                // append accepted cells / single-body leaves to the list
+                const uint64_t am = ballot64(add);
+                if (add) {
+                    const int at = cnt + (int)__popcll(am & lanemask_lt());
+                    s_list[at] = make_float4((float)dxc, (float)dyc, r.mf, 0.f);
+                }
+                cnt += (int)__popcll(am);
+                if (LEARN) {
+                    const bool cell = add && c0 >= 0;
+                    const uint64_t cmk = ballot64(cell);
+                    if (cell) {
+                        const uint32_t at = cbase + ccnt + (uint32_t)__popcll(cmk & lanemask_lt());
+                        if (at < cell_cap) cell_slot[region + at] = (uint32_t)node;
+                    }
+                    ccnt += (uint32_t)__popcll(cmk);
+                }
+                // push the non-empty children of opened cells
+                const int32_t cm = open ? r.mask : 0;
+                const uint32_t nc = (uint32_t)__popc(cm);
+                const uint32_t incl = wave_incl_scan(nc);
+                const int total = (int)__shfl(incl, 63, 64);
+                if (open) {
+                    int at = sp + (int)(incl - nc);
 #pragma unroll
-                for (int q = 3; q >= 0; q--)
-                    if ((cm >> q) & 1) s_stack[at++] = c0 + q;
-            }
-            sp += total;
-            // multi-body leaves (level 30): every lane sums their bodies directly
-            uint64_t mm = ballot64(have && c0 < 0 && be > bs);
-            while (mm) {
-                const int src = __ffsll((unsigned long long)mm) - 1;
-                mm &= mm - 1;
-                const int32_t js = __shfl(bs, src, 64), je = __shfl(be, src, 64);
-                own_multi |= (me >= js && me <= je);
-                for (int32_t j = js; j <= je; j++) {
-                    if (j == me) continue;
-                    const double2 q = pos[j];
-                    const double ex = q.x - p.x, ey = q.y - p.y;
-                    interact(t, ex, ey, fma(ex, ex, ey * ey), (float)mass[j], eps2f);
+                    for (int q = 3; q >= 0; q--)
+                        if ((cm >> q) & 1) s_stack[at++] = c0 + q;
+                }
+                sp += total;
+                // multi-body leaves (level 30): every lane sums their bodies directly
+                uint64_t mm = ballot64(have && c0 < 0 && be > bs);
+                while (mm) {
+                    const int src = __ffsll((unsigned long long)mm) - 1;
+                    mm &= mm - 1;
+                    const int32_t js = __shfl(bs, src, 64), je = __shfl(be, src, 64);
+                    own_multi |= (me >= js && me <= je);
+                    for (int32_t j = js; j <= je; j++) {
+                        if (j == me) continue;
+                        const double2 q = pos[j];
+                        const double ex = q.x - p.x, ey = q.y - p.y;
+                        interact(t, ex, ey, fma(ex, ex, ey * ey), (float)mass[j], eps2f);
+                    }
+                }
+                if (capped && listed + (uint32_t)cnt > (uint32_t)budget && nt + 8 <= 16) {
+                    split = true;
+                    break;
+                }
+                if (cnt > kGFlush) {
+                    eval_list<ZERO_EPS>(s_list, cnt, bx, by, eps2f, t);
+                    listed += cnt;
+                    cnt = 0;
                 }
             }
-            if (capped && listed + (uint32_t)cnt > (uint32_t)budget && nt + 2 <= 16) {
-                split = true;
-                break;
+            if (overflow) break;
+            if (split) {
+                // Pieces of 8 bodies (halves below that): all but the first go
+                // to the shared queue (or stay here when it is full), so the
+                // pieces of a heavy group are walked by several wavefronts.
+                const int pieces = cn > 8 ? (cn + 7) / 8 : 2;
+                const int len = (cn + pieces - 1) / pieces;
+                for (int q = pieces - 1; q >= 1; q--) {
+                    const int pf = f + q * len, pc = min(len, f + cn - pf);
+                    if (pc <= 0) continue;
+                    const uint32_t e2 = (uint32_t)((pf << 8) | pc);
+                    if (!queue_push(meta, queue, queue_cap, g, e2)) {
+                        if (l == 0) s_todo[nt] = (int32_t)e2;
+                        nt++;
+                    }
+                }
+                if (l == 0) s_todo[nt] = (f << 8) | min(len, cn);
+                nt++;
+                continue;
             }
-            if (cnt > kGFlush) {
+            if (cnt > 0) {
                 eval_list<ZERO_EPS>(s_list, cnt, bx, by, eps2f, t);
                 listed += cnt;
-                cnt = 0;
+            }
+            // the subgroup is finished: its members write their results
+            if (LEARN) {
+                uint32_t ks = 0;
+                if (l == 0) ks = atomicAdd(&cell_cnt[g], 1u);
+                ks = __shfl(ks, 0, 64) & 63u;
+                if (member) body_w[i] = make_float4(bx, by, __int_as_float((int)ks), 0.f);
+                if (l == 0) {
+                    SubRec sr;
+                    sr.ox = ox;
+                    sr.oy = oy;
+                    sr.start = (uint32_t)(region + cbase);
+                    sr.count = ccnt;
+                    subs[(uint64_t)g * 64 + ks] = sr;
+                }
+                cbase += ccnt;
+            }
+            if (member) {
+                const double ax = (double)t.ax - (double)t.cx;
+                const double ay = (double)t.ay - (double)t.cy;
+                // a body's own single-body leaf is in its group's list (a zero term)
+                const uint32_t terms = listed - (own_multi ? 0u : 1u) + t.n;
+                if (LEARN) {
+                    acc_out[i] = make_double2(ax, ay);
+                    if (!KICK) inter_out[i] = terms;
+                } else if (KICK) {
+                    const double gdt = k->G_dt;
+                    double2 v = vel[i];
+                    v.x += ax * gdt;
+                    v.y += ay * gdt;
+                    vel[i] = v;
+                } else {
+                    const double G = k->G;
+                    acc_out[i] = make_double2(ax * G, ay * G);
+                    inter_out[i] = terms;
+                }
             }
         }
-        if (split) {
-            const int h = cn >> 1;
-            if (l == 0) {
-                s_todo[nt] = ((f + h) << 8) | (cn - h);
-                s_todo[nt + 1] = (f << 8) | h;
-            }
-            nt += 2;
-            continue;
-        }
-        if (overflow) break;
-        if (cnt > 0) {
-            eval_list<ZERO_EPS>(s_list, cnt, bx, by, eps2f, t);
-            listed += cnt;
-        }
-        if (LEARN) {
-            if (member) body_w[i] = make_float4(bx, by, __int_as_float(nsub), 0.f);
-            if (l == 0) {
-                SubRec sr;
-                sr.ox = ox;
-                sr.oy = oy;
-                sr.start = cbase;
-                sr.count = ccnt;
-                subs[(uint64_t)g * 64 + nsub] = sr;
-            }
-            cbase += ccnt;
-            nsub++;
-        }
-        if (member) {
-            fin = t;
-            fin_listed = listed;
-            fin_own_multi = own_multi;
-        }
+        // this group is done (or the stack overflowed): take a queued subgroup
+        uint32_t qg, qsub;
+        if (!queue_pop(meta, queue, queue_cap, qg, qsub)) break;
+        if (overflow) continue;  // drain the queue without walking
+        g = qg;
+        if (l == 0) s_todo[0] = (int32_t)qsub;
+        nt = 1;
     }
     if (overflow && l == 0) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_STACK);
     if (LEARN && l == 0) {
-        cell_cnt[g] = (uint32_t)nsub;
         atomicAdd(&meta[META_CELLS_TOTAL], cbase);
         if (cbase > cell_cap) {
             atomicMax(&meta[META_CELLS_NEEDED], cbase);
             atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_CELL_CAPACITY);
         }
-    }
-    if (!valid) return;
-    const double ax = (double)fin.ax - (double)fin.cx;
-    const double ay = (double)fin.ay - (double)fin.cy;
-    if (LEARN) {
-        acc_out[i] = make_double2(ax, ay);
-        if (!KICK) inter_out[i] = fin_listed - (fin_own_multi ? 0u : 1u) + fin.n;
-    } else if (KICK) {
-        const double gdt = k->G_dt;
-        double2 v = vel[i];
-        v.x += ax * gdt;
-        v.y += ay * gdt;
-        vel[i] = v;
-    } else {
-        const double G = k->G;
-        acc_out[i] = make_double2(ax * G, ay * G);
-        // a body's own single-body leaf is in its group's list (a zero term)
-        inter_out[i] = fin_listed - (fin_own_multi ? 0u : 1u) + fin.n;
     }
 }
 
@@ -456,7 +528,7 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
                        c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src], c->b.acc,     \
                        c->b.inter, n, c->b.consts, c->b.meta, (int)(count_step && !learn),     \
                        budget, c->b.keys[0], c->b.cell_slot, c->b.cell_cnt, c->cell_cap,       \
-                       c->b.subs, c->b.body_w)
+                       c->b.subs, c->b.body_w, c->b.queue, c->queue_cap)
 #define NB_TRAVG3(K, Z, GD) if (learn) NB_TRAVG(false, Z, GD, true); else NB_TRAVG(K, Z, GD, false)
 #define NB_TRAVG2(K, Z) if (gd) { NB_TRAVG3(K, Z, true); } else { NB_TRAVG3(K, Z, false); }
         if (kick) {

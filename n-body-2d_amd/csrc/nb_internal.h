@@ -31,15 +31,19 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;         // keys per thread per tile
 constexpr int kSortTile = kSortThreads * kSortItems;
 
-// Meta words kept in device memory (u32 each).
+// Meta words kept in device memory (u32 each). Words that every wavefront
+// of a kernel updates or polls sit on 128-byte L2 lines of their own: atomics
+// on a line serialise in L2, and would stall the reads of its other words.
 enum Meta : int {
     META_INTERNAL = 0,   // internal nodes of the last build
     META_FLAGS = 1,      // FLAG_* bits of the current step
     META_STEPS_DONE = 2, // completed steps since the host last cleared it
     META_MULTI_LEAVES = 3, // multi-body leaves queued by the current build
     META_CELLS_NEEDED = 4, // most accepted cells of one group (learned far field)
-    META_CELLS_TOTAL = 5,  // accepted cells of all groups (learned far field)
-    META_COUNT = 8
+    META_CELLS_TOTAL = 32, // accepted cells of all groups (learned far field)
+    META_QHEAD = 64,       // traversal work queue: entries pushed
+    META_QTAIL = 96,       // traversal work queue: entries taken
+    META_COUNT = 128
 };
 enum Flag : uint32_t {
     FLAG_TREE_CAPACITY = 1u,  // internal nodes exceed the allocated capacity
@@ -65,7 +69,7 @@ struct __attribute__((aligned(64))) NodeRec {
 static_assert(sizeof(NodeRec) == 64, "NodeRec must be 64 bytes");
 
 // A finished subgroup of a traversal group (learned far field): its box
-// centre and its range of accepted cells in the group's region.
+// centre and its range of accepted cells in cell_slot (absolute indices).
 struct SubRec {
     double ox, oy;
     uint32_t start, count;
@@ -126,6 +130,7 @@ struct Buffers {
     uint32_t* cell_cnt;    // [groups] finished subgroups
     SubRec* subs;          // [groups][64]
     float4* body_w;        // [N] {offset from the subgroup centre (x, y), subgroup, -}
+    uint64_t* queue;       // [queue_cap] traversal work queue (all zero between kernels)
 };
 
 }  // namespace nb
@@ -159,6 +164,7 @@ struct nb_ctx {
     uint32_t train_parts = 0;       // partial-gradient rows allocated
     uint32_t train_step = 0;        // Adam step counter
     uint32_t cell_cap = 0;          // accepted-cell region per group
+    uint32_t queue_cap = 0;         // traversal work-queue entries
     uint64_t last_cells = 0;        // accepted cells of the last nb_compute_forces
     std::string err;
 };

@@ -336,7 +336,8 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X
 
 // ---------------------------------------------------------------------------
 // Far-field network in the step: one wavefront per group of 64 bodies. For
-// each finished subgroup the accepted cells (tree slots written by the walk)
+// each finished subgroup the accepted cells (tree slots written by the walk,
+// possibly in another group's region when another wavefront walked it)
 // go through the network 64 at a time at the subgroup's box centre (lane l
 // prepares cell l; tile q of the MFMA pass is cells 16q..16q+15). Each lane
 // also adds its cell's softened-quadrupole Jacobian, so a subgroup ends with
@@ -375,18 +376,18 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     MlpWeights w;
     mlp_load_weights(w, tiles, l);
     const int nsub = (int)min(cell_cnt[g], 64u);
-    const uint32_t* cg = cell_slot + (uint64_t)g * cell_cap;
+    const uint64_t slots = (uint64_t)groups * cell_cap;
     const int col = l & 15;
     for (int ks = 0; ks < nsub; ks++) {
         const SubRec sub = subs[(uint64_t)g * 64 + ks];
-        const uint32_t end = min(sub.start + sub.count, cell_cap);
+        const uint64_t end = min((uint64_t)sub.start + sub.count, slots);
         float cx = 0.f, cy = 0.f, k00 = 0.f, k01 = 0.f, k10 = 0.f, k11 = 0.f;
-        for (uint32_t b0 = sub.start; b0 < end; b0 += 64) {
-            const uint32_t cell = b0 + l;
+        for (uint64_t b0 = sub.start; b0 < end; b0 += 64) {
+            const uint64_t cell = b0 + l;
             const bool ok = cell < end;
             float dx = 1.f, dy = 0.f, s = 1.f, M = 1.f, ixx = 0.f, ixy = 0.f, iyy = 0.f;
             if (ok) {
-                const NodeRec* r = nodes + cg[cell];
+                const NodeRec* r = nodes + cell_slot[cell];
                 dx = (float)(r->cx - sub.ox);
                 dy = (float)(r->cy - sub.oy);
                 s = (float)sqrt(r->wsq);

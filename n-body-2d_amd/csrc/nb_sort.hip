@@ -98,6 +98,8 @@ __global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ pa
         const double w = xmax - xmin;
         box[4] = w * w;  // root width_sq (BarnesHutCuda.cu:717)
         meta[META_MULTI_LEAVES] = 0;  // the tree build of this step queues from 0
+        meta[META_QHEAD] = 0;         // the traversal work queue of this step starts empty
+        meta[META_QTAIL] = 0;
         // flags stay set for the rest of the batch of steps
         if (!isfinite(xmin) || !isfinite(xmax) || !isfinite(ymin) || !isfinite(ymax))
             meta[META_FLAGS] |= (uint32_t)FLAG_NONFINITE;

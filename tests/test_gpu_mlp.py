@@ -222,7 +222,7 @@ def test_far_field_correction_matches_oracle():
     for g in range(len(nsub)):
         for k in range(int(nsub[g])):
             st, ct = rng_[g, k]
-            nd = nodes[slots[g, st:st + ct]]
+            nd = nodes[slots.ravel()[st:st + ct]]
             assert np.all(nd["c0"] >= 1)
             X, S, U = [], [], []
             K = np.zeros((2, 2))
