@@ -137,6 +137,20 @@ int nb_params_default(nb_params* p);
 int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* pos_xy,
               const double* vel_xy, nb_ctx** ctx_out);
 
+/* Batched ensemble: one context for `nsys` independent systems (no force
+ * between systems), system s being the bodies [offsets[s], offsets[s+1]) of
+ * the arrays (offsets rise strictly from 0 to n = offsets[nsys]; nsys <=
+ * 65,536). Each step runs every system through one set of launches: one
+ * sort (two extra passes on the system number), one tree build (one root per
+ * system) and one traversal launch over all traversal groups; each system's
+ * result equals a context of that system alone, bit for bit. No counterpart
+ * in the reference, which simulates one universe (main.cpp:26-38). Needs
+ * algorithm = BARNES_HUT and traversal = GROUP. nb_get_state and
+ * nb_compute_forces use the input order of all n bodies. */
+int nb_create_batch(const nb_params* p, uint32_t nsys, const uint64_t* offsets,
+                    const double* mass, const double* pos_xy, const double* vel_xy,
+                    nb_ctx** ctx_out);
+
 /* Replaces: k iterations of BarnesHutCuda::simulate's loop body
  * (BarnesHutCuda.cpp:30-40) / AllPairsSim::simulate (AllPairs.cpp:23-29) /
  * BarnesHut::simulate (BarnesHut.cpp:48-69). Returns after the GPU finished. */

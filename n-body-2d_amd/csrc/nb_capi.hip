@@ -60,14 +60,14 @@ void dfree(T*& p) {
 bool is_bh(const nb_ctx* c) { return c->p.algorithm == NB_ALGO_BARNES_HUT; }
 bool is_dk(const nb_ctx* c) { return c->p.integrator == NB_INTEG_DK; }
 
-uint64_t tree_bytes(uint64_t icap) {
+uint64_t tree_bytes(uint64_t icap, uint64_t nsys) {
     const uint64_t ng = (icap + 255) / 256;
-    return (1 + 4 * icap) * sizeof(NodeRec) + icap * (3 * sizeof(uint32_t) + 2) +
+    return (nsys + 4 * icap) * sizeof(NodeRec) + icap * (3 * sizeof(uint32_t) + 2) +
            kMaxLevel * ng * sizeof(uint32_t);
 }
 
 void free_tree(nb_ctx* c) {
-    if (c->icap) c->device_bytes -= tree_bytes(c->icap);
+    if (c->icap) c->device_bytes -= tree_bytes(c->icap, c->nsys);
     dfree(c->b.nodes);
     dfree(c->b.inode_slot);
     dfree(c->b.inode_level);
@@ -81,14 +81,14 @@ void free_tree(nb_ctx* c) {
 int alloc_tree(nb_ctx* c, uint64_t icap) {
     free_tree(c);
     const uint64_t before = c->device_bytes;
-    NB_HIP(c, dalloc(c, &c->b.nodes, 1 + 4 * icap));
+    NB_HIP(c, dalloc(c, &c->b.nodes, c->nsys + 4 * icap));
     NB_HIP(c, dalloc(c, &c->b.inode_slot, icap));
     NB_HIP(c, dalloc(c, &c->b.inode_level, icap));
     NB_HIP(c, dalloc(c, &c->b.inode_mask, icap));
     NB_HIP(c, dalloc(c, &c->b.node_a, icap));
     NB_HIP(c, dalloc(c, &c->b.by_level, icap));
     NB_HIP(c, dalloc(c, &c->b.lvl_cnt, kMaxLevel * ((icap + 255) / 256)));
-    c->device_bytes = before + tree_bytes(icap);
+    c->device_bytes = before + tree_bytes(icap, c->nsys);
     c->icap = icap;
     return NB_OK;
 }
@@ -122,6 +122,10 @@ void free_all(nb_ctx* c) {
     dfree(b.scan_tmp);
     dfree(b.multi_list);
     dfree(b.queue);
+    dfree(b.sys_of);
+    dfree(b.sys_off);
+    dfree(b.grp_first);
+    dfree(b.grp_info);
     dfree(b.acc);
     dfree(b.inter);
     dfree(b.ap_part);
@@ -347,8 +351,15 @@ int nb_params_default(nb_params* p) {
     return NB_OK;
 }
 
-int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* pos_xy,
-              const double* vel_xy, nb_ctx** ctx_out) {
+}  // extern "C"
+
+namespace {
+
+// One context for `nsys` independent systems whose bodies are the contiguous
+// ranges [offsets[s], offsets[s+1]) (nsys = 1: the whole array).
+int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* offsets,
+                const double* mass, const double* pos_xy, const double* vel_xy,
+                nb_ctx** ctx_out) {
     if (!ctx_out) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx_out is NULL");
     *ctx_out = nullptr;
     std::string why;
@@ -357,6 +368,22 @@ int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* 
     if (n > 0x7fffffffull) return fail(nullptr, NB_ERR_INVALID_ARG, "n exceeds 2^31-1 bodies");
     if (n && (!mass || !pos_xy || !vel_xy))
         return fail(nullptr, NB_ERR_INVALID_ARG, "NULL body array");
+    if (nsys > 1) {
+        if (p->algorithm != NB_ALGO_BARNES_HUT || p->traversal != NB_TRAV_GROUP)
+            return fail(nullptr, NB_ERR_NOT_SUPPORTED,
+                        "several systems need algorithm=BARNES_HUT and traversal=GROUP");
+        if (nsys > 65536) return fail(nullptr, NB_ERR_INVALID_ARG, "at most 65536 systems");
+    }
+    std::vector<uint32_t> off(nsys + 1);
+    for (uint32_t s = 0; s <= nsys; s++) {
+        const uint64_t o = offsets ? offsets[s] : (s ? n : 0);
+        if ((s == 0 && o != 0) || (s == nsys && o != n) || (s && n && o <= off[s - 1]) ||
+            o > n)
+            return fail(nullptr, NB_ERR_INVALID_ARG,
+                        "offsets must rise strictly from 0 to n (no empty system)");
+        off[s] = (uint32_t)o;
+    }
+    if (n == 0 && nsys > 1) return fail(nullptr, NB_ERR_INVALID_ARG, "no bodies");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(nullptr, NB_ERR_NO_DEVICE, "no HIP device");
@@ -364,6 +391,7 @@ int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* 
     if (!c) return fail(nullptr, NB_ERR_OUT_OF_MEMORY, "host allocation failed");
     c->p = *p;
     c->n = n;
+    c->nsys = nsys;
     c->dt = p->dt;
     auto bail = [&](int code) {
         g_err = c->err;
@@ -402,7 +430,30 @@ int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* 
                                                     (uint64_t)kMaxLevel * (30 * n / 256 + 2) / 1024) + 2;
     NB_TRY(dalloc(c, &b.sort_hist, bh ? (uint64_t)kSortBins * c->sort_tiles : 0));
     NB_TRY(dalloc(c, &b.sort_tot, kSortBins));
-    NB_TRY(dalloc(c, &b.box, 8));
+    NB_TRY(dalloc(c, &b.box, (uint64_t)kBoxStride * nsys));
+    // systems and traversal groups (<= 64 bodies, never across two systems)
+    std::vector<uint32_t> sys_of(n), grp_first, grp_info;
+    for (uint32_t s = 0; s < nsys; s++) {
+        for (uint32_t i = off[s]; i < off[s + 1]; i++) sys_of[i] = s;
+        for (uint32_t g = off[s]; g < off[s + 1]; g += 64) {
+            grp_first.push_back(g);
+            grp_info.push_back((s << 8) | std::min<uint32_t>(64, off[s + 1] - g));
+        }
+    }
+    c->ngroups = (uint32_t)grp_first.size();
+    NB_TRY(dalloc(c, &b.sys_of, n));
+    NB_TRY(dalloc(c, &b.sys_off, nsys + 1));
+    NB_TRY(dalloc(c, &b.grp_first, c->ngroups));
+    NB_TRY(dalloc(c, &b.grp_info, c->ngroups));
+    if (n) {
+        NB_TRY(hipMemcpy(b.sys_of, sys_of.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+        NB_TRY(hipMemcpy(b.grp_first, grp_first.data(), c->ngroups * sizeof(uint32_t),
+                         hipMemcpyHostToDevice));
+        NB_TRY(hipMemcpy(b.grp_info, grp_info.data(), c->ngroups * sizeof(uint32_t),
+                         hipMemcpyHostToDevice));
+    }
+    NB_TRY(hipMemcpy(b.sys_off, off.data(), (nsys + 1) * sizeof(uint32_t),
+                     hipMemcpyHostToDevice));
     NB_TRY(dalloc(c, &b.bbox_part, 4ull * c->bbox_blocks));
     NB_TRY(dalloc(c, &b.delta, bh ? n : 0));
     NB_TRY(dalloc(c, &b.cnt, bh ? n : 0));
@@ -410,7 +461,7 @@ int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* 
     NB_TRY(dalloc(c, &b.multi_list, bh ? n / 2 + 1 : 0));
     if (bh) {
         // work queue of the group traversal, zero between kernels
-        c->queue_cap = (uint32_t)std::min<uint64_t>(4 * ((n + 63) / 64) + 64, 0xffffffffull);
+        c->queue_cap = (uint32_t)std::min<uint64_t>(4ull * c->ngroups + 64, 0xffffffffull);
         NB_TRY(dalloc(c, &b.queue, c->queue_cap));
         NB_TRY(hipMemsetAsync(b.queue, 0, c->queue_cap * sizeof(uint64_t), c->stream));
     }
@@ -465,6 +516,22 @@ int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* 
     if (rc) return bail(rc);
     *ctx_out = c;
     return NB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nb_create(const nb_params* p, uint64_t n, const double* mass, const double* pos_xy,
+              const double* vel_xy, nb_ctx** ctx_out) {
+    return create_impl(p, n, 1, nullptr, mass, pos_xy, vel_xy, ctx_out);
+}
+
+int nb_create_batch(const nb_params* p, uint32_t nsys, const uint64_t* offsets,
+                    const double* mass, const double* pos_xy, const double* vel_xy,
+                    nb_ctx** ctx_out) {
+    if (nsys == 0 || !offsets) return fail(nullptr, NB_ERR_INVALID_ARG, "no systems");
+    return create_impl(p, offsets[nsys], nsys, offsets, mass, pos_xy, vel_xy, ctx_out);
 }
 
 int nb_step(nb_ctx* c, uint64_t k) {
@@ -607,7 +674,7 @@ int nb_get_info(nb_ctx* c, nb_info* info) {
     info->iteration = c->iteration;
     info->simulated_time = c->sim_time;
     info->internal_nodes = c->last_internal;
-    info->tree_nodes = c->n ? 1 + 4 * c->last_internal : 0;
+    info->tree_nodes = c->n ? c->nsys + 4 * c->last_internal : 0;
     info->node_capacity = c->icap;
     info->device_bytes = c->device_bytes;
     info->far_field_cells = c->last_cells;
@@ -670,14 +737,14 @@ int nb_debug_build(nb_ctx* c, double* box4, uint64_t* sorted_keys, uint32_t* ori
     if (orig_index)
         NB_HIP(c, hipMemcpy(orig_index, c->b.orig[c->cur], n * sizeof(uint32_t),
                             hipMemcpyDeviceToHost));
-    if (node_count) *node_count = n > 1 ? 1 + 4 * c->last_internal : 1;
+    if (node_count) *node_count = c->nsys + 4 * c->last_internal;
     return NB_OK;
 }
 
 int nb_debug_tree(nb_ctx* c, nb_node* nodes, uint64_t cap) {
     if (!c || !nodes) return fail(c, NB_ERR_INVALID_ARG, "NULL argument");
     if (c->n == 0) return NB_OK;
-    const uint64_t count = c->n > 1 ? 1 + 4 * c->last_internal : 1;
+    const uint64_t count = c->nsys + 4 * c->last_internal;
     if (cap < count) return fail(c, NB_ERR_INVALID_ARG, "cap smaller than the node count");
     std::vector<NodeRec> h(count);
     NB_HIP(c, hipMemcpy(h.data(), c->b.nodes, count * sizeof(NodeRec), hipMemcpyDeviceToHost));

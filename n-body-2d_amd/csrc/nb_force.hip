@@ -258,7 +258,8 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     uint32_t* __restrict__ meta, int count_step, int budget,
     const uint64_t* __restrict__ keys, uint32_t* __restrict__ cell_slot,
     uint32_t* __restrict__ cell_cnt, uint32_t cell_cap, SubRec* __restrict__ subs,
-    float4* __restrict__ body_w, uint64_t* __restrict__ queue, uint32_t queue_cap) {
+    float4* __restrict__ body_w, uint64_t* __restrict__ queue, uint32_t queue_cap,
+    const uint32_t* __restrict__ grp_first, const uint32_t* __restrict__ grp_info) {
     if (meta[META_FLAGS] & kHaltFlags) return;
     __shared__ __attribute__((aligned(16))) int32_t s_stack[kGStack];
     __shared__ float4 s_list[kGList];
@@ -279,14 +280,18 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     int nt = 1;
     if (l == 0) s_todo[0] = 64;
     for (;;) {
-        const uint32_t i = g * 64 + l;
-        const bool valid = i < n;
-        const double2 p = valid ? pos[i] : pos[g * 64];
+        // group g: bodies first .. first + gn - 1 of system gsys
+        const uint32_t first = grp_first[g], ginfo = grp_info[g];
+        const int gn = (int)(ginfo & 255u);
+        const int32_t root = (int32_t)(ginfo >> 8);
+        const uint32_t i = first + l;
+        const bool valid = l < gn;
+        const double2 p = valid ? pos[i] : pos[first];
         const int32_t me = (int32_t)i;
         // Morton jump to the next body: digits shared with it (64 = none follows).
         const uint64_t key = valid ? keys[i] : 0ull;
         const uint64_t key_next = __shfl_down(key, 1, 64);
-        const int jump = (valid && l < 63 && i + 1 < n) ? __clzll(key ^ key_next) : 64;
+        const int jump = (valid && l + 1 < gn) ? __clzll(key ^ key_next) : 64;
         while (nt > 0 && !overflow) {
             const int32_t e = readfirst(s_todo[--nt]);
             const int f = e >> 8, cn = e & 255;
@@ -344,15 +349,15 @@ __global__ __launch_bounds__(64) void k_traverse_group(
             const double ox = 0.5 * (xmin + xmax), oy = 0.5 * (ymin + ymax);
             const double hx = 0.5 * (xmax - xmin), hy = 0.5 * (ymax - ymin);
             const float bx = (float)(p.x - ox), by = (float)(p.y - oy);
-            const int32_t g_first = (int32_t)(g * 64 + f);
-            const int32_t g_last = (int32_t)min(g * 64 + f + cn - 1, n - 1);
+            const int32_t g_first = (int32_t)(first + f);
+            const int32_t g_last = (int32_t)(first + min(f + cn, gn) - 1);
             const bool capped = cn > 1;
             Acc t;
             int sp = 1, cnt = 0;
             uint32_t ccnt = 0;  // cells of this subgroup written after cbase
             uint32_t listed = 0;
             bool own_multi = false, split = false;
-            if (l == 0) s_stack[0] = 0;
+            if (l == 0) s_stack[0] = root;  // the root of this group's system
             while (sp > 0) {
                 int take = min(64, sp);
                 take = min(take, (kGStack - sp) / 3);
@@ -518,7 +523,7 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
     const uint32_t n = (uint32_t)c->n;
     if (n == 0) return;
     if (c->p.traversal == NB_TRAV_GROUP) {
-        const uint32_t groups = (n + 63) / 64;
+        const uint32_t groups = c->ngroups;
         const bool z = !(c->p.softening > 0.0);
         const bool gd = c->p.self_guard != 0;
         const int budget = c->p.group_budget > 0 ? c->p.group_budget : kGBudget;
@@ -528,7 +533,8 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
                        c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src], c->b.acc,     \
                        c->b.inter, n, c->b.consts, c->b.meta, (int)(count_step && !learn),     \
                        budget, c->b.keys[0], c->b.cell_slot, c->b.cell_cnt, c->cell_cap,       \
-                       c->b.subs, c->b.body_w, c->b.queue, c->queue_cap)
+                       c->b.subs, c->b.body_w, c->b.queue, c->queue_cap, c->b.grp_first,       \
+                       c->b.grp_info)
 #define NB_TRAVG3(K, Z, GD) if (learn) NB_TRAVG(false, Z, GD, true); else NB_TRAVG(K, Z, GD, false)
 #define NB_TRAVG2(K, Z) if (gd) { NB_TRAVG3(K, Z, true); } else { NB_TRAVG3(K, Z, false); }
         if (kick) {

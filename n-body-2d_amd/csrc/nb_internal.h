@@ -6,10 +6,12 @@
 //            steps, as BarnesHutCuda does (BarnesHutCuda.cu:701-708);
 //            orig[i] is the caller's index of sorted body i.
 //   sort     keys u64[N] x2, vals u32[N] x2, per-tile digit histograms.
-//   tree     NodeRec[1 + 4*I] slots (I = internal nodes). Internal node p
-//            (numbered in pre-order) owns the four child slots 1+4p .. 4+4p,
-//            the layout of the reference's contiguous children
-//            (BarnesHutCuda.cu:212-221) with a deterministic numbering.
+//   tree     NodeRec[S + 4*I] slots (S = systems, I = internal nodes): slot s
+//            is the root of system s, and internal node p (numbered in
+//            pre-order) owns the four child slots S+4p .. S+4p+3, the layout
+//            of the reference's contiguous children (BarnesHutCuda.cu:212-221)
+//            with a deterministic numbering. One system: root 0, children
+//            1+4p .. 4+4p.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,6 +32,7 @@ constexpr int kSortBins = 1 << kSortBits;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;         // keys per thread per tile
 constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kBoxStride = 8;          // doubles of box data per system
 
 // Meta words kept in device memory (u32 each). Words that every wavefront
 // of a kernel updates or polls sit on 128-byte L2 lines of their own: atomics
@@ -94,7 +97,12 @@ struct Buffers {
     uint32_t* vals[2];
     uint32_t* sort_hist;   // [bins][tiles]
     uint32_t* sort_tot;    // [bins]
-    double* box;           // x_min, x_max, y_min, y_max, root_wsq
+    double* box;           // per system [8]: x_min, x_max, y_min, y_max, root_wsq
+    uint32_t* sys_of;      // [N] system of the body at position i (positions of a
+                           //     system never change: sorting permutes inside it)
+    uint32_t* sys_off;     // [nsys + 1] first position of each system
+    uint32_t* grp_first;   // [ngroups] first body of traversal group g
+    uint32_t* grp_info;    // [ngroups] (system << 8) | bodies (1..64)
     double* bbox_part;     // [blocks][4]
     int8_t* delta;         // [N] common-prefix digit count of pair (i, i+1)
     uint32_t* cnt;         // [N] internal nodes owned by pair i, then its offset
@@ -138,6 +146,8 @@ struct Buffers {
 struct nb_ctx {
     nb_params p;
     uint64_t n = 0;
+    uint32_t nsys = 1;           // independent systems (contiguous body ranges)
+    uint32_t ngroups = 0;        // traversal groups: <= 64 bodies, inside one system
     int device = 0;
     hipStream_t stream = nullptr;
     nb::Buffers b{};

@@ -355,7 +355,9 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     double2* __restrict__ acc, double2* __restrict__ vel, uint32_t n,
     const StepConsts* __restrict__ k, uint32_t* __restrict__ meta,
     const NodeRec* __restrict__ nodes, const double* __restrict__ box,
-    const void* __restrict__ tiles, const float* __restrict__ bias, int count_step) {
+    const void* __restrict__ tiles, const float* __restrict__ bias, int count_step,
+    uint32_t groups, const uint32_t* __restrict__ grp_first,
+    const uint32_t* __restrict__ grp_info) {
     if (meta[META_FLAGS] & kHaltFlags) return;
     __shared__ __attribute__((aligned(16))) float sb[kMlpBiasLen];
     __shared__ float su[kApplyWaves][6 * 64];        // [C_x, C_y, K00, K01, K10, K11][subgroup]
@@ -365,14 +367,15 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     if (count_step && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
     __syncthreads();
     const int l = lane_id(), wv = threadIdx.x >> 6;
-    const uint32_t groups = (n + 63) / 64;
     const uint32_t g = blockIdx.x * kApplyWaves + wv;
     if (g >= groups) return;
+    const uint32_t first = grp_first[g], ginfo = grp_info[g];
+    const uint32_t gn = ginfo & 255u, sys = ginfo >> 8;
     float* u = su[wv];
     float* f_all = sf[wv];
     const float eps2f = (float)k->eps2;
-    const float inv_mroot = (float)(1.0 / nodes[0].m);
-    const float inv_ell = mlp_inv_ell(eps2f, box[4]);
+    const float inv_mroot = (float)(1.0 / nodes[sys].m);
+    const float inv_ell = mlp_inv_ell(eps2f, box[(uint64_t)kBoxStride * sys + 4]);
     MlpWeights w;
     mlp_load_weights(w, tiles, l);
     const int nsub = (int)min(cell_cnt[g], 64u);
@@ -454,8 +457,8 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
         }
         __builtin_amdgcn_wave_barrier();
     }
-    const uint32_t i = g * 64 + l;
-    if (i >= n) return;
+    if ((uint32_t)l >= gn) return;
+    const uint32_t i = first + l;
     const float4 bw = body_w[i];
     const int ks = __float_as_int(bw.z) & 63;
     // correction at offset b: C - K b
@@ -476,14 +479,15 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
 
 void launch_mlp_apply(nb_ctx* c, int src, bool kick, bool count_step) {
     const uint32_t n = (uint32_t)c->n;
-    const uint32_t groups = (n + 63) / 64;
+    const uint32_t groups = c->ngroups;
     const uint32_t blocks = (groups + kApplyWaves - 1) / kApplyWaves;
     const int count = count_step ? 1 : 0;
 #define NB_APPLY(K)                                                                            \
     hipLaunchKernelGGL((k_mlp_apply<K>), dim3(blocks), dim3(64 * kApplyWaves), 0, c->stream,   \
                        c->b.cell_slot, c->b.cell_cnt, c->cell_cap, c->b.subs, c->b.body_w,     \
                        c->b.acc, c->b.vel[src], n, c->b.consts, c->b.meta, c->b.nodes,         \
-                       c->b.box, c->b.mlp_tiles, c->b.mlp_bias, count)
+                       c->b.box, c->b.mlp_tiles, c->b.mlp_bias, count, groups, c->b.grp_first, \
+                       c->b.grp_info)
     if (kick) NB_APPLY(true); else NB_APPLY(false);
 #undef NB_APPLY
 }
@@ -632,7 +636,7 @@ int prepare_mlp(nb_ctx* c) { return ensure_mlp(c); }
 
 int alloc_cells(nb_ctx* c, uint32_t cap) {
     Buffers& b = c->b;
-    const uint64_t groups = (c->n + 63) / 64;
+    const uint64_t groups = c->ngroups;
     if (b.cell_slot) (void)hipFree(b.cell_slot);
     b.cell_slot = nullptr;
     if (c->cell_cap) c->device_bytes -= groups * c->cell_cap * sizeof(uint32_t);
@@ -695,6 +699,8 @@ int nb_mlp_generate(nb_ctx* c, double theta, uint32_t sample_bodies, uint64_t ma
     if (c->p.algorithm != NB_ALGO_BARNES_HUT)
         return mlp_fail(c, NB_ERR_NOT_SUPPORTED, "pairs need a Barnes-Hut context");
     if (c->n < 2) return mlp_fail(c, NB_ERR_INVALID_ARG, "need at least two bodies");
+    if (c->nsys != 1)
+        return mlp_fail(c, NB_ERR_NOT_SUPPORTED, "pairs are generated from one system");
     if (max_pairs == 0 || max_pairs > 0xffffffffull)
         return mlp_fail(c, NB_ERR_INVALID_ARG, "max_pairs must be in [1, 2^32)");
     // The scan of per-sample counts uses the context's scan scratch, sized for n.
@@ -843,7 +849,7 @@ int nb_debug_far_field(nb_ctx* c, uint32_t* nsub, double* sub_centre, uint32_t* 
     if (!c) return NB_ERR_INVALID_ARG;
     if (!c->b.cell_slot)
         return mlp_fail(c, NB_ERR_NOT_SUPPORTED, "no learned far field in this context");
-    const uint64_t n = c->n, groups = (n + 63) / 64;
+    const uint64_t n = c->n, groups = c->ngroups;
     if (cell_cap_out) *cell_cap_out = c->cell_cap;
     if (nsub) MLP_HIP(c, scopy(c, nsub, c->b.cell_cnt, groups * 4, hipMemcpyDeviceToHost));
     if (sub_centre || sub_range) {

@@ -106,7 +106,77 @@ __global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ pa
     }
 }
 
+// Several systems: one workgroup per system reduces and squares its box.
+__global__ __launch_bounds__(kBboxThreads) void k_bbox_sys(const double2* __restrict__ pos,
+                                                           const uint32_t* __restrict__ sys_off,
+                                                           double* __restrict__ box,
+                                                           uint32_t* __restrict__ meta) {
+    const uint32_t s = blockIdx.x;
+    const uint32_t a = sys_off[s], e = sys_off[s + 1];
+    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    for (uint32_t i = a + threadIdx.x; i < e; i += kBboxThreads) {
+        const double2 p = pos[i];
+        xmin = fmin(xmin, p.x);
+        xmax = fmax(xmax, p.x);
+        ymin = fmin(ymin, p.y);
+        ymax = fmax(ymax, p.y);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        xmin = fmin(xmin, __shfl_xor(xmin, d, 64));
+        xmax = fmax(xmax, __shfl_xor(xmax, d, 64));
+        ymin = fmin(ymin, __shfl_xor(ymin, d, 64));
+        ymax = fmax(ymax, __shfl_xor(ymax, d, 64));
+    }
+    __shared__ double sm[4][kBboxThreads / 64];
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        sm[0][w] = xmin;
+        sm[1][w] = xmax;
+        sm[2][w] = ymin;
+        sm[3][w] = ymax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBboxThreads / 64; k++) {
+            xmin = fmin(xmin, sm[0][k]);
+            xmax = fmax(xmax, sm[1][k]);
+            ymin = fmin(ymin, sm[2][k]);
+            ymax = fmax(ymax, sm[3][k]);
+        }
+        const double xr = xmax - xmin, yr = ymax - ymin;
+        if (xr > yr) {
+            const double mid = (ymin + ymax) * 0.5;
+            ymin = mid - xr * 0.5;
+            ymax = mid + xr * 0.5;
+        } else {
+            const double mid = (xmin + xmax) * 0.5;
+            xmin = mid - yr * 0.5;
+            xmax = mid + yr * 0.5;
+        }
+        double* bx = box + (uint64_t)kBoxStride * s;
+        bx[0] = xmin;
+        bx[1] = xmax;
+        bx[2] = ymin;
+        bx[3] = ymax;
+        const double wd = xmax - xmin;
+        bx[4] = wd * wd;
+        if (s == 0) {
+            meta[META_MULTI_LEAVES] = 0;
+            meta[META_QHEAD] = 0;
+            meta[META_QTAIL] = 0;
+        }
+        if (!isfinite(xmin) || !isfinite(xmax) || !isfinite(ymin) || !isfinite(ymax))
+            atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_NONFINITE);
+    }
+}
+
 void launch_bbox(nb_ctx* c, int src) {
+    if (c->nsys > 1) {
+        hipLaunchKernelGGL(k_bbox_sys, dim3(c->nsys), dim3(kBboxThreads), 0, c->stream,
+                           c->b.pos[src], c->b.sys_off, c->b.box, c->b.meta);
+        return;
+    }
     hipLaunchKernelGGL(k_bbox_partial, dim3(c->bbox_blocks), dim3(kBboxThreads), 0, c->stream,
                        c->b.pos[src], (uint32_t)c->n, c->b.bbox_part);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, c->stream, c->b.bbox_part,
@@ -135,11 +205,13 @@ __device__ __forceinline__ uint32_t quantise(double v, double lo, double inv) {
 
 __global__ __launch_bounds__(256) void k_morton(const double2* __restrict__ pos, uint32_t n,
                                                 const double* __restrict__ box,
+                                                const uint32_t* __restrict__ sys_of,
                                                 uint64_t* __restrict__ keys,
                                                 uint32_t* __restrict__ vals) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const double x_min = box[0], x_max = box[1], y_min = box[2], y_max = box[3];
+    const double* bx = box + (uint64_t)kBoxStride * sys_of[i];
+    const double x_min = bx[0], x_max = bx[1], y_min = bx[2], y_max = bx[3];
     const double inv_w = 1.0 / (x_max - x_min);
     const double inv_h = 1.0 / (y_max - y_min);
     const double2 p = pos[i];
@@ -152,7 +224,7 @@ __global__ __launch_bounds__(256) void k_morton(const double2* __restrict__ pos,
 void launch_morton(nb_ctx* c, int src) {
     const uint32_t n = (uint32_t)c->n;
     hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->b.pos[src], n,
-                       c->b.box, c->b.keys[0], c->b.vals[0]);
+                       c->b.box, c->b.sys_of, c->b.keys[0], c->b.vals[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -177,7 +249,19 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
     return peers;
 }
 
+// SYS: the digit is taken from the system of the key's source position
+// (sys_of[val]) instead of the key, for the passes that keep systems apart.
+template <bool SYS>
+__device__ __forceinline__ uint32_t sort_digit(uint64_t key, uint32_t val, int shift,
+                                               const uint32_t* __restrict__ sys_of) {
+    if (SYS) return (sys_of[val] >> shift) & (kSortBins - 1);
+    return (uint32_t)(key >> shift) & (kSortBins - 1);
+}
+
+template <bool SYS>
 __global__ __launch_bounds__(kSortThreads) void k_rs_hist(const uint64_t* __restrict__ keys,
+                                                          const uint32_t* __restrict__ vals,
+                                                          const uint32_t* __restrict__ sys_of,
                                                           uint32_t n, int shift,
                                                           uint32_t* __restrict__ hist,
                                                           uint32_t tiles) {
@@ -189,7 +273,9 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_hist(const uint64_t* __rest
     for (int r = 0; r < kSortItems; r++) {
         const uint64_t idx = base + (uint64_t)r * kSortThreads + threadIdx.x;
         const bool valid = idx < n;
-        const uint32_t d = valid ? (uint32_t)(keys[idx] >> shift) & (kSortBins - 1) : 0u;
+        const uint32_t d =
+            valid ? sort_digit<SYS>(SYS ? 0ull : keys[idx], SYS ? vals[idx] : 0u, shift, sys_of)
+                  : 0u;
         const uint64_t peers = digit_peers(d, valid);
         if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
     }
@@ -214,10 +300,11 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_scan(uint32_t* __restrict__
     if (threadIdx.x == 0) tot[blockIdx.x] = carry;
 }
 
+template <bool SYS>
 __global__ __launch_bounds__(kSortThreads) void k_rs_scatter(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, uint32_t n, int shift, const uint32_t* __restrict__ hist,
-    const uint32_t* __restrict__ tot, uint32_t tiles) {
+    uint32_t* __restrict__ vout, const uint32_t* __restrict__ sys_of, uint32_t n, int shift,
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ tot, uint32_t tiles) {
     constexpr int NW = kSortThreads / 64;
     __shared__ uint32_t run[kSortBins];      // next output position of each digit
     __shared__ uint32_t wcnt[NW][kSortBins]; // keys of each digit per wave, this round
@@ -244,7 +331,7 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_scatter(
     for (int r = 0; r < kSortItems; r++) {
         const uint64_t idx = base + (uint64_t)r * kSortThreads + t;
         const bool valid = idx < n;
-        const uint32_t d = (uint32_t)(key[r] >> shift) & (kSortBins - 1);
+        const uint32_t d = valid ? sort_digit<SYS>(key[r], val[r], shift, sys_of) : 0u;
         const uint64_t peers = digit_peers(d, valid);
         const uint64_t below = peers & lanemask_lt();
         if (valid && below == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
@@ -267,19 +354,36 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_scatter(
     }
 }
 
+// Several systems: two more stable passes on the 16-bit system number (of
+// the key's source position) put the systems back in place, each sorted by
+// its own keys. An even pass count leaves the result in keys[0] / vals[0].
 void launch_radix_sort(nb_ctx* c) {
     const uint32_t n = (uint32_t)c->n;
     const uint32_t tiles = c->sort_tiles;
-    for (int pass = 0; pass < 64 / kSortBits; pass++) {
-        const int shift = pass * kSortBits;
+    const int key_passes = 64 / kSortBits, sys_passes = c->nsys > 1 ? 2 : 0;
+    for (int pass = 0; pass < key_passes + sys_passes; pass++) {
+        const bool sys = pass >= key_passes;
+        const int shift = (sys ? pass - key_passes : pass) * kSortBits;
         const int in = pass & 1, out = in ^ 1;
-        hipLaunchKernelGGL(k_rs_hist, dim3(tiles), dim3(kSortThreads), 0, c->stream,
-                           c->b.keys[in], n, shift, c->b.sort_hist, tiles);
+        if (sys)
+            hipLaunchKernelGGL(k_rs_hist<true>, dim3(tiles), dim3(kSortThreads), 0, c->stream,
+                               c->b.keys[in], c->b.vals[in], c->b.sys_of, n, shift,
+                               c->b.sort_hist, tiles);
+        else
+            hipLaunchKernelGGL(k_rs_hist<false>, dim3(tiles), dim3(kSortThreads), 0, c->stream,
+                               c->b.keys[in], c->b.vals[in], c->b.sys_of, n, shift,
+                               c->b.sort_hist, tiles);
         hipLaunchKernelGGL(k_rs_scan, dim3(kSortBins), dim3(kSortThreads), 0, c->stream,
                            c->b.sort_hist, tiles, c->b.sort_tot);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(tiles), dim3(kSortThreads), 0, c->stream,
-                           c->b.keys[in], c->b.vals[in], c->b.keys[out], c->b.vals[out], n, shift,
-                           c->b.sort_hist, c->b.sort_tot, tiles);
+        if (sys)
+            hipLaunchKernelGGL(k_rs_scatter<true>, dim3(tiles), dim3(kSortThreads), 0, c->stream,
+                               c->b.keys[in], c->b.vals[in], c->b.keys[out], c->b.vals[out],
+                               c->b.sys_of, n, shift, c->b.sort_hist, c->b.sort_tot, tiles);
+        else
+            hipLaunchKernelGGL(k_rs_scatter<false>, dim3(tiles), dim3(kSortThreads), 0,
+                               c->stream, c->b.keys[in], c->b.vals[in], c->b.keys[out],
+                               c->b.vals[out], c->b.sys_of, n, shift, c->b.sort_hist,
+                               c->b.sort_tot, tiles);
     }
 }
 

@@ -12,7 +12,10 @@
 //   (those are exactly the internal nodes whose first body is i)
 // so an exclusive scan of the per-pair counts numbers every internal node in
 // pre-order (sorted by first body, then level), and internal node p owns child
-// slots 1+4p..4+4p. No atomics, no host synchronisation, deterministic ids.
+// slots S+4p..S+4p+3 (S = number of systems; slot s is the root of system s).
+// No atomics, no host synchronisation, deterministic ids. With several
+// systems, delta = -1 between the last body of one system and the first of
+// the next, so every system gets its own root and its own subtree.
 #include "nb_device.h"
 #include "nb_internal.h"
 
@@ -28,13 +31,15 @@ __device__ __forceinline__ int pair_delta(uint64_t a, uint64_t b) {
 // Per pair: delta and the number of internal nodes it owns.
 // HBM traffic: 8 B keys read (x3 from cache), 5 B written per pair.
 __global__ __launch_bounds__(256) void k_tree_delta(const uint64_t* __restrict__ keys, uint32_t n,
+                                                    const uint32_t* __restrict__ sys_of,
                                                     int8_t* __restrict__ delta,
                                                     uint32_t* __restrict__ cnt) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i + 1 >= n) return;
     const uint64_t k0 = keys[i], k1 = keys[i + 1];
-    const int d = pair_delta(k0, k1);
-    const int prev = i ? pair_delta(keys[i - 1], k0) : -1;
+    const uint32_t s0 = sys_of[i];
+    const int d = s0 == sys_of[i + 1] ? pair_delta(k0, k1) : -1;
+    const int prev = (i && sys_of[i - 1] == s0) ? pair_delta(keys[i - 1], k0) : -1;
     const int hi = d < kMaxLevel - 1 ? d : kMaxLevel - 1;
     delta[i] = (int8_t)d;
     cnt[i] = hi > prev ? (uint32_t)(hi - prev) : 0u;
@@ -59,16 +64,21 @@ __device__ __forceinline__ void write_empty(NodeRec* r, double wsq) {
 // child ranges (binary searches on the digit, lower_bound_quadrant of
 // BarnesHutCuda.cu:153-163) and the child slots. Leaves get their COM here;
 // multi-body leaves (only possible at level 30) are queued for k_leaf_multi.
-__device__ void build_node(uint32_t p, const uint64_t* __restrict__ keys, uint32_t n,
+__device__ void build_node(uint32_t p, const uint64_t* __restrict__ keys, uint32_t nsys,
+                           const uint32_t* __restrict__ sys_of,
+                           const uint32_t* __restrict__ sys_off,
                            const double2* __restrict__ pos, const double* __restrict__ mass,
                            const int8_t* __restrict__ delta, const uint32_t* __restrict__ off,
-                           double root_wsq, const uint32_t* __restrict__ node_a,
+                           const double* __restrict__ box, const uint32_t* __restrict__ node_a,
                            NodeRec* __restrict__ nodes, uint32_t* __restrict__ inode_slot,
                            const uint8_t* __restrict__ inode_level,
                            uint8_t* __restrict__ inode_mask, uint32_t* __restrict__ multi_list,
                            uint32_t* __restrict__ meta) {
     const uint32_t a = node_a[p];
     const int L = inode_level[p];
+    const uint32_t sys = sys_of[a];
+    const uint32_t n = sys_off[sys + 1];  // end of this node's system
+    const double root_wsq = box[(uint64_t)kBoxStride * sys + 4];
     uint32_t b;
     if (L == 0) {
         b = n - 1;
@@ -113,7 +123,7 @@ __device__ void build_node(uint32_t p, const uint64_t* __restrict__ keys, uint32
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const uint32_t cs = bq[q], ce1 = bq[q + 1];
-        NodeRec* r = &nodes[1 + 4ull * p + q];
+        NodeRec* r = &nodes[nsys + 4ull * p + q];
         if (cs >= ce1) {
             write_empty(r, cwsq);
             continue;
@@ -138,26 +148,26 @@ __device__ void build_node(uint32_t p, const uint64_t* __restrict__ keys, uint32
                 r->iyy = 0.f;
             } else {
                 const uint32_t k = atomicAdd(&meta[META_MULTI_LEAVES], 1u);
-                multi_list[k] = (uint32_t)(1 + 4ull * p + q);
+                multi_list[k] = (uint32_t)(nsys + 4ull * p + q);
             }
         } else {
             const int prev = cs ? (int)delta[cs - 1] : -1;
             const uint32_t cid = off[cs] + (uint32_t)(L + 1 - prev - 1);
             r->wsq = cwsq;
-            r->c0 = (int32_t)(1 + 4ull * cid);
+            r->c0 = (int32_t)(nsys + 4ull * cid);
             r->bs = (int32_t)cs;
             r->be = (int32_t)ce;
-            inode_slot[cid] = (uint32_t)(1 + 4ull * p + q);
+            inode_slot[cid] = (uint32_t)(nsys + 4ull * p + q);
         }
     }
     inode_mask[p] = (uint8_t)mask;
-    if (p == 0) {
-        NodeRec* r = &nodes[0];
+    if (L == 0) {  // the root of system `sys` lives in slot `sys`
+        NodeRec* r = &nodes[sys];
         r->wsq = root_wsq;
-        r->c0 = 1;
-        r->bs = 0;
-        r->be = (int32_t)(n - 1);
-        inode_slot[0] = 0;
+        r->c0 = (int32_t)(nsys + 4ull * p);
+        r->bs = (int32_t)a;
+        r->be = (int32_t)b;
+        inode_slot[p] = sys;
     }
 }
 
@@ -165,6 +175,8 @@ __device__ void build_node(uint32_t p, const uint64_t* __restrict__ keys, uint32
 // internal node ids off[i] + (L - delta_{i-1} - 1).
 __global__ __launch_bounds__(256) void k_tree_enum(const int8_t* __restrict__ delta,
                                                    const uint32_t* __restrict__ off, uint32_t n,
+                                                   const uint32_t* __restrict__ sys_of,
+                                                   const uint32_t* __restrict__ sys_off,
                                                    uint32_t* __restrict__ meta, uint64_t icap,
                                                    uint32_t* __restrict__ node_a,
                                                    uint8_t* __restrict__ inode_level,
@@ -178,21 +190,22 @@ __global__ __launch_bounds__(256) void k_tree_enum(const int8_t* __restrict__ de
         if (i == 0 && I > icap) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_TREE_CAPACITY);
         return;
     }
-    if (n == 1) {
-        if (i == 0) {
-            NodeRec* r = &nodes[0];
-            const double2 x = pos[0];
-            r->cx = x.x;
-            r->cy = x.y;
-            r->m = mass[0];
-            r->mf = (float)mass[0];
-            r->wsq = box[4];
-            r->c0 = -1;
-            r->mask = 0;
-            r->bs = 0;
-            r->be = 0;
-            r->ixx = r->ixy = r->iyy = 0.f;
-        }
+    if (i >= n) return;
+    const uint32_t sys = sys_of[i];
+    if (sys_off[sys] == i && sys_off[sys + 1] == i + 1) {
+        // a system of one body: its root is a leaf
+        NodeRec* r = &nodes[sys];
+        const double2 x = pos[i];
+        r->cx = x.x;
+        r->cy = x.y;
+        r->m = mass[i];
+        r->mf = (float)mass[i];
+        r->wsq = box[(uint64_t)kBoxStride * sys + 4];
+        r->c0 = -1;
+        r->mask = 0;
+        r->bs = (int32_t)i;
+        r->be = (int32_t)i;
+        r->ixx = r->ixy = r->iyy = 0.f;
         return;
     }
     if (i + 1 >= n) return;
@@ -208,7 +221,8 @@ __global__ __launch_bounds__(256) void k_tree_enum(const int8_t* __restrict__ de
 }
 
 __global__ __launch_bounds__(256) void k_tree_nodes(
-    const uint64_t* __restrict__ keys, uint32_t n, const double2* __restrict__ pos,
+    const uint64_t* __restrict__ keys, uint32_t nsys, const uint32_t* __restrict__ sys_of,
+    const uint32_t* __restrict__ sys_off, const double2* __restrict__ pos,
     const double* __restrict__ mass, const int8_t* __restrict__ delta,
     const uint32_t* __restrict__ off, const double* __restrict__ box, uint32_t* __restrict__ meta,
     uint64_t icap, const uint32_t* __restrict__ node_a, NodeRec* __restrict__ nodes,
@@ -218,8 +232,8 @@ __global__ __launch_bounds__(256) void k_tree_nodes(
     if (I > icap || (meta[META_FLAGS] & FLAG_NONFINITE)) return;
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     if (p >= I) return;
-    build_node(p, keys, n, pos, mass, delta, off, box[4], node_a, nodes, inode_slot, inode_level,
-               inode_mask, multi_list, meta);
+    build_node(p, keys, nsys, sys_of, sys_off, pos, mass, delta, off, box, node_a, nodes,
+               inode_slot, inode_level, inode_mask, multi_list, meta);
 }
 
 // Multi-body leaf COM, one wave per leaf, in the order of compute_leaf_com
@@ -389,7 +403,7 @@ __global__ __launch_bounds__(256) void k_level_scatter(const uint32_t* __restric
 // The internal nodes of one level: segment [lvl_off[L * nblk], lvl_off[(L+1) * nblk])
 // of by_level (the last level ends at I).
 __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __restrict__ meta,
-                                                   uint64_t icap, uint32_t nblk,
+                                                   uint64_t icap, uint32_t nblk, uint32_t nsys,
                                                    const uint32_t* __restrict__ lvl_off,
                                                    const uint32_t* __restrict__ by_level,
                                                    const uint8_t* __restrict__ inode_mask,
@@ -403,7 +417,7 @@ __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __rest
         const uint32_t p = by_level[k];
         Com ch[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) ch[q] = load_com(&nodes[1 + 4ull * p + q]);
+        for (int q = 0; q < 4; q++) ch[q] = load_com(&nodes[nsys + 4ull * p + q]);
         store_com(&nodes[inode_slot[p]], combine(ch), inode_mask[p]);
     }
 }
@@ -412,19 +426,21 @@ void launch_tree_build(nb_ctx* c, int src) {
     const uint32_t n = (uint32_t)c->n;
     const uint32_t pairs = n > 1 ? n - 1 : 0;
     const uint32_t pg = pairs ? (pairs + 255) / 256 : 1;
+    const uint32_t bg = (n + 255) / 256;
     if (pairs) {
         hipLaunchKernelGGL(k_tree_delta, dim3(pg), dim3(256), 0, c->stream, c->b.keys[0], n,
-                           c->b.delta, c->b.cnt);
+                           c->b.sys_of, c->b.delta, c->b.cnt);
     }
     launch_scan_u32(c, c->b.cnt, pairs, &c->b.meta[META_INTERNAL]);
-    hipLaunchKernelGGL(k_tree_enum, dim3(pg), dim3(256), 0, c->stream, c->b.delta, c->b.cnt, n,
-                       c->b.meta, (uint64_t)c->icap, c->b.node_a, c->b.inode_level,
-                       c->b.pos[src], c->b.mass[src], c->b.box, c->b.nodes);
+    hipLaunchKernelGGL(k_tree_enum, dim3(bg), dim3(256), 0, c->stream, c->b.delta, c->b.cnt, n,
+                       c->b.sys_of, c->b.sys_off, c->b.meta, (uint64_t)c->icap, c->b.node_a,
+                       c->b.inode_level, c->b.pos[src], c->b.mass[src], c->b.box, c->b.nodes);
     const uint32_t ng = (uint32_t)((c->icap + 255) / 256);
-    hipLaunchKernelGGL(k_tree_nodes, dim3(ng), dim3(256), 0, c->stream, c->b.keys[0], n,
-                       c->b.pos[src], c->b.mass[src], c->b.delta, c->b.cnt, c->b.box, c->b.meta,
-                       (uint64_t)c->icap, c->b.node_a, c->b.nodes, c->b.inode_slot,
-                       c->b.inode_level, c->b.inode_mask, c->b.multi_list);
+    hipLaunchKernelGGL(k_tree_nodes, dim3(ng), dim3(256), 0, c->stream, c->b.keys[0], c->nsys,
+                       c->b.sys_of, c->b.sys_off, c->b.pos[src], c->b.mass[src], c->b.delta,
+                       c->b.cnt, c->b.box, c->b.meta, (uint64_t)c->icap, c->b.node_a,
+                       c->b.nodes, c->b.inode_slot, c->b.inode_level, c->b.inode_mask,
+                       c->b.multi_list);
     hipLaunchKernelGGL(k_leaf_multi, dim3(64), dim3(256), 0, c->stream, c->b.pos[src],
                        c->b.mass[src], c->b.multi_list, c->b.meta, c->b.nodes);
     hipLaunchKernelGGL(k_level_hist, dim3(ng), dim3(256), 0, c->stream, c->b.meta,
@@ -437,8 +453,8 @@ void launch_tree_build(nb_ctx* c, int src) {
     const uint32_t cg = ng < 512 ? ng : 512;
     for (int L = kMaxLevel - 1; L >= 0; L--) {
         hipLaunchKernelGGL(k_com_level, dim3(cg), dim3(256), 0, c->stream, L, c->b.meta,
-                           (uint64_t)c->icap, ng, c->b.lvl_cnt, c->b.by_level, c->b.inode_mask,
-                           c->b.inode_slot, c->b.nodes);
+                           (uint64_t)c->icap, ng, c->nsys, c->b.lvl_cnt, c->b.by_level,
+                           c->b.inode_mask, c->b.inode_slot, c->b.nodes);
     }
 }
 

@@ -38,7 +38,7 @@ EXPORTS = (
     "nb_last_error", "nb_status_string", "nb_debug_build", "nb_debug_tree",
     "nb_direct_sum_f64", "nb_mlp_set_weights", "nb_mlp_get_weights", "nb_mlp_init",
     "nb_mlp_generate", "nb_mlp_get_pairs", "nb_mlp_set_pairs", "nb_mlp_train", "nb_mlp_grad",
-    "nb_mlp_forward", "nb_debug_far_field",
+    "nb_mlp_forward", "nb_debug_far_field", "nb_create_batch",
 )
 MLP_PARAMS = 4866
 MLP_INPUTS = 8
@@ -82,6 +82,8 @@ def lib() -> C.CDLL:
         dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
         L.nb_params_default.argtypes = [C.POINTER(Params)]
         L.nb_create.argtypes = [C.POINTER(Params), C.c_uint64, P, P, P, C.POINTER(C.c_void_p)]
+        L.nb_create_batch.argtypes = [C.POINTER(Params), C.c_uint32, P, P, P, P,
+                                      C.POINTER(C.c_void_p)]
         L.nb_step.argtypes = [P, C.c_uint64]
         L.nb_get_state.argtypes = [P, P, P]
         L.nb_compute_forces.argtypes = [P, P, P, C.POINTER(C.c_uint64)]
@@ -137,7 +139,9 @@ def params(**kw) -> Params:
 class Context:
     """Owns one nb_ctx (bodies resident on the GPU)."""
 
-    def __init__(self, mass, pos, vel, **kw):
+    def __init__(self, mass, pos, vel, offsets=None, **kw):
+        """offsets (nsys + 1 rising indices from 0 to n): a batch of independent
+        systems in one context (nb_create_batch)."""
         self.n = int(len(mass))
         m = np.ascontiguousarray(mass, dtype=np.float64)
         x = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
@@ -147,7 +151,15 @@ class Context:
         self.params = params(**kw)
         h = C.c_void_p()
         L = lib()
-        st = L.nb_create(C.byref(self.params), self.n, _ptr(m), _ptr(x), _ptr(v), C.byref(h))
+        if offsets is None:
+            st = L.nb_create(C.byref(self.params), self.n, _ptr(m), _ptr(x), _ptr(v),
+                             C.byref(h))
+        else:
+            off = np.ascontiguousarray(offsets, dtype=np.uint64)
+            if off.size < 2 or int(off[-1]) != self.n:
+                raise ValueError("offsets must end at the number of bodies")
+            st = L.nb_create_batch(C.byref(self.params), off.size - 1, _ptr(off), _ptr(m),
+                                   _ptr(x), _ptr(v), C.byref(h))
         if st != NB_OK:
             raise NBodyError(st, L.nb_last_error(None).decode())
         self._h = h

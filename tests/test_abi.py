@@ -70,3 +70,25 @@ def test_null_arrays_and_null_ctx_are_rejected():
     assert L.nb_step(None, 1) == abi.NB_ERR_INVALID_ARG
     assert L.nb_get_state(None, None, None) == abi.NB_ERR_INVALID_ARG
     assert L.nb_status_string(abi.NB_ERR_OVERFLOW) == b"traversal stack overflow"
+
+
+@pytest.mark.parametrize("offsets,kw,status", [
+    ([0, 2, 4], dict(traversal=abi.TRAV_EXACT), abi.NB_ERR_NOT_SUPPORTED),
+    ([0, 2, 4], dict(algorithm=abi.ALGO_ALL_PAIRS, traversal=abi.TRAV_GROUP),
+     abi.NB_ERR_NOT_SUPPORTED),
+    ([0, 2, 2, 4], dict(traversal=abi.TRAV_GROUP), abi.NB_ERR_INVALID_ARG),
+    ([1, 2, 4], dict(traversal=abi.TRAV_GROUP), abi.NB_ERR_INVALID_ARG),
+])
+def test_batch_rejects_bad_offsets_and_unsupported_modes(offsets, kw, status):
+    m = np.ones(4)
+    x = np.zeros((4, 2))
+    with pytest.raises(abi.NBodyError) as e:
+        abi.Context(m, x, x, offsets=offsets, **kw)
+    assert e.value.status == status
+
+
+def test_batch_offsets_must_cover_the_arrays():
+    m = np.ones(4)
+    x = np.zeros((4, 2))
+    with pytest.raises(ValueError):
+        abi.Context(m, x, x, offsets=[0, 3, 5], traversal=abi.TRAV_GROUP)
