@@ -127,6 +127,10 @@ def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
         leg["far_field_cells"] = int(info.far_field_cells)
         leg["far_field_cells_per_group"] = info.far_field_cells / ((n + 63) // 64)
 
+    if args.tune_disk:
+        tn = ctx.autotune(2)
+        leg["autotune"] = {"group_budget": tn.group_budget, "settings": tn.candidates,
+                           "best_force_ms": tn.best_ms, "default_force_ms": tn.default_ms}
     ctx.step(args.warmup)
     t0 = time.perf_counter()
     ctx.step(args.steps)
@@ -197,6 +201,11 @@ def run_allpairs(nbody2d, ics, args) -> dict:
     abi = nbody2d.abi
     ctx = nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, softening=EPS,
                           algorithm=abi.ALGO_ALL_PAIRS, integrator=abi.INTEG_DK)
+    tune = None
+    if not args.no_tune:
+        tn = ctx.autotune(2)
+        tune = {"rows": tn.ap_rows, "split": tn.ap_split, "settings": tn.candidates,
+                "best_force_ms": tn.best_ms, "default_force_ms": tn.default_ms}
     t = ics.sample_targets(n, args.error_sample, seed=1)
     ref = nbody2d.direct_sum_f64(pos, mass, 1.0, EPS, t)
     acc, _, total = ctx.forces()
@@ -225,6 +234,7 @@ def run_allpairs(nbody2d, ics, args) -> dict:
                    "n_bodies": n},
         "force_error": {"rms": rms, "max": mx, "sample": int(len(t)),
                         "judge": "GPU FP64 direct sum"},
+        "autotune": tune,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic_from_profile("k_allpairs"),
@@ -316,6 +326,11 @@ def run_ensemble(nbody2d, ics, args) -> dict:
     # batched
     b = nbody2d.Context(mass, pos, vel, offsets=off, **kw)
     b.mlp_set_weights(w)
+    tune = None
+    if not args.no_tune:
+        tn = b.autotune(2)
+        tune = {"group_budget": tn.group_budget, "settings": tn.candidates,
+                "best_force_ms": tn.best_ms, "default_force_ms": tn.default_ms}
     # force error on 8 seeded systems against the FP64 direct sum
     fb, _, _ = b.forces()
     rng = np.random.default_rng(1)
@@ -368,6 +383,7 @@ def run_ensemble(nbody2d, ics, args) -> dict:
                         "judge": "GPU FP64 direct sum", "state": "initial"},
         "phase_ms_per_step_eager": per,
         "far_field_cells_per_step": int(info.far_field_cells),
+        "autotune": tune,
         "training": {"pairs": pairs, "loss": loss, "seconds": train_s,
                      "system": int(sizes[big])},
         "roofline": None,
@@ -395,6 +411,11 @@ def main() -> int:
                          "direct sum; two_disks: N=4,194,304 plain Barnes-Hut; ensemble: "
                          "512 systems in one batch context vs looped single contexts")
     ap.add_argument("--systems", type=int, default=512, help="ensemble: number of systems")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip nb_autotune before timing (all-pairs and ensemble workloads)")
+    ap.add_argument("--tune-disk", action="store_true",
+                    help="also autotune the disk legs (tuned on the initial state only, "
+                         "which the exploding disk soon leaves)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run a device copy of this many bytes first (counter calibration)")
     args = ap.parse_args()

@@ -200,6 +200,26 @@ int nb_debug_tree(nb_ctx* ctx, nb_node* nodes, uint64_t cap);
 int nb_direct_sum_f64(uint64_t n, const double* pos_xy, const double* mass, double G,
                       double softening, uint64_t m, const int64_t* targets, double* acc_xy);
 
+/* Autotuner: times one force evaluation of the current state (sort, tree and
+ * traversal, or the all-pairs kernel; best of `repeats`) for each setting of
+ * the context's launch tunables and keeps the fastest -- GROUP traversal:
+ * the list budget after which a traversal group is split (1,024 .. 65,536);
+ * all-pairs: bodies per thread (2, 4, 8) x the split of the j range (1 ..
+ * 64). The state is not advanced; the captured step graphs are rebuilt. No
+ * counterpart in the reference. The tree's stack depth (2,048 LDS entries,
+ * above the deepest walk) and the 64 bodies per wavefront are fixed by the
+ * kernels' design and not tuned. */
+typedef struct nb_tuning {
+    int32_t group_budget; /* chosen list budget (GROUP traversal) */
+    int32_t ap_rows;      /* chosen all-pairs bodies per thread */
+    int32_t ap_split;     /* chosen all-pairs j split */
+    int32_t candidates;   /* settings timed */
+    double best_ms;       /* force evaluation with the chosen setting */
+    double default_ms;    /* force evaluation with the setting before tuning */
+} nb_tuning;
+
+int nb_autotune(nb_ctx* ctx, uint32_t repeats, nb_tuning* out);
+
 /* ---------------------------------------------------------------------------
  * Learned far-field correction. No counterpart in the reference: its force
  * is the monopole of every accepted cell (BarnesHutCuda.cu:461-471). The

@@ -25,6 +25,8 @@ thread_local std::string g_err;
 // Default accepted-cell region per group of 64 bodies (learned far field);
 // a step that needs more halts, and the region grows to twice the need.
 constexpr uint32_t kInitialCells = 4096;
+// The group traversal's budget when nb_params.group_budget is 0.
+constexpr int32_t kDefaultBudget = 8192;
 
 int fail(nb_ctx* c, int code, const std::string& msg) {
     if (c)
@@ -475,7 +477,11 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
         split = std::max<uint32_t>(1, std::min<uint32_t>(split, 64));
         while (split > 1 && (n + split - 1) / split < 256) split--;
         c->ap_split = split;
-        NB_TRY(dalloc(c, &b.ap_part, (uint64_t)split * n));
+        // room for the largest split the autotuner tries
+        uint32_t smax = 64;
+        while (smax > split && (n + smax - 1) / smax < 256) smax >>= 1;
+        c->ap_split_max = std::max(smax, split);
+        NB_TRY(dalloc(c, &b.ap_part, (uint64_t)c->ap_split_max * n));
     }
     if (n) {
         NB_TRY(hipMemcpyAsync(b.pos[0], pos_xy, n * sizeof(double2), hipMemcpyHostToDevice,
@@ -693,6 +699,102 @@ int nb_get_timing(nb_ctx* c, double* phase_ms, int nphase, uint64_t* steps_timed
     if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
     for (int k = 0; k < nphase && k < NB_PHASE_COUNT; k++) phase_ms[k] = c->phase_ms[k];
     if (steps_timed) *steps_timed = c->steps_timed;
+    return NB_OK;
+}
+
+int nb_autotune(nb_ctx* c, uint32_t repeats, nb_tuning* out) {
+    if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
+    if (out) std::memset(out, 0, sizeof(*out));
+    if (c->n == 0) return NB_OK;
+    if (repeats == 0) repeats = 1;
+    const bool timing = c->timing;
+    // Force evaluation time (ms, best of `repeats`) with the current settings.
+    auto measure = [&](double* ms_out) -> int {
+        double best = 1e300;
+        for (uint32_t r = 0; r < repeats;) {
+            int rc = clear_meta(c);
+            if (rc) return rc;
+            c->timing = true;
+            c->marked = 0;
+            c->cur = enqueue_force(c, c->cur, false, false);
+            c->timing = timing;
+            NB_HIP(c, hipGetLastError());
+            rc = read_meta(c);
+            if (rc) return rc;
+            const uint32_t flags = c->host_meta[META_FLAGS];
+            c->last_internal = c->host_meta[META_INTERNAL];
+            if (flags & FLAG_NONFINITE) return fail(c, NB_ERR_NUMERIC, "non-finite positions");
+            if (flags & FLAG_STACK) return fail(c, NB_ERR_OVERFLOW, "traversal stack overflow");
+            if (flags & (FLAG_TREE_CAPACITY | FLAG_CELL_CAPACITY)) {
+                rc = grow_for(c, flags);  // then time this setting again
+                if (rc) return rc;
+                continue;
+            }
+            double ms = 0.0;
+            for (int ph : {(int)NB_PHASE_FORCE, (int)NB_PHASE_FAR_FIELD}) {
+                if (!((c->marked >> ph) & 1u)) continue;
+                float t = 0.f;
+                NB_HIP(c, hipEventElapsedTime(&t, c->ev[2 * ph], c->ev[2 * ph + 1]));
+                ms += t;
+            }
+            best = std::min(best, ms);
+            r++;
+        }
+        *ms_out = best;
+        return NB_OK;
+    };
+    nb_tuning t{};
+    double base = 0.0;
+    int rc = measure(&base);
+    if (rc) return rc;
+    double best = base;
+    t.candidates = 1;
+    if (is_bh(c) && c->p.traversal == NB_TRAV_GROUP) {
+        const int32_t orig = c->p.group_budget;
+        int32_t pick = orig;
+        for (int32_t budget : {1024, 2048, 4096, 8192, 16384, 65536}) {
+            if (budget == (orig > 0 ? orig : kDefaultBudget)) continue;
+            c->p.group_budget = budget;
+            double ms = 0.0;
+            rc = measure(&ms);
+            if (rc) return rc;
+            t.candidates++;
+            if (ms < best) {
+                best = ms;
+                pick = budget;
+            }
+        }
+        c->p.group_budget = pick;
+    } else if (!is_bh(c)) {
+        const uint32_t r0 = c->ap_rows, s0 = c->ap_split;
+        uint32_t pr = r0, ps = s0;
+        for (uint32_t rows : {2u, 4u, 8u}) {
+            for (uint32_t split = 1; split <= c->ap_split_max; split <<= 1) {
+                if ((rows == r0 && split == s0) || (c->n + split - 1) / split < 256) continue;
+                c->ap_rows = rows;
+                c->ap_split = split;
+                double ms = 0.0;
+                rc = measure(&ms);
+                if (rc) return rc;
+                t.candidates++;
+                if (ms < best) {
+                    best = ms;
+                    pr = rows;
+                    ps = split;
+                }
+            }
+        }
+        c->ap_rows = pr;
+        c->ap_split = ps;
+    }
+    rc = build_graphs(c);
+    if (rc) return rc;
+    t.group_budget = c->p.group_budget > 0 ? c->p.group_budget : kDefaultBudget;
+    t.ap_rows = (int32_t)c->ap_rows;
+    t.ap_split = (int32_t)c->ap_split;
+    t.best_ms = best;
+    t.default_ms = base;
+    if (out) *out = t;
     return NB_OK;
 }
 

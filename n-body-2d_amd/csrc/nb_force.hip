@@ -569,9 +569,8 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
 // Kahan compensation.
 // ---------------------------------------------------------------------------
 constexpr int kApThreads = 256;
-constexpr int kApR = 4;
 
-template <bool ZERO_EPS>
+template <bool ZERO_EPS, int kApR>
 __global__ __launch_bounds__(kApThreads) void k_allpairs(const double2* __restrict__ pos,
                                                          const double* __restrict__ mass,
                                                          uint32_t n, uint32_t chunk,
@@ -659,19 +658,30 @@ __global__ __launch_bounds__(256) void k_allpairs_finish(
     }
 }
 
+// Bodies per thread (c->ap_rows: 2, 4 or 8) and the j split (c->ap_split)
+// are the all-pairs launch's tunables (nb_autotune).
 void launch_allpairs(nb_ctx* c, int src, bool kick, bool count_step) {
     const uint32_t n = (uint32_t)c->n;
     if (n == 0) return;
-    const uint32_t ib = (n + kApThreads * kApR - 1) / (kApThreads * kApR);
+    const uint32_t R = c->ap_rows;
+    const uint32_t ib = (n + kApThreads * R - 1) / (kApThreads * R);
     const uint32_t split = c->ap_split;
     uint32_t chunk = (n + split - 1) / split;
     chunk = (chunk + kApThreads - 1) / kApThreads * kApThreads;
-    if (c->p.softening > 0.0)
-        hipLaunchKernelGGL((k_allpairs<false>), dim3(ib, split), dim3(kApThreads), 0, c->stream,
-                           c->b.pos[src], c->b.mass[src], n, chunk, c->b.consts, c->b.ap_part);
-    else
-        hipLaunchKernelGGL((k_allpairs<true>), dim3(ib, split), dim3(kApThreads), 0, c->stream,
-                           c->b.pos[src], c->b.mass[src], n, chunk, c->b.consts, c->b.ap_part);
+    const bool z = !(c->p.softening > 0.0);
+#define NB_AP(Z, RR)                                                                           \
+    hipLaunchKernelGGL((k_allpairs<Z, RR>), dim3(ib, split), dim3(kApThreads), 0, c->stream,   \
+                       c->b.pos[src], c->b.mass[src], n, chunk, c->b.consts, c->b.ap_part)
+#define NB_AP2(RR) if (z) NB_AP(true, RR); else NB_AP(false, RR)
+    if (R == 2) {
+        NB_AP2(2);
+    } else if (R == 8) {
+        NB_AP2(8);
+    } else {
+        NB_AP2(4);
+    }
+#undef NB_AP2
+#undef NB_AP
     hipLaunchKernelGGL(k_allpairs_finish, dim3((n + 255) / 256), dim3(256), 0, c->stream,
                        c->b.ap_part, split, n, c->b.consts, c->b.vel[src], c->b.acc, c->b.inter,
                        (int)kick, c->b.meta, (int)count_step);

@@ -157,6 +157,8 @@ struct nb_ctx {
     uint32_t sort_tiles = 0;
     uint32_t scan_blocks = 0;
     uint32_t ap_split = 1;
+    uint32_t ap_rows = 4;        // all-pairs bodies per thread (2, 4 or 8)
+    uint32_t ap_split_max = 1;   // j splits the partial buffer holds
     uint64_t device_bytes = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     bool timing = false;

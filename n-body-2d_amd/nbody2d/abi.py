@@ -38,7 +38,7 @@ EXPORTS = (
     "nb_last_error", "nb_status_string", "nb_debug_build", "nb_debug_tree",
     "nb_direct_sum_f64", "nb_mlp_set_weights", "nb_mlp_get_weights", "nb_mlp_init",
     "nb_mlp_generate", "nb_mlp_get_pairs", "nb_mlp_set_pairs", "nb_mlp_train", "nb_mlp_grad",
-    "nb_mlp_forward", "nb_debug_far_field", "nb_create_batch",
+    "nb_mlp_forward", "nb_debug_far_field", "nb_create_batch", "nb_autotune",
 )
 MLP_PARAMS = 4866
 MLP_INPUTS = 8
@@ -60,6 +60,13 @@ class Info(C.Structure):
         ("tree_nodes", C.c_uint64), ("internal_nodes", C.c_uint64),
         ("node_capacity", C.c_uint64), ("device_bytes", C.c_uint64),
         ("far_field_cells", C.c_uint64),
+    ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("group_budget", C.c_int32), ("ap_rows", C.c_int32), ("ap_split", C.c_int32),
+        ("candidates", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double),
     ]
 
 
@@ -113,6 +120,7 @@ def lib() -> C.CDLL:
         L.nb_mlp_grad.argtypes = [P, P, P, C.c_uint64, P, C.POINTER(C.c_float)]
         L.nb_mlp_forward.argtypes = [P, P, C.c_uint64, P]
         L.nb_debug_far_field.argtypes = [P, P, P, P, P, P, P, C.POINTER(C.c_uint32)]
+        L.nb_autotune.argtypes = [P, C.c_uint32, C.POINTER(Tuning)]
         del dp
         _lib = L
     return _lib
@@ -229,6 +237,11 @@ class Context:
         self._check(lib().nb_debug_tree(self._h, _ptr(nodes), count))
         return nodes
 
+
+    def autotune(self, repeats: int = 3) -> "Tuning":
+        t = Tuning()
+        self._check(lib().nb_autotune(self._h, int(repeats), C.byref(t)))
+        return t
 
     # ---- learned far field ----
     def mlp_set_weights(self, w) -> None:
