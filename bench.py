@@ -140,6 +140,9 @@ def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
     print(f"[bench] {leg['far_field']} theta={theta}: {leg['ms_per_step']:.3f} ms/step",
           flush=True)
 
+    if not learned:
+        leg["readback"] = readback_rates(ctx, min(args.steps, 10))
+
     # Kernel durations: the same steps launched eagerly with hipEvents around
     # every phase on the context stream.
     ctx.set_timing(True)
@@ -160,6 +163,35 @@ def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
                                       "steps": 2 * args.steps + args.warmup}
     leg["interactions_per_body_after_steps"] = total2 / n
     return leg
+
+
+def readback_rates(ctx, steps: int) -> dict:
+    """ms per step of one-step calls when the host needs the state in its own
+    body order after every step: no readback, the blocking readback
+    (nb_get_state, the reference's per-step scatter + copy), and the
+    asynchronous snapshot whose copy overlaps the next step."""
+    def timed(body):
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            body()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    out = {"steps": steps, "step_only_ms": timed(lambda: ctx.step(1)),
+           "blocking_ms": timed(lambda: (ctx.step(1), ctx.state()))}
+    prev = [None]
+
+    def pipelined():
+        ctx.step(1)
+        t = ctx.snapshot_begin()
+        if prev[0] is not None:
+            ctx.snapshot_wait(prev[0], copy=False)
+        prev[0] = t
+
+    out["snapshot_ms"] = timed(pipelined)
+    ctx.snapshot_wait(prev[0], copy=False)
+    print(f"[bench] readback per step: none {out['step_only_ms']:.3f} ms, blocking "
+          f"{out['blocking_ms']:.3f} ms, snapshot {out['snapshot_ms']:.3f} ms", flush=True)
+    return out
 
 
 def roofline_walk(leg, traversal_name):

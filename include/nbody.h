@@ -160,6 +160,27 @@ int nb_step(nb_ctx* ctx, uint64_t k);
  * caller's original body order. Either pointer may be NULL. */
 int nb_get_state(nb_ctx* ctx, double* pos_xy, double* vel_xy);
 
+/* Asynchronous, double-buffered state snapshot for rendering and output, in
+ * place of the reference's blocking per-step scatter + copy
+ * (scatter_to_original_order + sync_from_gpu_bodies, BarnesHutCuda.cu:498-506,
+ * 632-646). nb_snapshot_begin un-permutes the current state into the
+ * caller's body order on the GPU (ordered after every step already issued)
+ * and copies it to pinned host memory on a second stream, then returns at
+ * once with a ticket (1, 2, 3, ...); later nb_step calls run while the copy
+ * is in flight. Two slots alternate: ticket t stays readable until ticket
+ * t + 2 is begun. The first call allocates the slots (64 B per body on the
+ * GPU, 64 B per body pinned on the host). */
+int nb_snapshot_begin(nb_ctx* ctx, uint64_t* ticket);
+/* *ready = 1 when the snapshot's copy has finished, else 0. */
+int nb_snapshot_query(nb_ctx* ctx, uint64_t ticket, int* ready);
+/* Waits for the copy and points *pos_xy / *vel_xy at the snapshot (n
+ * interleaved x,y FP64 each, original order; either may be NULL), valid
+ * until ticket + 2 is begun or the context is destroyed; *iteration (may be
+ * NULL) is the step count the snapshot was taken at. A ticket that was never
+ * issued or whose slot was reused is NB_ERR_INVALID_ARG. */
+int nb_snapshot_wait(nb_ctx* ctx, uint64_t ticket, const double** pos_xy,
+                     const double** vel_xy, uint64_t* iteration);
+
 /* Accelerations (G included) of the current state in original order, and the
  * number of force terms per body (may be NULL) and in total (may be NULL).
  * Replaces the force half of compute_force (BarnesHutCuda.cu:382-486) without

@@ -8,6 +8,7 @@
 // host round trips or the per-step device-to-host copy of :632-646).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -102,7 +103,34 @@ void destroy_graphs(nb_ctx* c) {
     }
 }
 
+void free_snapshots(nb_ctx* c) {
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    for (auto& s : c->snap) {
+        dfree(s.dev);
+        if (s.host) (void)hipHostFree(s.host);
+        if (s.ready) (void)hipEventDestroy(s.ready);
+        if (s.done) (void)hipEventDestroy(s.done);
+        s = nb_ctx::Snapshot{};
+    }
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
+    c->copy_stream = nullptr;
+}
+
+// Snapshot slots, allocated on the first nb_snapshot_begin: all or nothing.
+hipError_t alloc_snapshots(nb_ctx* c) {
+    hipError_t e = hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking);
+    for (auto& s : c->snap) {
+        if (!e) e = hipEventCreateWithFlags(&s.ready, hipEventDisableTiming);
+        if (!e) e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+        if (!e) e = dalloc(c, &s.dev, 2 * c->n);
+        if (!e) e = hipHostMalloc((void**)&s.host, std::max<uint64_t>(32 * c->n, 256));
+    }
+    if (e) free_snapshots(c);
+    return e;
+}
+
 void free_all(nb_ctx* c) {
+    free_snapshots(c);
     destroy_graphs(c);
     free_tree(c);
     free_mlp(c);
@@ -271,6 +299,12 @@ int clear_meta(nb_ctx* c) {
     return NB_OK;
 }
 
+// The learned walk's accepted-cell count, a 64-bit counter in two meta words.
+uint64_t cells_total(const nb_ctx* c) {
+    return (uint64_t)c->host_meta[META_CELLS_TOTAL] |
+           (uint64_t)c->host_meta[META_CELLS_TOTAL + 1] << 32;
+}
+
 int read_meta(nb_ctx* c) {
     NB_HIP(c, hipMemcpyAsync(c->host_meta, c->b.meta, META_COUNT * sizeof(uint32_t),
                              hipMemcpyDeviceToHost, c->stream));
@@ -319,7 +353,7 @@ int build_and_force(nb_ctx* c, bool forces) {
         if (rc) return rc;
         const uint32_t flags = c->host_meta[META_FLAGS];
         c->last_internal = c->host_meta[META_INTERNAL];
-        if (forces) c->last_cells = c->host_meta[META_CELLS_TOTAL];
+        if (forces) c->last_cells = cells_total(c);
         if (flags & FLAG_NONFINITE) return fail(c, NB_ERR_NUMERIC, "non-finite body positions");
         if (flags & FLAG_STACK) return fail(c, NB_ERR_OVERFLOW, "traversal stack overflow");
         if (!(flags & (FLAG_TREE_CAPACITY | FLAG_CELL_CAPACITY))) return NB_OK;
@@ -578,7 +612,7 @@ int nb_step(nb_ctx* c, uint64_t k) {
         const uint32_t flags = c->host_meta[META_FLAGS];
         const uint64_t done = std::min<uint64_t>(c->host_meta[META_STEPS_DONE], remaining);
         c->last_internal = c->host_meta[META_INTERNAL];
-        if (done) c->last_cells = c->host_meta[META_CELLS_TOTAL] / done;
+        if (done) c->last_cells = cells_total(c) / done;
         c->iteration += done;
         c->sim_time += (double)done * c->dt;
         remaining -= done;
@@ -634,6 +668,62 @@ int nb_get_state(nb_ctx* c, double* pos_xy, double* vel_xy) {
             vel_xy[2 * j + 1] = v[2 * i + 1];
         }
     }
+    return NB_OK;
+}
+
+int nb_snapshot_begin(nb_ctx* c, uint64_t* ticket) {
+    if (!c || !ticket) return fail(c, NB_ERR_INVALID_ARG, "ctx or ticket is NULL");
+    const uint64_t n = c->n;
+    const uint64_t t = c->snap_next;
+    nb_ctx::Snapshot& s = c->snap[t & 1];
+    if (!c->copy_stream) NB_HIP(c, alloc_snapshots(c));
+    if (n > 0) {
+        // the slot's previous copy must have left the device buffer before the
+        // un-permute overwrites it
+        if (s.ticket) NB_HIP(c, hipStreamWaitEvent(c->stream, s.done, 0));
+        launch_unpermute(c, s.dev);
+        NB_HIP(c, hipGetLastError());
+        NB_HIP(c, hipEventRecord(s.ready, c->stream));
+        NB_HIP(c, hipStreamWaitEvent(c->copy_stream, s.ready, 0));
+        NB_HIP(c, hipMemcpyAsync(s.host, s.dev, 32 * n, hipMemcpyDeviceToHost, c->copy_stream));
+    }
+    NB_HIP(c, hipEventRecord(s.done, c->copy_stream));
+    s.ticket = t;
+    s.iteration = c->iteration;
+    c->snap_next = t + 1;
+    *ticket = t;
+    return NB_OK;
+}
+
+static nb_ctx::Snapshot* find_snapshot(nb_ctx* c, uint64_t ticket) {
+    if (!c || ticket == 0) return nullptr;
+    nb_ctx::Snapshot& s = c->snap[ticket & 1];
+    return s.ticket == ticket ? &s : nullptr;
+}
+
+int nb_snapshot_query(nb_ctx* c, uint64_t ticket, int* ready) {
+    if (!c || !ready) return fail(c, NB_ERR_INVALID_ARG, "ctx or ready is NULL");
+    nb_ctx::Snapshot* s = find_snapshot(c, ticket);
+    if (!s) return fail(c, NB_ERR_INVALID_ARG, "no snapshot holds this ticket");
+    const hipError_t e = hipEventQuery(s->done);
+    if (e == hipErrorNotReady) {
+        *ready = 0;
+        return NB_OK;
+    }
+    NB_HIP(c, e);
+    *ready = 1;
+    return NB_OK;
+}
+
+int nb_snapshot_wait(nb_ctx* c, uint64_t ticket, const double** pos_xy, const double** vel_xy,
+                     uint64_t* iteration) {
+    if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
+    nb_ctx::Snapshot* s = find_snapshot(c, ticket);
+    if (!s) return fail(c, NB_ERR_INVALID_ARG, "no snapshot holds this ticket");
+    NB_HIP(c, hipEventSynchronize(s->done));
+    if (pos_xy) *pos_xy = s->host;
+    if (vel_xy) *vel_xy = s->host + 2 * c->n;
+    if (iteration) *iteration = s->iteration;
     return NB_OK;
 }
 

@@ -511,7 +511,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     }
     if (overflow && l == 0) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_STACK);
     if (LEARN && l == 0) {
-        atomicAdd(&meta[META_CELLS_TOTAL], cbase);
+        atomicAdd((unsigned long long*)&meta[META_CELLS_TOTAL], (unsigned long long)cbase);
         if (cbase > cell_cap) {
             atomicMax(&meta[META_CELLS_NEEDED], cbase);
             atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_CELL_CAPACITY);

@@ -43,7 +43,7 @@ enum Meta : int {
     META_STEPS_DONE = 2, // completed steps since the host last cleared it
     META_MULTI_LEAVES = 3, // multi-body leaves queued by the current build
     META_CELLS_NEEDED = 4, // most accepted cells of one group (learned far field)
-    META_CELLS_TOTAL = 32, // accepted cells of all groups (learned far field)
+    META_CELLS_TOTAL = 32, // accepted cells of all groups (learned far field), u64 in 32..33
     META_QHEAD = 64,       // traversal work queue: entries pushed
     META_QTAIL = 96,       // traversal work queue: entries taken
     META_COUNT = 128
@@ -178,6 +178,17 @@ struct nb_ctx {
     uint32_t cell_cap = 0;          // accepted-cell region per group
     uint32_t queue_cap = 0;         // traversal work-queue entries
     uint64_t last_cells = 0;        // accepted cells of the last nb_compute_forces
+    // asynchronous snapshots: two slots, used by tickets alternately
+    struct Snapshot {
+        double2* dev = nullptr;     // [2n] positions then velocities, original order
+        double* host = nullptr;     // pinned [4n] copy of dev
+        hipEvent_t ready = nullptr; // un-permute done (context stream)
+        hipEvent_t done = nullptr;  // device-to-host copy done (copy stream)
+        uint64_t ticket = 0;        // 0: never used
+        uint64_t iteration = 0;     // steps taken when the snapshot was begun
+    } snap[2];
+    hipStream_t copy_stream = nullptr;
+    uint64_t snap_next = 1;
     std::string err;
 };
 
@@ -188,6 +199,7 @@ void launch_bbox(nb_ctx* c, int src);
 void launch_morton(nb_ctx* c, int src);
 void launch_radix_sort(nb_ctx* c);
 void launch_gather(nb_ctx* c, int src);
+void launch_unpermute(nb_ctx* c, double2* out);
 void launch_scan_u32(nb_ctx* c, uint32_t* data, uint32_t count, uint32_t* total_out);
 
 // ---- launchers (nb_tree.hip) ----

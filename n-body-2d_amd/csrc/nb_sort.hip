@@ -414,6 +414,29 @@ void launch_gather(nb_ctx* c, int src) {
 }
 
 // ---------------------------------------------------------------------------
+// Un-permute the state into the caller's body order for a snapshot
+// (scatter_to_original_order, BarnesHutCuda.cu:498-506): out[0..n) holds the
+// positions and out[n..2n) the velocities. HBM traffic: 4 B index + 32 B read
+// + 32 B written per body.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_unpermute(const uint32_t* __restrict__ orig,
+                                                   const double2* __restrict__ pos,
+                                                   const double2* __restrict__ vel,
+                                                   double2* __restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = orig[i];
+    out[j] = pos[i];
+    out[(uint64_t)n + j] = vel[i];
+}
+
+void launch_unpermute(nb_ctx* c, double2* out) {
+    const uint32_t n = (uint32_t)c->n;
+    hipLaunchKernelGGL(k_unpermute, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                       c->b.orig[c->cur], c->b.pos[c->cur], c->b.vel[c->cur], out, n);
+}
+
+// ---------------------------------------------------------------------------
 // Exclusive in-place scan of u32 (reduce / top-level scan / downsweep).
 // ---------------------------------------------------------------------------
 constexpr int kScanThreads = 256;

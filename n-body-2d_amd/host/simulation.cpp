@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <random>
 
 #include "nbody2d_host.hpp"
@@ -136,10 +137,19 @@ bool Simulation::should_stop() {
 
 // One iteration of the reference's loop body: step, refresh Bodies in the
 // caller's order (BarnesHutCuda.cu:632-646), then post_iteration
-// (Simulation.cpp:68-72): stats, dt latch, iteration++.
+// (Simulation.cpp:68-72): stats, dt latch, iteration++. The un-permute runs
+// on the GPU and the copy lands in pinned memory (a snapshot slot).
 void Simulation::step_once() {
     check(nb_step(ctx_, 1), "nb_step");
-    check(nb_get_state(ctx_, bodies_.pos.data(), bodies_.vel.data()), "nb_get_state");
+    uint64_t ticket = 0;
+    const double *p = nullptr, *v = nullptr;
+    check(nb_snapshot_begin(ctx_, &ticket), "nb_snapshot_begin");
+    check(nb_snapshot_wait(ctx_, ticket, &p, &v, nullptr), "nb_snapshot_wait");
+    const uint64_t n = bodies_.n();
+    if (n) {
+        std::memcpy(bodies_.pos.data(), p, 2 * n * sizeof(double));
+        std::memcpy(bodies_.vel.data(), v, 2 * n * sizeof(double));
+    }
     sim_time_ += dt_;
     iteration_++;
     {

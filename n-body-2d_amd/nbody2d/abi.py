@@ -39,6 +39,7 @@ EXPORTS = (
     "nb_direct_sum_f64", "nb_mlp_set_weights", "nb_mlp_get_weights", "nb_mlp_init",
     "nb_mlp_generate", "nb_mlp_get_pairs", "nb_mlp_set_pairs", "nb_mlp_train", "nb_mlp_grad",
     "nb_mlp_forward", "nb_debug_far_field", "nb_create_batch", "nb_autotune",
+    "nb_snapshot_begin", "nb_snapshot_query", "nb_snapshot_wait",
 )
 MLP_PARAMS = 4866
 MLP_INPUTS = 8
@@ -121,6 +122,11 @@ def lib() -> C.CDLL:
         L.nb_mlp_forward.argtypes = [P, P, C.c_uint64, P]
         L.nb_debug_far_field.argtypes = [P, P, P, P, P, P, P, C.POINTER(C.c_uint32)]
         L.nb_autotune.argtypes = [P, C.c_uint32, C.POINTER(Tuning)]
+        L.nb_snapshot_begin.argtypes = [P, C.POINTER(C.c_uint64)]
+        L.nb_snapshot_query.argtypes = [P, C.c_uint64, C.POINTER(C.c_int)]
+        L.nb_snapshot_wait.argtypes = [P, C.c_uint64, C.POINTER(C.POINTER(C.c_double)),
+                                       C.POINTER(C.POINTER(C.c_double)),
+                                       C.POINTER(C.c_uint64)]
         del dp
         _lib = L
     return _lib
@@ -198,6 +204,32 @@ class Context:
         vel = np.empty((self.n, 2))
         self._check(lib().nb_get_state(self._h, _ptr(pos), _ptr(vel)))
         return pos, vel
+
+    def snapshot_begin(self) -> int:
+        """Starts an asynchronous copy of the current state; returns its ticket."""
+        t = C.c_uint64()
+        self._check(lib().nb_snapshot_begin(self._h, C.byref(t)))
+        return int(t.value)
+
+    def snapshot_ready(self, ticket: int) -> bool:
+        r = C.c_int()
+        self._check(lib().nb_snapshot_query(self._h, int(ticket), C.byref(r)))
+        return bool(r.value)
+
+    def snapshot_wait(self, ticket: int, copy: bool = True):
+        """(pos, vel, iteration) of a snapshot. With copy=False the arrays are
+        views of the pinned slot, overwritten once ticket + 2 is begun."""
+        pp, vp = C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+        it = C.c_uint64()
+        self._check(lib().nb_snapshot_wait(self._h, int(ticket), C.byref(pp), C.byref(vp),
+                                           C.byref(it)))
+        if self.n == 0:
+            return np.zeros((0, 2)), np.zeros((0, 2)), int(it.value)
+        pos = np.ctypeslib.as_array(pp, shape=(self.n, 2))
+        vel = np.ctypeslib.as_array(vp, shape=(self.n, 2))
+        if copy:
+            pos, vel = pos.copy(), vel.copy()
+        return pos, vel, int(it.value)
 
     def forces(self):
         acc = np.empty((self.n, 2))
