@@ -158,11 +158,15 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
 // from the group centre (the FP64 subtraction is done once per entry).
 // Phase 2: every lane sums the list (broadcast LDS reads), 14 flops per entry,
 // FP32 tile sums added with Kahan compensation. The list is flushed whenever
-// fewer than 64 free entries remain. Taking up to 64 cells per step from the
-// top of the stack needs about 1,500 entries on a 1M-body disk at theta 0.5,
-// hence 2,048; the take is throttled so the stack can never overflow.
+// fewer than 64 free entries remain. A step takes up to 64 cells from the top
+// of the stack and pushes at most 4 children for each (net +3). The take is
+// throttled to keep kGReserve entries free: from any stack depth sp, taking
+// one cell at a time (depth first) never needs more than sp + 3 * 30 + 1
+// entries, because a cell at level L has at most 30 - L levels below it, so
+// the stack cannot overflow.
 // ---------------------------------------------------------------------------
 constexpr int kGStack = 2048;
+constexpr int kGReserve = 3 * kMaxLevel + 4;
 constexpr int kGList = 192;
 constexpr int kGFlush = kGList - 64;
 // Before its walk a group is split at its largest Morton jump when the two
@@ -359,9 +363,8 @@ __global__ __launch_bounds__(64) void k_traverse_group(
             bool own_multi = false, split = false;
             if (l == 0) s_stack[0] = root;  // the root of this group's system
             while (sp > 0) {
-                int take = min(64, sp);
-                take = min(take, (kGStack - sp) / 3);
-                if (take <= 0) {
+                const int take = max(1, min(min(64, sp), (kGStack - kGReserve - sp) / 3));
+                if (sp + 3 > kGStack) {  // unreachable: see kGReserve
                     overflow = true;
                     break;
                 }
