@@ -63,7 +63,8 @@ enum nb_far_field { NB_FAR_MONOPOLE = 0, NB_FAR_LEARNED = 1 };
  * when the test holds for the whole group's bounding box (a conservative
  * superset of every member's test), collects the accepted cells and leaf
  * bodies in an LDS interaction list, then every lane sums the whole list; a
- * group whose list grows past group_budget is split in halves, down to
+ * group whose list grows past group_budget is split into pieces of 8 bodies
+ * (halves below that), shared with other wavefronts and walked again, down to
  * single bodies, whose group test is exactly their own opening test.
  * GROUP applies more (never fewer) force terms per body than EXACT at the
  * same theta, so its force error is no larger. */
@@ -84,7 +85,7 @@ typedef struct nb_params {
     int32_t use_graph;  /* 1: steps replay a captured hipGraph (default) */
     int32_t traversal;  /* enum nb_traversal */
     int32_t group_budget; /* GROUP: interaction-list entries after which a group
-                             is split in halves and walked again (0 = 8192) */
+                             is split into pieces and walked again (0 = 16384) */
     int32_t cell_capacity; /* LEARNED: initial accepted-cell entries per group of
                               64 bodies (0 = 4096); a step that needs more
                               halts, the region grows and the step runs again */
@@ -102,7 +103,8 @@ typedef struct nb_info {
     uint64_t far_field_cells; /* accepted cells the network corrected per force
                                  evaluation (far_field = LEARNED): the last
                                  nb_compute_forces, or the mean over the last
-                                 nb_step call; each is evaluated at 4 points */
+                                 nb_step call; the network runs once per cell
+                                 and traversal subgroup */
 } nb_info;
 
 /* Timing phases reported by nb_get_timing (milliseconds, summed over the

@@ -26,8 +26,6 @@ thread_local std::string g_err;
 // Default accepted-cell region per group of 64 bodies (learned far field);
 // a step that needs more halts, and the region grows to twice the need.
 constexpr uint32_t kInitialCells = 4096;
-// The group traversal's budget when nb_params.group_budget is 0.
-constexpr int32_t kDefaultBudget = 8192;
 
 int fail(nb_ctx* c, int code, const std::string& msg) {
     if (c)
@@ -843,7 +841,7 @@ int nb_autotune(nb_ctx* c, uint32_t repeats, nb_tuning* out) {
         const int32_t orig = c->p.group_budget;
         int32_t pick = orig;
         for (int32_t budget : {1024, 2048, 4096, 8192, 16384, 65536}) {
-            if (budget == (orig > 0 ? orig : kDefaultBudget)) continue;
+            if (budget == (orig > 0 ? orig : kGroupBudget)) continue;
             c->p.group_budget = budget;
             double ms = 0.0;
             rc = measure(&ms);
@@ -879,7 +877,7 @@ int nb_autotune(nb_ctx* c, uint32_t repeats, nb_tuning* out) {
     }
     rc = build_graphs(c);
     if (rc) return rc;
-    t.group_budget = c->p.group_budget > 0 ? c->p.group_budget : kDefaultBudget;
+    t.group_budget = c->p.group_budget > 0 ? c->p.group_budget : kGroupBudget;
     t.ap_rows = (int32_t)c->ap_rows;
     t.ap_split = (int32_t)c->ap_split;
     t.best_ms = best;

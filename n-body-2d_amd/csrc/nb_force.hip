@@ -171,10 +171,9 @@ constexpr int kGList = 192;
 constexpr int kGFlush = kGList - 64;
 // Before its walk a group is split at its largest Morton jump when the two
 // parts are much more compact than the whole. As a safety net, a group whose
-// interaction list still passes `budget` entries (default 8192) is split into
+// interaction list still passes `budget` entries (default kGroupBudget) is split into
 // pieces of 8 bodies (halves below that), shared with other wavefronts
 // through the work queue, and walked again, down to single bodies.
-constexpr int kGBudget = 8192;
 
 template <bool ZERO_EPS>
 __device__ __forceinline__ void eval_list(const float4* __restrict__ list, int cnt, float bx,
@@ -529,7 +528,7 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
         const uint32_t groups = c->ngroups;
         const bool z = !(c->p.softening > 0.0);
         const bool gd = c->p.self_guard != 0;
-        const int budget = c->p.group_budget > 0 ? c->p.group_budget : kGBudget;
+        const int budget = c->p.group_budget > 0 ? c->p.group_budget : kGroupBudget;
         const bool learn = c->p.far_field == NB_FAR_LEARNED;
 #define NB_TRAVG(K, Z, GD, LN)                                                                 \
     hipLaunchKernelGGL((k_traverse_group<K, Z, GD, LN>), dim3(groups), dim3(64), 0, c->stream, \
