@@ -48,6 +48,13 @@ FLOPS_PER_INTERACTION = 14
 MLP_FLOPS_PER_EVAL = 2 * (8 * 64 + 64 * 64 + 64 * 2)
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
+HBM_PEAK_TBS = 8.0
+# Algorithmic HBM bytes per body: radix sort 8 x (12 B read + 12 B written) plus
+# the gather (4 B index + 44 B read + 44 B written); tree build 32 B read per
+# body plus one 64-byte record written per tree slot.
+SORT_BYTES_PER_BODY = 8 * 24 + 92
+BUILD_BYTES_PER_BODY = 32
+NODE_BYTES = 64
 TRAIN_PAIRS = 1 << 22
 TRAIN_SAMPLE_BODIES = 8192
 METRIC = "N=1M body-steps/s: plain BH vs learned far-field BH at equal force error"
@@ -144,14 +151,24 @@ def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
         leg["readback"] = readback_rates(ctx, min(args.steps, 10))
 
     # Kernel durations: the same steps launched eagerly with hipEvents around
-    # every phase on the context stream.
+    # every phase on the context stream. Before each one, nb_compute_forces on
+    # the same state counts the interactions (and accepted cells) that the
+    # step's walk is about to process, so the roofline divides the work of
+    # exactly the timed launches by their time.
     ctx.set_timing(True)
-    ctx.step(args.steps)
+    inter_timed = cells_timed = 0
+    for _ in range(args.steps):
+        _, _, tot = ctx.forces()
+        inter_timed += tot
+        if learned:
+            cells_timed += int(ctx.info().far_field_cells)
+        ctx.step(1)
     phase_ms, steps_timed = ctx.timing()
     ctx.set_timing(False)
     leg["phase_ms_per_step_eager"] = {k: v / steps_timed for k, v in phase_ms.items()}
+    leg["interactions_per_step_timed"] = inter_timed / steps_timed
     if learned:
-        leg["far_field_cells_per_step_timed"] = int(ctx.info().far_field_cells)
+        leg["far_field_cells_per_step_timed"] = cells_timed / steps_timed
 
     # and on the state after the timed steps (each leg's own trajectory)
     acc2, _, total2 = ctx.forces()
@@ -163,6 +180,19 @@ def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
                                       "steps": 2 * args.steps + args.warmup}
     leg["interactions_per_body_after_steps"] = total2 / n
     return leg
+
+
+def hbm_sort_build(leg, n: int) -> dict:
+    """Sort (+ gather) and tree build against HBM peak: algorithmic bytes per
+    step over the phase's eager hipEvent time."""
+    ph = leg["phase_ms_per_step_eager"]
+    out = {}
+    for name, nbytes in (("sort", SORT_BYTES_PER_BODY * n),
+                         ("build", BUILD_BYTES_PER_BODY * n + NODE_BYTES * leg["tree_nodes"])):
+        tbs = nbytes / (ph[name] * 1e-3) / 1e12
+        out[name] = {"bytes": nbytes, "ms": ph[name], "achieved_tb_s": tbs,
+                     "peak_tb_s": HBM_PEAK_TBS, "frac": tbs / HBM_PEAK_TBS}
+    return out
 
 
 def readback_rates(ctx, steps: int) -> dict:
@@ -196,17 +226,18 @@ def readback_rates(ctx, steps: int) -> dict:
 
 def roofline_walk(leg, traversal_name):
     force_ms = leg["phase_ms_per_step_eager"]["force"]
-    achieved = leg["interactions"] * FLOPS_PER_INTERACTION / (force_ms * 1e-3) / 1e12
+    achieved = leg["interactions_per_step_timed"] * FLOPS_PER_INTERACTION / (force_ms * 1e-3) / 1e12
     kernel = "k_traverse_group" if traversal_name == "group" else "k_traverse"
+    key = kernel + (" learned" if leg["far_field"] == "learned" else "")
     return {
         "bound": "mfma",
         "achieved": achieved,
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS,
-        "traffic": traffic_from_profile(kernel),
-        "kernel": kernel + " (FP32 vector; 14 flops per interaction)",
-        "interactions_per_launch": leg["interactions"],
+        "traffic": traffic_from_profile(key),
+        "kernel": key + " (FP32 vector; 14 flops per interaction)",
+        "interactions_per_launch": leg["interactions_per_step_timed"],
         "avg_launch_ms": force_ms,
     }
 
@@ -299,11 +330,14 @@ def run_two_disks(nbody2d, ics, args) -> dict:
     ref2 = nbody2d.direct_sum_f64(p_now, mass, 1.0, EPS, t)
     e1 = ics.force_errors(acc2[t], ref2)
     ctx.set_timing(True)
-    ctx.step(min(args.steps, 10))
+    inter_timed = 0
+    for _ in range(min(args.steps, 10)):
+        inter_timed += ctx.forces()[2]  # the interactions of the next step's walk
+        ctx.step(1)
     phase_ms, steps_timed = ctx.timing()
     ctx.close()
     per = {k: v / steps_timed for k, v in phase_ms.items()}
-    achieved = total2 * FLOPS_PER_INTERACTION / (per["force"] * 1e-3) / 1e12
+    achieved = inter_timed / steps_timed * FLOPS_PER_INTERACTION / (per["force"] * 1e-3) / 1e12
     return {
         "metric": "N=4M two-disk body-steps/s (plain BH)",
         "value": n * args.steps / wall, "unit": "body-steps/s",
@@ -516,6 +550,7 @@ def main() -> int:
         "force_error": main_leg["force_error"],
         "legs": legs,
         "roofline": roofline_walk(main_leg, main_leg["traversal"]),
+        "hbm_sort_build": hbm_sort_build(main_leg, n),
     }
     if len(legs) == 2:
         out["learned_over_plain"] = (legs["learned"]["body_steps_per_s"]
@@ -532,6 +567,7 @@ def main() -> int:
                 "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": flops / (ff_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
                 "kernel": "k_mlp_apply (BF16 MFMA; 9,472 flops per accepted cell)",
+                "traffic": traffic_from_profile("k_mlp_apply"),
                 "avg_launch_ms": ff_ms,
             }
     if not args.no_cpu:

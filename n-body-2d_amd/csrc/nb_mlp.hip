@@ -7,7 +7,8 @@
 // at theta (cells holding the body itself are never accepted); every accepted
 // internal cell with at most kPairMaxBodies bodies becomes a pair. Its target
 // is the exact FP64 Plummer acceleration of the cell's bodies at the sample
-// body minus the cell's monopole, in units of M s^2 / r^4.
+// body minus the cell's monopole, along (u, w) in units of M s^2 r / R^5
+// (the network's output frame, nb_mlp.h).
 #include <algorithm>
 #include <cmath>
 #include <cstring>

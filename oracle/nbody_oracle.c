@@ -24,6 +24,18 @@
  *   or_tree_forces            theta=0 equals or_direct_sum; two-body analytic
  *   or_model_step             free particle / constant-velocity closed form
  *   or_direct_sum             two-body and ring closed forms, Newton-3
+ *   Part D (the reference has no counterpart; pinned by mathematics):
+ *   or_quadrupole             order of the residual exact - monopole -
+ *                             quadrupole against brute-force sums
+ *   or_quadrupole_jacobian    central differences of or_quadrupole
+ *   or_cell_correction        brute-force pair sums; zero for one body
+ *   or_mlp_features           hand-computed values
+ *   or_mlp_forward            a network built to compute a known linear map;
+ *                             bf16 rounding against bit-level rounding
+ *   or_mlp_grad               central differences of or_mlp_forward's loss
+ * Parity unpinned: the reference GPU's own COM and rsqrt rounding (nvcc's
+ * contraction choices; this file fixes one explicit choice) and the
+ * clock-seeded sampled softening branch (Simulation.cpp:104-122).
  */
 #define _GNU_SOURCE
 #include "nbody_oracle.h"
