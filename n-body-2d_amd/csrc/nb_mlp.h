@@ -107,12 +107,22 @@ __device__ inline f32x4 relu4(f32x4 v) {
     return r;
 }
 
+// ReLU of two packed BF16: a BF16 with its sign bit set is a negative int16,
+// so max(int16, 0) per half (v_pk_max_i16) zeroes exactly the negative values
+// (and -0) and leaves the rest: one instruction for two activations, instead
+// of an FP32 max (with its canonicalisation) per activation before packing.
+__device__ inline uint32_t relu_bf16x2(uint32_t v) {
+    typedef short s16x2 __attribute__((ext_vector_type(2)));
+    const s16x2 x = __builtin_elementwise_max(__builtin_bit_cast(s16x2, v), (s16x2){0, 0});
+    return __builtin_bit_cast(uint32_t, x);
+}
+
 // Two output tiles of a layer (ReLU applied) as the B operand of one K block.
 __device__ inline bf16x8 pack8_relu(f32x4 a, f32x4 b) {
-    a = relu4(a);
-    b = relu4(b);
-    return __builtin_bit_cast(bf16x8, make_uint4(pk2(a.x, a.y), pk2(a.z, a.w), pk2(b.x, b.y),
-                                                 pk2(b.z, b.w)));
+    return __builtin_bit_cast(bf16x8, make_uint4(relu_bf16x2(pk2(a.x, a.y)),
+                                                 relu_bf16x2(pk2(a.z, a.w)),
+                                                 relu_bf16x2(pk2(b.x, b.y)),
+                                                 relu_bf16x2(pk2(b.z, b.w))));
 }
 
 // The 8 inputs of one (cell, point) pair; returns the output scale and the

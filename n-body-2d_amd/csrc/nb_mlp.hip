@@ -382,23 +382,47 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     const int nsub = (int)min(cell_cnt[g], 64u);
     const uint64_t slots = (uint64_t)groups * cell_cap;
     const int col = l & 15;
+    // The records of the next batch of cells are loaded while the current
+    // batch runs through the network (raw fields, converted when used).
+    struct CellIn {
+        double cx, cy, wsq;
+        float mf, ixx, ixy, iyy;
+        bool ok;
+    };
+    auto load_cells = [&](uint64_t b0, uint64_t end) {
+        CellIn c{0.0, 0.0, 1.0, 1.f, 0.f, 0.f, 0.f, false};
+        const uint64_t cell = b0 + l;
+        if (cell < end) {
+            const NodeRec* r = nodes + cell_slot[cell];
+            c.cx = r->cx;
+            c.cy = r->cy;
+            c.wsq = r->wsq;
+            c.mf = r->mf;
+            c.ixx = r->ixx;
+            c.ixy = r->ixy;
+            c.iyy = r->iyy;
+            c.ok = true;
+        }
+        return c;
+    };
     for (int ks = 0; ks < nsub; ks++) {
         const SubRec sub = subs[(uint64_t)g * 64 + ks];
         const uint64_t end = min((uint64_t)sub.start + sub.count, slots);
         float cx = 0.f, cy = 0.f, k00 = 0.f, k01 = 0.f, k10 = 0.f, k11 = 0.f;
+        CellIn nx = load_cells(sub.start, end);
         for (uint64_t b0 = sub.start; b0 < end; b0 += 64) {
-            const uint64_t cell = b0 + l;
-            const bool ok = cell < end;
+            const CellIn cur = nx;
+            if (b0 + 64 < end) nx = load_cells(b0 + 64, end);
+            const bool ok = cur.ok;
             float dx = 1.f, dy = 0.f, s = 1.f, M = 1.f, ixx = 0.f, ixy = 0.f, iyy = 0.f;
             if (ok) {
-                const NodeRec* r = nodes + cell_slot[cell];
-                dx = (float)(r->cx - sub.ox);
-                dy = (float)(r->cy - sub.oy);
-                s = (float)sqrt(r->wsq);
-                M = r->mf;
-                ixx = r->ixx;
-                ixy = r->ixy;
-                iyy = r->iyy;
+                dx = (float)(cur.cx - sub.ox);
+                dy = (float)(cur.cy - sub.oy);
+                s = sqrtf((float)cur.wsq);
+                M = cur.mf;
+                ixx = cur.ixx;
+                ixy = cur.ixy;
+                iyy = cur.iyy;
             }
             float f[8], ux, uy, J[4];
             float scale = mlp_features(dx, dy, s, M, ixx, ixy, iyy, eps2f, inv_mroot, inv_ell, f,
