@@ -1,0 +1,54 @@
+"""bench.py's derived figures and its JSON contract pieces that need no GPU:
+the roofline arithmetic, the sort/build HBM fractions, the traffic figures it
+reads from the committed rocprofv3 summaries, and the one-GPU guard."""
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+REPO = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(REPO))
+import bench  # noqa: E402
+
+
+def _leg(**kw):
+    leg = {"far_field": "monopole", "interactions": 700 * 2**20,
+           "interactions_per_step_timed": 7.5e8, "tree_nodes": 3_000_000,
+           "phase_ms_per_step_eager": {"sort": 0.25, "build": 0.4, "force": 0.6}}
+    leg.update(kw)
+    return leg
+
+
+def test_roofline_uses_the_timed_launches():
+    leg = _leg()
+    r = bench.roofline_walk(leg, "group")
+    # 7.5e8 interactions x 14 flops in 0.6 ms
+    assert r["achieved"] == pytest.approx(7.5e8 * 14 / 0.6e-3 / 1e12)
+    assert r["frac"] == pytest.approx(r["achieved"] / 157.3)
+    assert r["peak"] == 157.3 and r["unit"] == "TFLOP/s"
+    assert r["interactions_per_launch"] == 7.5e8 and r["avg_launch_ms"] == 0.6
+
+
+def test_roofline_traffic_comes_from_the_profiles_per_leg():
+    plain = bench.roofline_walk(_leg(), "group")["traffic"]
+    learned = bench.roofline_walk(_leg(far_field="learned"), "group")["traffic"]
+    assert plain is not None and learned is not None
+    assert 1e8 < plain < 1e9 and 1e8 < learned < 1e9
+    assert bench.traffic_from_profile("k_mlp_apply") is not None
+    assert bench.traffic_from_profile("no_such_kernel") is None
+
+
+def test_sort_build_hbm_fractions():
+    n = 1 << 20
+    h = bench.hbm_sort_build(_leg(), n)
+    assert h["sort"]["bytes"] == (8 * 24 + 92) * n
+    assert h["build"]["bytes"] == 32 * n + 64 * 3_000_000
+    assert h["sort"]["achieved_tb_s"] == pytest.approx(h["sort"]["bytes"] / 0.25e-3 / 1e12)
+    assert h["build"]["frac"] == pytest.approx(h["build"]["achieved_tb_s"] / 8.0)
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, str(REPO / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
