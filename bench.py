@@ -55,8 +55,14 @@ HBM_PEAK_TBS = 8.0
 SORT_BYTES_PER_BODY = 8 * 24 + 92
 BUILD_BYTES_PER_BODY = 32
 NODE_BYTES = 64
-TRAIN_PAIRS = 1 << 22
-TRAIN_SAMPLE_BODIES = 8192
+# Learned leg training: pairs from 65,536 sample bodies, then Adam at batch
+# 16,384 for S steps at 2e-3, S/2 at 5e-4 and S/2 at 1e-4 (S = --train-steps).
+TRAIN_PAIRS = 1 << 24
+TRAIN_SAMPLE_BODIES = 65536
+TRAIN_BATCH = 16384
+TRAIN_SCHEDULE = ((1.0, 2e-3), (0.5, 5e-4), (0.5, 1e-4))
+# Plain legs timed to find the plain theta at the learned leg's error.
+EQUAL_ERROR_THETAS = (0.4, 0.35, 0.3, 0.25)
 METRIC = "N=1M body-steps/s: plain BH vs learned far-field BH at equal force error"
 
 
@@ -110,14 +116,16 @@ def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
         ctx.mlp_init(1)
         pairs = ctx.mlp_generate(theta, TRAIN_SAMPLE_BODIES, TRAIN_PAIRS, seed=7)
         t1 = time.perf_counter()
-        l1 = ctx.mlp_train(args.train_steps, 4096, 2e-3, seed=1)
-        l2 = ctx.mlp_train(args.train_steps // 2, 4096, 4e-4, seed=2)
+        losses, steps = [], 0
+        for i, (frac, lr) in enumerate(TRAIN_SCHEDULE):
+            k = max(1, int(args.train_steps * frac))
+            losses.append(ctx.mlp_train(k, TRAIN_BATCH, lr, seed=1 + i))
+            steps += k
         t2 = time.perf_counter()
         leg["training"] = {"pairs": pairs, "sample_bodies": TRAIN_SAMPLE_BODIES,
-                           "adam_steps": args.train_steps + args.train_steps // 2,
-                           "batch": 4096, "loss": [l1, l2],
+                           "adam_steps": steps, "batch": TRAIN_BATCH, "loss": losses,
                            "generate_s": t1 - t0, "train_s": t2 - t1}
-        print(f"[bench] trained on {pairs} pairs: loss {l1:.3e} -> {l2:.3e} "
+        print(f"[bench] trained on {pairs} pairs: loss {losses[0]:.3e} -> {losses[-1]:.3e} "
               f"({t1 - t0:.2f} s + {t2 - t1:.2f} s)", flush=True)
     # Force error on the initial state (the same positions for every leg).
     t = ics.sample_targets(n, args.error_sample, seed=1)
@@ -180,6 +188,45 @@ def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
                                       "steps": 2 * args.steps + args.warmup}
     leg["interactions_per_body_after_steps"] = total2 / n
     return leg
+
+
+def plain_point(nbody2d, ics, mass, pos, vel, theta, args) -> dict:
+    """Plain group Barnes-Hut at one theta: RMS/max error on the initial state
+    and graph-replayed ms per step."""
+    abi = nbody2d.abi
+    n = len(mass)
+    with nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, theta=theta, softening=EPS,
+                         traversal=abi.TRAV_GROUP) as ctx:
+        t = ics.sample_targets(n, args.error_sample, seed=1)
+        ref = nbody2d.direct_sum_f64(pos, mass, 1.0, EPS, t)
+        acc, _, total = ctx.forces()
+        rms, mx = ics.force_errors(acc[t], ref)
+        ctx.step(args.warmup)
+        t0 = time.perf_counter()
+        ctx.step(args.steps)
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    print(f"[bench] plain theta={theta}: {ms:.3f} ms/step, rms {rms:.3e}", flush=True)
+    return {"theta": theta, "rms": rms, "max": mx, "ms_per_step": ms,
+            "interactions_per_body": total / n}
+
+
+def equal_error(points, learned) -> dict:
+    """Plain BH's ms per step at the learned leg's RMS error, interpolated
+    (linear in log error) between the two plain points that bracket it."""
+    pts = sorted(points, key=lambda p: p["rms"])
+    target = learned["force_error"]["rms"]
+    out = {"learned_rms": target, "learned_ms_per_step": learned["ms_per_step"],
+           "plain_points": points}
+    for lo, hi in zip(pts, pts[1:]):
+        if lo["rms"] <= target <= hi["rms"]:
+            w = (np.log(target) - np.log(lo["rms"])) / (np.log(hi["rms"]) - np.log(lo["rms"]))
+            ms = lo["ms_per_step"] + w * (hi["ms_per_step"] - lo["ms_per_step"])
+            th = lo["theta"] + w * (hi["theta"] - lo["theta"])
+            out.update({"plain_theta": th, "plain_ms_per_step": ms,
+                        "learned_over_plain": ms / learned["ms_per_step"]})
+            return out
+    out.update({"plain_theta": None, "plain_ms_per_step": None, "learned_over_plain": None})
+    return out
 
 
 def hbm_sort_build(leg, n: int) -> dict:
@@ -381,8 +428,8 @@ def run_ensemble(nbody2d, ics, args) -> dict:
     with nbody2d.Context(*systems[big], G=1.0, dt=DT, theta=theta, softening=EPS) as t:
         t.mlp_init(1)
         pairs = t.mlp_generate(theta, 8192, 1 << 22, seed=7)
-        t.mlp_train(args.train_steps, 4096, 2e-3, seed=1)
-        loss = t.mlp_train(args.train_steps // 2, 4096, 4e-4, seed=2)
+        t.mlp_train(4000, 4096, 2e-3, seed=1)
+        loss = t.mlp_train(2000, 4096, 4e-4, seed=2)
         w = t.mlp_weights()
     train_s = time.perf_counter() - t0
     kw = dict(G=1.0, dt=DT, theta=theta, softening=EPS, traversal=abi.TRAV_GROUP,
@@ -470,7 +517,10 @@ def main() -> int:
                     help="traversal of the plain leg (the learned leg needs group)")
     ap.add_argument("--theta-plain", type=float, default=THETA_PLAIN)
     ap.add_argument("--theta-learned", type=float, default=THETA_LEARNED)
-    ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--train-steps", type=int, default=20000,
+                    help="learned leg: Adam steps of the first training stage")
+    ap.add_argument("--no-equal-error", action="store_true",
+                    help="skip the plain legs that locate plain BH at the learned error")
     ap.add_argument("--workload", choices=("disk", "allpairs", "two_disks", "ensemble"),
                     default="disk",
                     help="disk: the N=1M plain-vs-learned headline; allpairs: N=262,144 "
@@ -552,6 +602,13 @@ def main() -> int:
         "roofline": roofline_walk(main_leg, main_leg["traversal"]),
         "hbm_sort_build": hbm_sort_build(main_leg, n),
     }
+    if len(legs) == 2 and not args.no_equal_error:
+        pts = [plain_point(nbody2d, ics, mass, pos, vel, th, args) for th in EQUAL_ERROR_THETAS]
+        pts.append({"theta": legs["plain"]["theta"], "rms": legs["plain"]["force_error"]["rms"],
+                    "max": legs["plain"]["force_error"]["max"],
+                    "ms_per_step": legs["plain"]["ms_per_step"],
+                    "interactions_per_body": legs["plain"]["interactions_per_body"]})
+        out["equal_error"] = equal_error(pts, legs["learned"])
     if len(legs) == 2:
         out["learned_over_plain"] = (legs["learned"]["body_steps_per_s"]
                                      / legs["plain"]["body_steps_per_s"])

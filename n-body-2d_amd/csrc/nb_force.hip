@@ -171,9 +171,15 @@ constexpr int kGList = 192;
 constexpr int kGFlush = kGList - 64;
 // Before its walk a group is split at its largest Morton jump when the two
 // parts are much more compact than the whole. As a safety net, a group whose
-// interaction list still passes `budget` entries (default kGroupBudget) is split into
-// pieces of 8 bodies (halves below that), shared with other wavefronts
-// through the work queue, and walked again, down to single bodies.
+// interaction list still passes `budget` entries (default kGroupBudget), or
+// whose walk takes more than kGMaxSteps steps, is split into pieces of 8
+// bodies (halves below that), shared with other wavefronts through the work
+// queue, and walked again, down to single bodies. A typical group walk takes
+// 15-25 steps; the long ones are groups spread over a sparse region (bodies
+// thrown out of a disk), whose wide box opens everything around it: caught
+// after kGMaxSteps steps instead of after a full budget of list entries (some
+// 440 steps), their pieces start while most wavefronts are still busy.
+constexpr uint32_t kGMaxSteps = 96;
 
 template <bool ZERO_EPS>
 __device__ __forceinline__ void eval_list(const float4* __restrict__ list, int cnt, float bx,
@@ -358,10 +364,11 @@ __global__ __launch_bounds__(64) void k_traverse_group(
             Acc t;
             int sp = 1, cnt = 0;
             uint32_t ccnt = 0;  // cells of this subgroup written after cbase
-            uint32_t listed = 0;
+            uint32_t listed = 0, steps = 0;
             bool own_multi = false, split = false;
             if (l == 0) s_stack[0] = root;  // the root of this group's system
             while (sp > 0) {
+                steps++;
                 const int take = max(1, min(min(64, sp), (kGStack - kGReserve - sp) / 3));
                 if (sp + 3 > kGStack) {  // unreachable: see kGReserve
                     overflow = true;
@@ -432,7 +439,9 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                         interact(t, ex, ey, fma(ex, ex, ey * ey), (float)mass[j], eps2f);
                     }
                 }
-                if (capped && listed + (uint32_t)cnt > (uint32_t)budget && nt + 8 <= 16) {
+                if (capped &&
+                    (listed + (uint32_t)cnt > (uint32_t)budget || (cn > 8 && steps > kGMaxSteps)) &&
+                    nt + 8 <= 16) {
                     split = true;
                     break;
                 }

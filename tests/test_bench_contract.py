@@ -52,3 +52,22 @@ def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, str(REPO / "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_equal_error_interpolation():
+    pts = [{"theta": 0.5, "rms": 1.4e-2, "ms_per_step": 1.4},
+           {"theta": 0.4, "rms": 9.0e-3, "ms_per_step": 1.7},
+           {"theta": 0.3, "rms": 5.0e-3, "ms_per_step": 2.1}]
+    learned = {"force_error": {"rms": 9.0e-3}, "ms_per_step": 1.7}
+    e = bench.equal_error(pts, learned)
+    assert e["plain_theta"] == pytest.approx(0.4) and e["plain_ms_per_step"] == pytest.approx(1.7)
+    assert e["learned_over_plain"] == pytest.approx(1.0)
+    # halfway in log error between 0.3 and 0.4
+    learned = {"force_error": {"rms": (5.0e-3 * 9.0e-3) ** 0.5}, "ms_per_step": 1.6}
+    e = bench.equal_error(pts, learned)
+    assert e["plain_theta"] == pytest.approx(0.35)
+    assert e["plain_ms_per_step"] == pytest.approx(1.9)
+    assert e["learned_over_plain"] == pytest.approx(1.9 / 1.6)
+    # outside the measured range: no claim
+    e = bench.equal_error(pts, {"force_error": {"rms": 1e-3}, "ms_per_step": 1.0})
+    assert e["learned_over_plain"] is None
