@@ -63,9 +63,10 @@ enum nb_far_field { NB_FAR_MONOPOLE = 0, NB_FAR_LEARNED = 1 };
  * when the test holds for the whole group's bounding box (a conservative
  * superset of every member's test), collects the accepted cells and leaf
  * bodies in an LDS interaction list, then every lane sums the whole list; a
- * group whose list grows past group_budget is split into pieces of 8 bodies
- * (halves below that), shared with other wavefronts and walked again, down to
- * single bodies, whose group test is exactly their own opening test.
+ * group whose list grows past group_budget (or whose walk runs unusually
+ * long) is split into pieces of 8 bodies, and a piece into single bodies,
+ * shared with other wavefronts and walked again; a single body's group test
+ * is exactly its own opening test.
  * GROUP applies more (never fewer) force terms per body than EXACT at the
  * same theta, so its force error is no larger. */
 enum nb_traversal { NB_TRAV_EXACT = 0, NB_TRAV_GROUP = 1 };

@@ -173,12 +173,13 @@ constexpr int kGFlush = kGList - 64;
 // parts are much more compact than the whole. As a safety net, a group whose
 // interaction list still passes `budget` entries (default kGroupBudget), or
 // whose walk takes more than kGMaxSteps steps, is split into pieces of 8
-// bodies (halves below that), shared with other wavefronts through the work
-// queue, and walked again, down to single bodies. A typical group walk takes
-// 15-25 steps; the long ones are groups spread over a sparse region (bodies
-// thrown out of a disk), whose wide box opens everything around it: caught
-// after kGMaxSteps steps instead of after a full budget of list entries (some
-// 440 steps), their pieces start while most wavefronts are still busy.
+// bodies (a piece of at most 8 into single bodies), shared with other
+// wavefronts through the work queue, and walked again. A typical group walk takes
+// 15-25 steps; the long ones are groups of bodies thrown across the disk,
+// whose wide box reaches into dense regions and opens everything there
+// (lists of 16k entries after some 440 steps, for 8 bodies). Caught after
+// kGMaxSteps steps, their pieces start while most wavefronts are still busy.
+// (A limit of 48 made ordinary walks split too, in cascades.)
 constexpr uint32_t kGMaxSteps = 96;
 
 template <bool ZERO_EPS>
@@ -201,7 +202,7 @@ __device__ __forceinline__ void eval_list(const float4* __restrict__ list, int c
 }
 
 // Work sharing: a subgroup whose list passes the budget is split into pieces
-// of 8 bodies (halves below that); all pieces but one go to a queue in device memory (meta[META_QHEAD], meta[META_QTAIL]
+// of 8 bodies (single bodies below that); all pieces but one go to a queue in device memory (meta[META_QHEAD], meta[META_QTAIL]
 // count pushes and pops; an entry is {group, first lane << 8 | lanes} + 1 in
 // the high word so that 0 means "not yet written"), and a wavefront that has
 // finished its own group takes queued subgroups before it exits. Every
@@ -439,8 +440,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                         interact(t, ex, ey, fma(ex, ex, ey * ey), (float)mass[j], eps2f);
                     }
                 }
-                if (capped &&
-                    (listed + (uint32_t)cnt > (uint32_t)budget || (cn > 8 && steps > kGMaxSteps)) &&
+                if (capped && (listed + (uint32_t)cnt > (uint32_t)budget || steps > kGMaxSteps) &&
                     nt + 8 <= 16) {
                     split = true;
                     break;
@@ -453,10 +453,10 @@ __global__ __launch_bounds__(64) void k_traverse_group(
             }
             if (overflow) break;
             if (split) {
-                // Pieces of 8 bodies (halves below that): all but the first go
+                // Pieces of 8 bodies (single bodies below that): all but the first go
                 // to the shared queue (or stay here when it is full), so the
                 // pieces of a heavy group are walked by several wavefronts.
-                const int pieces = cn > 8 ? (cn + 7) / 8 : 2;
+                const int pieces = cn > 8 ? (cn + 7) / 8 : cn;
                 const int len = (cn + pieces - 1) / pieces;
                 for (int q = pieces - 1; q >= 1; q--) {
                     const int pf = f + q * len, pc = min(len, f + cn - pf);
