@@ -442,6 +442,11 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
         }                                                                                  \
     } while (0)
     NB_TRY(hipGetDevice(&c->device));
+    {
+        int cus = 0;
+        NB_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        if (cus > 0) c->cus = (uint32_t)cus;
+    }
     NB_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) NB_TRY(hipEventCreate(&e));
     NB_TRY(hipHostMalloc((void**)&c->host_meta, META_COUNT * sizeof(uint32_t)));

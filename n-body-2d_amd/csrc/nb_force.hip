@@ -174,13 +174,20 @@ constexpr int kGFlush = kGList - 64;
 // interaction list still passes `budget` entries (default kGroupBudget), or
 // whose walk takes more than kGMaxSteps steps, is split into pieces of 8
 // bodies (a piece of at most 8 into single bodies), shared with other
-// wavefronts through the work queue, and walked again. A typical group walk takes
-// 15-25 steps; the long ones are groups of bodies thrown across the disk,
-// whose wide box reaches into dense regions and opens everything there
-// (lists of 16k entries after some 440 steps, for 8 bodies). Caught after
-// kGMaxSteps steps, their pieces start while most wavefronts are still busy.
-// (A limit of 48 made ordinary walks split too, in cascades.)
-constexpr uint32_t kGMaxSteps = 96;
+// wavefronts through the work queue, and walked again. A typical group walk
+// takes 15-25 steps; the long ones (some 440 steps, thousands of entries)
+// are groups whose wide box reaches into dense regions, e.g. bodies thrown
+// across a disk. Such a walk only matters when it outlasts the launch, so the
+// step limit is what the launch's resident wavefronts get through anyway:
+// kGWalkSteps per round of walks, at least kGMinSteps. On a 1M-body disk
+// (5 rounds) that is 96 steps and the split pieces start while most
+// wavefronts are still busy; on a 5.6M-body ensemble (26 rounds) the long
+// walks hide behind the others, and splitting them, each piece as long
+// again, made the launch 18x slower. (A flat limit of 48 split ordinary walks
+// too, in cascades.)
+constexpr uint32_t kGMinSteps = 96;
+constexpr uint32_t kGWalkSteps = 20;
+constexpr uint32_t kGWavesPerCU = 13;  // LDS-bound residency of k_traverse_group
 
 template <bool ZERO_EPS>
 __device__ __forceinline__ void eval_list(const float4* __restrict__ list, int cnt, float bx,
@@ -265,7 +272,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
     const double* __restrict__ mass, double2* __restrict__ vel, double2* __restrict__ acc_out,
     uint32_t* __restrict__ inter_out, uint32_t n, const StepConsts* __restrict__ k,
-    uint32_t* __restrict__ meta, int count_step, int budget,
+    uint32_t* __restrict__ meta, int count_step, int budget, uint32_t max_steps,
     const uint64_t* __restrict__ keys, uint32_t* __restrict__ cell_slot,
     uint32_t* __restrict__ cell_cnt, uint32_t cell_cap, SubRec* __restrict__ subs,
     float4* __restrict__ body_w, uint64_t* __restrict__ queue, uint32_t queue_cap,
@@ -440,7 +447,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                         interact(t, ex, ey, fma(ex, ex, ey * ey), (float)mass[j], eps2f);
                     }
                 }
-                if (capped && (listed + (uint32_t)cnt > (uint32_t)budget || steps > kGMaxSteps) &&
+                if (capped && (listed + (uint32_t)cnt > (uint32_t)budget || steps > max_steps) &&
                     nt + 8 <= 16) {
                     split = true;
                     break;
@@ -538,13 +545,18 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
         const bool z = !(c->p.softening > 0.0);
         const bool gd = c->p.self_guard != 0;
         const int budget = c->p.group_budget > 0 ? c->p.group_budget : kGroupBudget;
+        const uint64_t rounds = ((uint64_t)groups + (uint64_t)c->cus * kGWavesPerCU - 1) /
+                                ((uint64_t)c->cus * kGWavesPerCU);
+        const uint32_t max_steps =
+            (uint32_t)std::max<uint64_t>(kGMinSteps, std::min<uint64_t>(rounds * kGWalkSteps, 1u << 30));
         const bool learn = c->p.far_field == NB_FAR_LEARNED;
 #define NB_TRAVG(K, Z, GD, LN)                                                                 \
     hipLaunchKernelGGL((k_traverse_group<K, Z, GD, LN>), dim3(groups), dim3(64), 0, c->stream, \
                        c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src], c->b.acc,     \
                        c->b.inter, n, c->b.consts, c->b.meta, (int)(count_step && !learn),     \
-                       budget, c->b.keys[0], c->b.cell_slot, c->b.cell_cnt, c->cell_cap,       \
-                       c->b.subs, c->b.body_w, c->b.queue, c->queue_cap, c->b.grp_first,       \
+                       budget, max_steps, c->b.keys[0], c->b.cell_slot, c->b.cell_cnt,         \
+                       c->cell_cap, c->b.subs, c->b.body_w, c->b.queue, c->queue_cap,          \
+                       c->b.grp_first,                                                          \
                        c->b.grp_info)
 #define NB_TRAVG3(K, Z, GD) if (learn) NB_TRAVG(false, Z, GD, true); else NB_TRAVG(K, Z, GD, false)
 #define NB_TRAVG2(K, Z) if (gd) { NB_TRAVG3(K, Z, true); } else { NB_TRAVG3(K, Z, false); }

@@ -155,6 +155,7 @@ struct nb_ctx {
     uint32_t nsys = 1;           // independent systems (contiguous body ranges)
     uint32_t ngroups = 0;        // traversal groups: <= 64 bodies, inside one system
     int device = 0;
+    uint32_t cus = 256;          // compute units of the device
     hipStream_t stream = nullptr;
     nb::Buffers b{};
     int cur = 0;                 // which copy holds the state
