@@ -415,10 +415,13 @@ __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __rest
     const uint32_t e = L + 1 < kMaxLevel ? lvl_off[(uint64_t)(L + 1) * nblk] : I;
     for (uint32_t k = b + blockIdx.x * 256 + threadIdx.x; k < e; k += gridDim.x * 256) {
         const uint32_t p = by_level[k];
+        const uint32_t mask = inode_mask[p];
+        // an empty slot holds exact zeros: skip its load, combine() adds the same zeros
         Com ch[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) ch[q] = load_com(&nodes[nsys + 4ull * p + q]);
-        store_com(&nodes[inode_slot[p]], combine(ch), inode_mask[p]);
+        for (int q = 0; q < 4; q++)
+            ch[q] = (mask >> q) & 1u ? load_com(&nodes[nsys + 4ull * p + q]) : Com{0.0, 0.0, 0.0, 0.f, 0.f, 0.f};
+        store_com(&nodes[inode_slot[p]], combine(ch), mask);
     }
 }
 
