@@ -86,7 +86,9 @@ typedef struct nb_params {
     int32_t use_graph;  /* 1: steps replay a captured hipGraph (default) */
     int32_t traversal;  /* enum nb_traversal */
     int32_t group_budget; /* GROUP: interaction-list entries after which a group
-                             is split into pieces and walked again (0 = 16384) */
+                             is split into pieces and walked again (0 = 16384);
+                             for theta < 0.5 it is scaled by 0.5/theta, as
+                             lists grow */
     int32_t cell_capacity; /* LEARNED: initial accepted-cell entries per group of
                               64 bodies (0 = 4096); a step that needs more
                               halts, the region grows and the step runs again */

@@ -184,7 +184,13 @@ constexpr int kGFlush = kGList - 64;
 // wavefronts are still busy; on a 5.6M-body ensemble (26 rounds) the long
 // walks hide behind the others, and splitting them, each piece as long
 // again, made the launch 18x slower. (A flat limit of 48 split ordinary walks
-// too, in cascades.)
+// too, in cascades.) Both this limit and the list budget hold for theta >=
+// 0.5; below it walks and lists grow (the accepted cells of a 2-D walk go as
+// 1/theta^2), so both limits grow by 0.5/theta -- without that nearly every
+// group splits, and the pieces, each walked again from the root, split again
+// into single bodies that use one lane of their wavefront (a 1M disk at
+// theta 0.15: 949 ms per step instead of 4.2). Growing them by (0.5/theta)^2
+// instead left more stragglers unsplit (theta 0.3: 1.91 vs 1.76 ms).
 constexpr uint32_t kGMinSteps = 96;
 constexpr uint32_t kGWalkSteps = 20;
 constexpr uint32_t kGWavesPerCU = 13;  // LDS-bound residency of k_traverse_group
@@ -544,11 +550,14 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
         const uint32_t groups = c->ngroups;
         const bool z = !(c->p.softening > 0.0);
         const bool gd = c->p.self_guard != 0;
-        const int budget = c->p.group_budget > 0 ? c->p.group_budget : kGroupBudget;
+        const double th = c->p.theta;
+        const double grow = th >= 0.5 ? 1.0 : th > 1e-3 ? 0.5 / th : 1e6;
+        const int budget = (int)std::min(
+            (double)(c->p.group_budget > 0 ? c->p.group_budget : kGroupBudget) * grow, 1e9);
         const uint64_t rounds = ((uint64_t)groups + (uint64_t)c->cus * kGWavesPerCU - 1) /
                                 ((uint64_t)c->cus * kGWavesPerCU);
-        const uint32_t max_steps =
-            (uint32_t)std::max<uint64_t>(kGMinSteps, std::min<uint64_t>(rounds * kGWalkSteps, 1u << 30));
+        const uint32_t max_steps = (uint32_t)std::min(
+            std::max((double)kGMinSteps * grow, (double)(rounds * kGWalkSteps)), 1e9);
         const bool learn = c->p.far_field == NB_FAR_LEARNED;
 #define NB_TRAVG(K, Z, GD, LN)                                                                 \
     hipLaunchKernelGGL((k_traverse_group<K, Z, GD, LN>), dim3(groups), dim3(64), 0, c->stream, \

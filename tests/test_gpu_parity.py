@@ -332,3 +332,28 @@ def test_group_traversal_large_disk_error_and_guard():
     print(errs)
     assert errs[("group", 0.5, 0)] <= errs[("exact", 0.5, 0)]
     assert errs[("group", 0.8, 1)] <= errs[("group", 0.8, 0)] * 1.0001
+
+
+def test_group_traversal_small_theta_keeps_groups():
+    """At small theta every group's walk and list are long; the split limits
+    grow with 1/theta so ordinary groups are not split into single-body
+    walks (which once made a theta = 0.15 step ~1000x slower than theta =
+    0.5 on a disk). Forces stay within the opening test's error."""
+    m, p, v = ics.exp_disk(1 << 18, seed=3)
+    t = ics.sample_targets(len(m), 1024, seed=5)
+    ref = nbody2d.direct_sum_f64(p, m, 1.0, 0.01, t)
+    ms = {}
+    for theta in (0.5, 0.15):
+        with _ctx(m, p, v, theta=theta, softening=0.01, traversal=nbody2d.abi.TRAV_GROUP) as c:
+            acc, _, _ = c.forces()
+            c.step(1)
+            c.set_timing(True)  # eager steps with hipEvents around each phase
+            c.step(2)
+            ph, steps = c.timing()
+            ms[theta] = ph["force"] / steps
+        rms, mx = ics.force_errors(acc[t], ref)
+        print(f"[gpu] N=262144 theta={theta} force {ms[theta]:.3f} ms rms={rms:.3e} max={mx:.3e}")
+        assert rms <= (2.5e-2 if theta == 0.5 else 3e-3)
+    # a 2-D walk's terms grow as 1/theta^2: about 11x here, and the walk's
+    # time by less (fixed per-step latency)
+    assert ms[0.15] < 15.0 * ms[0.5], ms
