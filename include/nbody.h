@@ -90,8 +90,10 @@ typedef struct nb_params {
                              for theta < 0.5 it is scaled by 0.5/theta, as
                              lists grow */
     int32_t cell_capacity; /* LEARNED: initial accepted-cell entries per group of
-                              64 bodies (0 = 4096); a step that needs more
-                              halts, the region grows and the step runs again */
+                              64 bodies (0 = twice the busiest group of a first
+                              walk at creation, at least 4096); a step that
+                              needs more halts, the region grows and the step
+                              runs again */
     int32_t reserved[4];
 } nb_params;
 

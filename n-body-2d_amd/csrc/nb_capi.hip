@@ -548,10 +548,20 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
     if (bh && n) {
         rc = alloc_tree(c, std::max<uint64_t>(n, 1024));
         if (rc) return bail(rc);
-        rc = build_and_force(c, false);  // sizes the node pool to the first tree
+        // Sizes the node pool to the first tree and, with the learned far
+        // field, the accepted-cell regions to twice the first walk's busiest
+        // group (the cells a walk accepts do not depend on the weights), so
+        // an evolving system seldom halts a step to grow them.
+        const bool learned = c->p.far_field == NB_FAR_LEARNED;
+        rc = build_and_force(c, learned);
         if (rc) return bail(rc);
         if (c->last_internal + c->last_internal / 4 > c->icap) {
             rc = alloc_tree(c, c->last_internal + c->last_internal / 4 + 1024);
+            if (rc) return bail(rc);
+        }
+        const uint64_t cells = 2ull * c->host_meta[META_CELLS_NEEDED] + 64;
+        if (learned && c->p.cell_capacity == 0 && cells > c->cell_cap && cells <= 0xffffffffull) {
+            rc = alloc_cells(c, (uint32_t)cells);
             if (rc) return bail(rc);
         }
     }

@@ -536,10 +536,8 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     if (overflow && l == 0) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_STACK);
     if (LEARN && l == 0) {
         atomicAdd((unsigned long long*)&meta[META_CELLS_TOTAL], (unsigned long long)cbase);
-        if (cbase > cell_cap) {
-            atomicMax(&meta[META_CELLS_NEEDED], cbase);
-            atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_CELL_CAPACITY);
-        }
+        atomicMax(&meta[META_CELLS_NEEDED], cbase);  // sizes the regions at creation
+        if (cbase > cell_cap) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_CELL_CAPACITY);
     }
 }
 

@@ -186,7 +186,10 @@ __global__ __launch_bounds__(256) void k_tree_enum(const int8_t* __restrict__ de
                                                    NodeRec* __restrict__ nodes) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const uint32_t I = meta[META_INTERNAL];
-    if (I > icap || (meta[META_FLAGS] & FLAG_NONFINITE)) {
+    // A step after one that halted (any halt flag) only carries the state
+    // along: its tree is not built. The node count stays a real one (the
+    // keys are still sorted), so it can size the node pool.
+    if (I > icap || (meta[META_FLAGS] & kHaltFlags)) {
         if (i == 0 && I > icap) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_TREE_CAPACITY);
         return;
     }
@@ -229,7 +232,7 @@ __global__ __launch_bounds__(256) void k_tree_nodes(
     uint32_t* __restrict__ inode_slot, const uint8_t* __restrict__ inode_level,
     uint8_t* __restrict__ inode_mask, uint32_t* __restrict__ multi_list) {
     const uint32_t I = meta[META_INTERNAL];
-    if (I > icap || (meta[META_FLAGS] & FLAG_NONFINITE)) return;
+    if (I > icap || (meta[META_FLAGS] & kHaltFlags)) return;
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     if (p >= I) return;
     build_node(p, keys, nsys, sys_of, sys_off, pos, mass, delta, off, box, node_a, nodes,
@@ -360,8 +363,7 @@ __device__ __forceinline__ void store_com(NodeRec* __restrict__ r, const Com& o,
 }
 
 __device__ __forceinline__ bool tree_halted(const uint32_t* meta, uint64_t icap) {
-    return meta[META_INTERNAL] > icap ||
-           (meta[META_FLAGS] & (FLAG_TREE_CAPACITY | FLAG_NONFINITE));
+    return meta[META_INTERNAL] > icap || (meta[META_FLAGS] & kHaltFlags);
 }
 
 // Counting sort of the internal nodes by level, so each level of the
