@@ -61,8 +61,10 @@ TRAIN_PAIRS = 1 << 24
 TRAIN_SAMPLE_BODIES = 65536
 TRAIN_BATCH = 16384
 TRAIN_SCHEDULE = ((1.0, 2e-3), (0.5, 5e-4), (0.5, 1e-4))
-# Plain legs timed to find the plain theta at the learned leg's error.
-EQUAL_ERROR_THETAS = (0.4, 0.35, 0.3, 0.25)
+# Plain legs timed to find the plain theta at a learned leg's error.
+EQUAL_ERROR_THETAS = (0.4, 0.35, 0.3, 0.25, 0.2, 0.15, 0.12, 0.1)
+# A second learned point, at the error that the 1e-3 target asks for.
+THETA_LEARNED_LOW = 0.5
 METRIC = "N=1M body-steps/s: plain BH vs learned far-field BH at equal force error"
 
 
@@ -207,6 +209,36 @@ def plain_point(nbody2d, ics, mass, pos, vel, theta, args) -> dict:
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
     print(f"[bench] plain theta={theta}: {ms:.3f} ms/step, rms {rms:.3e}", flush=True)
     return {"theta": theta, "rms": rms, "max": mx, "ms_per_step": ms,
+            "interactions_per_body": total / n}
+
+
+def learned_point(nbody2d, ics, mass, pos, vel, theta, train, args) -> dict:
+    """Learned far field at one theta, trained as the learned leg (or with
+    zero weights: the softened quadrupole term alone): RMS/max error on the
+    initial state and graph-replayed ms per step."""
+    abi = nbody2d.abi
+    n = len(mass)
+    guard = 1 if theta > 0.7071 else 0
+    with nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, theta=theta, softening=EPS,
+                         traversal=abi.TRAV_GROUP, self_guard=guard,
+                         far_field=abi.FAR_LEARNED) as ctx:
+        if train:
+            ctx.mlp_init(1)
+            ctx.mlp_generate(theta, TRAIN_SAMPLE_BODIES, TRAIN_PAIRS, seed=7)
+            for i, (frac, lr) in enumerate(TRAIN_SCHEDULE):
+                ctx.mlp_train(max(1, int(args.train_steps * frac)), TRAIN_BATCH, lr, seed=1 + i)
+        t = ics.sample_targets(n, args.error_sample, seed=1)
+        ref = nbody2d.direct_sum_f64(pos, mass, 1.0, EPS, t)
+        acc, _, total = ctx.forces()
+        rms, mx = ics.force_errors(acc[t], ref)
+        ctx.step(args.warmup)
+        t0 = time.perf_counter()
+        ctx.step(args.steps)
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    what = "trained network" if train else "zero weights (quadrupole term only)"
+    print(f"[bench] learned theta={theta}, {what}: {ms:.3f} ms/step, rms {rms:.3e}", flush=True)
+    return {"theta": theta, "network": "trained" if train else "zero", "rms": rms, "max": mx,
+            "ms_per_step": ms, "force_error": {"rms": rms, "max": mx},
             "interactions_per_body": total / n}
 
 
@@ -609,6 +641,14 @@ def main() -> int:
                     "ms_per_step": legs["plain"]["ms_per_step"],
                     "interactions_per_body": legs["plain"]["interactions_per_body"]})
         out["equal_error"] = equal_error(pts, legs["learned"])
+        # the learned far field at lower theta, and what its network adds over
+        # the quadrupole term alone (zero weights) at both thetas
+        low = learned_point(nbody2d, ics, mass, pos, vel, THETA_LEARNED_LOW, True, args)
+        out["equal_error_low_theta"] = equal_error(pts, low)
+        out["equal_error_low_theta"]["learned_theta"] = THETA_LEARNED_LOW
+        out["quadrupole_only"] = [
+            learned_point(nbody2d, ics, mass, pos, vel, th, False, args)
+            for th in (legs["learned"]["theta"], THETA_LEARNED_LOW)]
     if len(legs) == 2:
         out["learned_over_plain"] = (legs["learned"]["body_steps_per_s"]
                                      / legs["plain"]["body_steps_per_s"])

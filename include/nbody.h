@@ -90,8 +90,8 @@ typedef struct nb_params {
                              for theta < 0.5 it is scaled by 0.5/theta, as
                              lists grow */
     int32_t cell_capacity; /* LEARNED: initial accepted-cell entries per group of
-                              64 bodies (0 = twice the busiest group of a first
-                              walk at creation, at least 4096); a step that
+                              64 bodies (0 = four times the busiest group of a
+                              first walk at creation, at least 4096); a step that
                               needs more halts, the region grows and the step
                               runs again */
     int32_t reserved[4];
@@ -251,13 +251,15 @@ int nb_autotune(nb_ctx* ctx, uint32_t repeats, nb_tuning* out);
 /* ---------------------------------------------------------------------------
  * Learned far-field correction. No counterpart in the reference: its force
  * is the monopole of every accepted cell (BarnesHutCuda.cu:461-471). The
- * network (8 -> 64 -> 64 -> 2, ReLU, BF16 weights and activations on MFMA,
- * FP32 accumulation) predicts a_exact - a_monopole of an accepted cell from
- * its offset, size/distance, mass fraction and second moments; a context with
- * far_field = NB_FAR_LEARNED adds it for every accepted internal cell.
+ * correction of an accepted cell, a_exact - a_monopole, is approximated by
+ * the cell's softened quadrupole term, computed from its second moments, plus
+ * a network (8 -> 64 -> 64 -> 2, ReLU, BF16 weights and activations on MFMA,
+ * FP32 accumulation) trained on the rest, from the cell's offset,
+ * size/distance, mass fraction and second moments; a context with far_field
+ * = NB_FAR_LEARNED adds it for every accepted internal cell.
  * Parameters are FP32 in the order W1[64][8], b1[64], W2[64][64], b2[64],
- * W3[2][64], b3[2] (NB_MLP_PARAMS values); a new context holds zeros (no
- * correction).
+ * W3[2][64], b3[2] (NB_MLP_PARAMS values); a new context holds zeros (the
+ * quadrupole term alone).
  * ------------------------------------------------------------------------- */
 #define NB_MLP_PARAMS 4866
 #define NB_MLP_INPUTS 8
@@ -273,8 +275,9 @@ int nb_mlp_init(nb_ctx* ctx, uint64_t seed);
  * test at `theta` (cells holding the body are never accepted), and keeps
  * every accepted internal cell of at most 8,192 bodies, up to `max_pairs`.
  * Each pair gets its 8 inputs and its target, the exact FP64 Plummer
- * acceleration of the cell's bodies minus the cell's monopole, in the
- * network's output units. *pairs_out = pairs held. */
+ * acceleration of the cell's bodies minus the cell's monopole and minus its
+ * softened quadrupole term, in the network's output units.
+ * *pairs_out = pairs held. */
 int nb_mlp_generate(nb_ctx* ctx, double theta, uint32_t sample_bodies, uint64_t max_pairs,
                     uint64_t seed, uint64_t* pairs_out);
 

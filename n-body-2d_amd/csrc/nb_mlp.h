@@ -25,12 +25,13 @@
 // second moments about the COM, taken in the frame (u, w):
 //   f0,f1 = u   f2 = s/R   f3 = log2(M/M_root)/16
 //   f4,f5,f6 = (u.Iu, u.Iw, w.Iw) / (M s^2)   f7 = log2(r/ell)/8
-// with ell = eps (or the root cell side when eps = 0). Output (y0, y1): the
-// correction a_exact - a_monopole of the Plummer kernel along (u, w), in units
-// of M s^2 r / R^5 (the scale of the softened quadrupole term). In these
-// units the quadrupole term is (3/2)(-f4 - f6 - 2 f4 + 5 rho f4, -2 f5) with
-// rho = r^2/R^2 = 1/(1 + 2^(-16 f7)) when ell = eps: linear in the moment
-// inputs, so the network mostly learns the higher orders.
+// with ell = eps (or the root cell side when eps = 0). Units of the
+// correction a_exact - a_monopole of the Plummer kernel along (u, w):
+// M s^2 r / R^5, the scale of the softened quadrupole term, which in these
+// units is q = (3/2)(5 rho f4 - 3 f4 - f6, -2 f5), rho = r^2/R^2. The
+// correction applied is q + (y0, y1): q is computed exactly and the network
+// output (y0, y1) is trained on the rest (octupole and higher orders, which
+// the second moments only hint at).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -129,7 +130,7 @@ __device__ inline bf16x8 pack8_relu(f32x4 a, f32x4 b) {
 // unit offset (ux, uy) that rotates the output back.
 __device__ inline float mlp_features(float dx, float dy, float s, float M, float ixx, float ixy,
                                      float iyy, float eps2, float inv_mroot, float inv_ell,
-                                     float f[8], float& ux, float& uy) {
+                                     float f[8], float& ux, float& uy, float q[2]) {
     const float r2 = fmaf(dx, dx, dy * dy);
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * inv_r;
@@ -152,6 +153,10 @@ __device__ inline float mlp_features(float dx, float dy, float s, float M, float
     f[6] = wiw * k;
     f[7] = __log2f(r * inv_ell) * 0.125f;
     const float iR2 = inv_R * inv_R;
+    // the softened quadrupole term in these units
+    const float rho = r2 * iR2;
+    q[0] = 1.5f * fmaf(5.f * rho, f[4], -fmaf(3.f, f[4], f[6]));
+    q[1] = -3.f * f[5];
     return ms2 * r * (iR2 * iR2 * inv_R);
 }
 

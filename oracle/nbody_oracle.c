@@ -30,6 +30,8 @@
  *   or_quadrupole_jacobian    central differences of or_quadrupole
  *   or_cell_correction        brute-force pair sums; zero for one body
  *   or_mlp_features           hand-computed values
+ *   or_mlp_quad_units         or_quadrupole taken along (u, w) over the
+ *                             output scale
  *   or_mlp_forward            a network built to compute a known linear map;
  *                             bf16 rounding against bit-level rounding
  *   or_mlp_grad               central differences of or_mlp_forward's loss
@@ -1107,6 +1109,16 @@ double or_mlp_features(double dx, double dy, double s, double M, double ixx, dou
     f[6] = (wx * (ixx * wx + ixy * wy) + wy * (ixy * wx + iyy * wy)) / ms2;
     f[7] = log2(r / ell) / 8.0;
     return ms2 * r / (R * R * R * R * R);
+}
+
+/* The softened quadrupole term (or_quadrupole) in the network's output
+ * units and frame, from the inputs: along (u, w) over M s^2 r / R^5,
+ *   q = (3/2) (5 rho f4 - 3 f4 - f6, -2 f5),   rho = r^2 / R^2.
+ * The learned correction is this term plus the network's output; the
+ * network is trained on the rest of a_exact - a_monopole. */
+void or_mlp_quad_units(const double* f, double rho, double* q) {
+    q[0] = 1.5 * (5.0 * rho * f[4] - 3.0 * f[4] - f[6]);
+    q[1] = -3.0 * f[5];
 }
 
 /* Exact Plummer acceleration of bodies [bs, be] at (x, y) minus the monopole

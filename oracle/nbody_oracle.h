@@ -144,6 +144,9 @@ void or_quadrupole_jacobian(double dx, double dy, double ixx, double ixy, double
 /* Network inputs f[8] of a (cell, point) pair; returns the output scale. */
 double or_mlp_features(double dx, double dy, double s, double M, double ixx, double ixy,
                        double iyy, double eps2, double m_root, double ell, double* f);
+/* The softened quadrupole term along (u, w) in output units, from the inputs
+ * f[8] and rho = r^2 / R^2. */
+void or_mlp_quad_units(const double* f, double rho, double* q);
 /* Exact minus monopole acceleration of bodies [bs, be] at (x, y). */
 void or_cell_correction(const double* pos, const double* mass, int64_t bs, int64_t be, double x,
                         double y, double eps2, double* out);

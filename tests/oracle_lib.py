@@ -62,6 +62,7 @@ def load(fast: bool = False) -> C.CDLL:
     L.or_mlp_features.restype = d
     L.or_cell_correction.argtypes = [_P, _P, i64, i64, d, d, d, _P]
     L.or_quadrupole_jacobian.argtypes = [d, d, d, d, d, d, _P]
+    L.or_mlp_quad_units.argtypes = [_P, d, _P]
     _libs[name] = L
     return L
 
@@ -244,6 +245,14 @@ def mlp_features(d, s, M, I, eps2, m_root, ell):
                                    float(I[1]), float(I[2]), float(eps2), float(m_root),
                                    float(ell), ptr(f))
     return f, scale
+
+
+def mlp_quad_units(f, rho):
+    """The softened quadrupole term along (u, w) in the network's units."""
+    fs = c64(f)
+    q = np.empty(2)
+    load().or_mlp_quad_units(ptr(fs), float(rho), ptr(q))
+    return q
 
 
 def cell_correction(pos, mass, bs, be, x, eps2):

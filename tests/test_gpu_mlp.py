@@ -8,15 +8,18 @@
   gradient  the training kernels' minibatch gradient against the oracle's
             back-propagation with the same rounding
   pairs     generated (cell, body) pairs: opening test, exclusion of the
-            body's own cells, inputs and exact-minus-monopole targets against
-            the oracle's FP64 restatement on the same tree
+            body's own cells, inputs and targets (exact minus monopole minus
+            the quadrupole term) against the oracle's FP64 restatement on the
+            same tree
   step      per body, learned minus monopole equals the oracle's correction
-            (network at the subgroup centre, quadrupole Jacobian across the
-            subgroup) on the walk's own cell lists; growing the accepted-cell
-            regions mid-step changes nothing; graph and eager steps agree
+            (quadrupole term plus network at the subgroup centre, quadrupole
+            Jacobian across the subgroup) on the walk's own cell lists;
+            growing the accepted-cell regions mid-step changes nothing; graph
+            and eager steps agree
   gain      a network trained on one disk lowers the force error of another
             disk at theta = 0.8 against the FP64 direct sum, well below the
-            monopole error at the same theta
+            monopole error at the same theta (the quadrupole term alone, zero
+            weights, is reported beside it)
 """
 import numpy as np
 import pytest
@@ -127,7 +130,10 @@ def test_training_pairs_match_oracle():
         np.testing.assert_allclose(x[q], f, rtol=2e-5, atol=2e-6)
         corr = O.cell_correction(sp, sm, n["bstart"], n["bend"], sp[body[q]], EPS * EPS)
         u = np.array([f[0], f[1]])
-        want = np.array([corr @ u, corr @ np.array([-u[1], u[0]])]) / scale
+        rho = (d[q] @ d[q]) / (d[q] @ d[q] + eps2)
+        # the target leaves out the quadrupole term, which the apply adds exactly
+        want = (np.array([corr @ u, corr @ np.array([-u[1], u[0]])]) / scale
+                - O.mlp_quad_units(f, rho))
         np.testing.assert_allclose(t[q], want, rtol=1e-4,
                                    atol=1e-5 * max(1.0, np.abs(want).max()))
 
@@ -171,13 +177,17 @@ def test_trained_network_lowers_force_error():
     ref = O.direct_sum(p, m, EPS * EPS, tg)
     kw = dict(theta=theta, softening=EPS, traversal=TRAV_GROUP, self_guard=1)
     with _ctx(m, p, v, far_field=FAR_MONOPOLE, **kw) as a, \
-            _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as b:
+            _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as b, \
+            _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as z:
         b.mlp_set_weights(w)
         fa, _, _ = a.forces()
         fb, _, _ = b.forces()
+        fz, _, _ = z.forces()  # zero weights: the quadrupole term alone
     ea = ics.force_errors(fa[tg], ref)
     eb = ics.force_errors(fb[tg], ref)
+    ez = ics.force_errors(fz[tg], ref)
     print(f"[gpu] theta {theta}: monopole rms {ea[0]:.3e} max {ea[1]:.3e}; "
+          f"quadrupole rms {ez[0]:.3e} max {ez[1]:.3e}; "
           f"learned rms {eb[0]:.3e} max {eb[1]:.3e}")
     assert eb[0] < 0.5 * ea[0]
 
@@ -199,9 +209,9 @@ def test_learned_graph_and_eager_steps_agree():
 
 def test_far_field_correction_matches_oracle():
     """Per body, learned minus monopole equals the oracle's sum over the
-    subgroup's accepted cells of the network's correction at the subgroup
-    centre, minus the summed softened-quadrupole Jacobians times the body's
-    offset from that centre."""
+    subgroup's accepted cells of the correction at the subgroup centre (the
+    softened quadrupole term plus the network's output), minus the summed
+    softened-quadrupole Jacobians times the body's offset from that centre."""
     m, p, v = ics.exp_disk(4096, seed=12)
     w = (np.random.default_rng(3).normal(size=4866) * 0.1).astype(np.float32)
     kw = dict(theta=0.8, softening=EPS, traversal=TRAV_GROUP, self_guard=1)
@@ -224,7 +234,7 @@ def test_far_field_correction_matches_oracle():
             st, ct = rng_[g, k]
             nd = nodes[slots.ravel()[st:st + ct]]
             assert np.all(nd["c0"] >= 1)
-            X, S, U = [], [], []
+            X, S, U, Q = [], [], [], []
             K = np.zeros((2, 2))
             for c in nd:
                 d = np.array([c["cx"], c["cy"]]) - centre[g, k]
@@ -234,11 +244,13 @@ def test_far_field_correction_matches_oracle():
                 X.append(f)
                 S.append(sc)
                 U.append(d / np.linalg.norm(d))
+                Q.append(O.mlp_quad_units(f, (d @ d) / (d @ d + eps2)))
                 K += O.quadrupole_jacobian(d, I, eps2)
             C = np.zeros(2)
             if ct:
                 y = O.mlp_forward(w.astype(np.float64), np.array(X, dtype=np.float32))
-                for yy, sc, u in zip(y.astype(np.float64), S, U):
+                for yy, sc, u, qq in zip(y.astype(np.float64), S, U, Q):
+                    yy = yy + qq
                     C += sc * (yy[0] * u + yy[1] * np.array([-u[1], u[0]]))
             members = np.nonzero((bsub == k) & (np.arange(len(bsub)) // 64 == g))[0]
             for i in members:

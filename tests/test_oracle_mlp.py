@@ -85,7 +85,8 @@ def test_mlp_features_hand_values():
 @pytest.mark.parametrize("eps2", [0.0, 0.3])
 def test_quadrupole_in_network_units_is_linear_in_moment_inputs(eps2):
     """In the network's units the quadrupole term is
-    (3/2)(-f4 - f6 - 2 f4 + 5 rho f4, -2 f5) along (u, w), rho = r^2 / R^2."""
+    (3/2)(-f4 - f6 - 2 f4 + 5 rho f4, -2 f5) along (u, w), rho = r^2 / R^2;
+    or_mlp_quad_units computes it from the inputs."""
     rng = np.random.default_rng(6)
     for _ in range(20):
         d = rng.normal(size=2) * 3
@@ -99,6 +100,8 @@ def test_quadrupole_in_network_units_is_linear_in_moment_inputs(eps2):
         rho = (d @ d) / (d @ d + eps2)
         want = 1.5 * np.array([-f[4] - f[6] - 2 * f[4] + 5 * rho * f[4], -2 * f[5]])
         np.testing.assert_allclose([q @ u / scale, q @ w / scale], want, rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(O.mlp_quad_units(f, rho), [q @ u / scale, q @ w / scale],
+                                   rtol=1e-10, atol=1e-12)
 
 
 def _linear_net(A, b):
