@@ -342,7 +342,8 @@ class Context:
         nsub = np.empty(groups, dtype=np.uint32)
         centre = np.empty((groups, 64, 2))
         rng = np.empty((groups, 64, 2), dtype=np.uint32)
-        slots = np.empty((groups, cap.value), dtype=np.uint32)
+        spill = max(64, groups // 4)  # spill regions after the per-group ones
+        slots = np.empty((groups + spill, cap.value), dtype=np.uint32)
         bsub = np.empty(self.n, dtype=np.int32)
         boff = np.empty((self.n, 2), dtype=np.float32)
         self._check(lib().nb_debug_far_field(self._h, _ptr(nsub), _ptr(centre), _ptr(rng),
