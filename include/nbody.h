@@ -90,13 +90,10 @@ typedef struct nb_params {
                              for theta < 0.5 it is scaled by 0.5/theta, as
                              lists grow */
     int32_t cell_capacity; /* LEARNED: initial accepted-cell entries per group of
-                              64 bodies (0 = twice the busiest wavefront of a
-                              first walk at creation, at least 4096); a
-                              wavefront that fills its region moves on to one
-                              of max(64, groups/4) spill regions; a subgroup
-                              larger than a region, or running out of spill
-                              regions, halts the step, the regions grow and
-                              the step runs again */
+                              64 bodies (0 = four times the busiest wavefront
+                              of a first walk at creation, at least 4096); a
+                              step that needs more halts, the regions grow (at
+                              least double) and the step runs again */
     int32_t reserved[4];
 } nb_params;
 
@@ -308,16 +305,16 @@ int nb_mlp_grad(nb_ctx* ctx, const float* features, const float* targets, uint64
 int nb_mlp_forward(nb_ctx* ctx, const float* features, uint64_t count, float* out);
 
 /* Bookkeeping of the learned far field in the last force evaluation, for
- * tests. Groups are 64 consecutive bodies of the tree's sorted order
- * (groups = ceil(n/64)). Per group: nsub[g] finished subgroups; per subgroup
+ * tests. Groups are up to 64 consecutive bodies of the tree's sorted order,
+ * never across two systems (*groups_out of them; ceil(n/64) with one
+ * system). Per group: nsub[g] finished subgroups; per subgroup
  * slot (64 per group): its box centre sub_centre[2k..2k+1] and its range
- * {start, count} in cell_slots, the tree slots of all regions (groups +
- * max(64, floor(groups / 4)) spill regions, *cell_cap_out entries each,
- * indexed from 0); per sorted body: its subgroup and its offset from that
- * subgroup's centre. Any output may be NULL. */
+ * {start, count} in cell_slots, the tree slots of all regions (groups x
+ * *cell_cap_out entries, indexed from 0); per sorted body: its subgroup and
+ * its offset from that subgroup's centre. Any output may be NULL. */
 int nb_debug_far_field(nb_ctx* ctx, uint32_t* nsub, double* sub_centre, uint32_t* sub_range,
                        uint32_t* cell_slots, int32_t* body_sub, float* body_off,
-                       uint32_t* cell_cap_out);
+                       uint32_t* cell_cap_out, uint32_t* groups_out);
 
 #ifdef __cplusplus
 }

@@ -320,8 +320,9 @@ int grow_for(nb_ctx* c, uint32_t flags) {
         if (rc) return rc;
     }
     if (flags & FLAG_CELL_CAPACITY) {
+        // at least double: the regions must grow for the step to converge
         const uint64_t need = c->host_meta[META_CELLS_NEEDED];
-        const uint64_t cap = 2 * need + 64;
+        const uint64_t cap = std::max<uint64_t>(2 * need + 64, 2ull * c->cell_cap);
         if (cap > 0xffffffffull) return fail(c, NB_ERR_OUT_OF_MEMORY, "accepted-cell region");
         int rc = alloc_cells(c, (uint32_t)cap);
         if (rc) return rc;
@@ -551,11 +552,12 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
         rc = alloc_tree(c, std::max<uint64_t>(n, 1024));
         if (rc) return bail(rc);
         // Sizes the node pool to the first tree and, with the learned far
-        // field, the accepted-cell regions to twice the first walk's busiest
-        // wavefront (the cells a walk accepts do not depend on the weights).
-        // A wavefront that fills its region within a step moves on to a
-        // spill chunk; only a subgroup larger than a region, or running out
-        // of chunks, halts the step, and then regions and graphs are rebuilt.
+        // field, the accepted-cell regions to four times the first walk's
+        // busiest group (the cells a walk accepts do not depend on the
+        // weights): once bodies scatter, wavefronts also walk queued pieces
+        // of split groups into their regions, and on a 1M disk the busiest
+        // region then holds about 3x the first walk's. A step that still
+        // overflows halts, and the regions and graphs are rebuilt (~4 ms).
         const bool learned = c->p.far_field == NB_FAR_LEARNED;
         rc = build_and_force(c, learned);
         if (rc) return bail(rc);
@@ -563,7 +565,7 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
             rc = alloc_tree(c, c->last_internal + c->last_internal / 4 + 1024);
             if (rc) return bail(rc);
         }
-        const uint64_t cells = 2ull * c->host_meta[META_CELLS_NEEDED] + 64;
+        const uint64_t cells = 4ull * c->host_meta[META_CELLS_NEEDED] + 64;
         if (learned && c->p.cell_capacity == 0 && cells > c->cell_cap && cells <= 0xffffffffull) {
             rc = alloc_cells(c, (uint32_t)cells);
             if (rc) return bail(rc);

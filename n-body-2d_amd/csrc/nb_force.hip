@@ -280,8 +280,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     uint32_t* __restrict__ inter_out, uint32_t n, const StepConsts* __restrict__ k,
     uint32_t* __restrict__ meta, int count_step, int budget, uint32_t max_steps,
     const uint64_t* __restrict__ keys, uint32_t* __restrict__ cell_slot,
-    uint32_t* __restrict__ cell_cnt, uint32_t cell_cap, uint32_t spill_chunks,
-    SubRec* __restrict__ subs,
+    uint32_t* __restrict__ cell_cnt, uint32_t cell_cap, SubRec* __restrict__ subs,
     float4* __restrict__ body_w, uint64_t* __restrict__ queue, uint32_t queue_cap,
     const uint32_t* __restrict__ grp_first, const uint32_t* __restrict__ grp_info) {
     if (meta[META_FLAGS] & kHaltFlags) return;
@@ -294,11 +293,8 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     const double theta_sq = k->theta_sq;
     const float eps2f = (float)k->eps2;
     bool overflow = false;
-    // LEARN: finished subgroups' cells fill the current region (the home
-    // region, then spill chunks) up to cbase; `kept` counts them all.
-    uint32_t cbase = 0, kept = 0;
-    uint64_t region = (uint64_t)home * cell_cap;
-    bool cells_over = false;
+    uint32_t cbase = 0;  // cells written to the home region (LEARN)
+    const uint64_t region = (uint64_t)home * cell_cap;
     if (LEARN && l == 0)
         __hip_atomic_store(&cell_cnt[home], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t g = home;
@@ -425,30 +421,11 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                 if (LEARN) {
                     const bool cell = add && c0 >= 0;
                     const uint64_t cmk = ballot64(cell);
-                    const uint32_t nnew = (uint32_t)__popcll(cmk);
-                    if (nnew && !cells_over && cbase + ccnt + nnew > cell_cap) {
-                        // The region is full: this subgroup's cells so far move to
-                        // a fresh spill chunk (a subgroup's cells stay contiguous)
-                        // and the wavefront goes on there. No chunk left, or more
-                        // cells in one subgroup than a region holds: the step halts.
-                        uint32_t kc = spill_chunks;
-                        if (l == 0 && ccnt + nnew <= cell_cap) kc = atomicAdd(&meta[META_SPILL], 1u);
-                        kc = __shfl(kc, 0, 64);
-                        if (kc < spill_chunks) {
-                            const uint64_t fresh = ((uint64_t)gridDim.x + kc) * cell_cap;
-                            __threadfence_block();  // the lanes' earlier writes, for the copy
-                            for (uint32_t e = (uint32_t)l; e < ccnt; e += 64)
-                                cell_slot[fresh + e] = cell_slot[region + cbase + e];
-                            region = fresh;
-                            cbase = 0;
-                        } else {
-                            cells_over = true;
-                        }
+                    if (cell) {
+                        const uint32_t at = cbase + ccnt + (uint32_t)__popcll(cmk & lanemask_lt());
+                        if (at < cell_cap) cell_slot[region + at] = (uint32_t)node;
                     }
-                    if (cell && !cells_over)
-                        cell_slot[region + cbase + ccnt + (uint32_t)__popcll(cmk & lanemask_lt())] =
-                            (uint32_t)node;
-                    ccnt += nnew;
+                    ccnt += (uint32_t)__popcll(cmk);
                 }
                 // push the non-empty children of opened cells
                 const int32_t cm = open ? r.mask : 0;
@@ -526,7 +503,6 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                     subs[(uint64_t)g * 64 + ks] = sr;
                 }
                 cbase += ccnt;
-                kept += ccnt;
             }
             if (member) {
                 const double ax = (double)t.ax - (double)t.cx;
@@ -559,9 +535,9 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     }
     if (overflow && l == 0) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_STACK);
     if (LEARN && l == 0) {
-        atomicAdd((unsigned long long*)&meta[META_CELLS_TOTAL], (unsigned long long)kept);
-        atomicMax(&meta[META_CELLS_NEEDED], kept);  // sizes the regions
-        if (cells_over) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_CELL_CAPACITY);
+        atomicAdd((unsigned long long*)&meta[META_CELLS_TOTAL], (unsigned long long)cbase);
+        atomicMax(&meta[META_CELLS_NEEDED], cbase);  // sizes the regions at creation
+        if (cbase > cell_cap) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_CELL_CAPACITY);
     }
 }
 
@@ -586,8 +562,7 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
                        c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src], c->b.acc,     \
                        c->b.inter, n, c->b.consts, c->b.meta, (int)(count_step && !learn),     \
                        budget, max_steps, c->b.keys[0], c->b.cell_slot, c->b.cell_cnt,         \
-                       c->cell_cap, c->spill_chunks, c->b.subs, c->b.body_w, c->b.queue,       \
-                       c->queue_cap,                                                           \
+                       c->cell_cap, c->b.subs, c->b.body_w, c->b.queue, c->queue_cap,          \
                        c->b.grp_first,                                                          \
                        c->b.grp_info)
 #define NB_TRAVG3(K, Z, GD) if (learn) NB_TRAVG(false, Z, GD, true); else NB_TRAVG(K, Z, GD, false)

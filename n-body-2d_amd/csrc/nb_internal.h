@@ -48,8 +48,7 @@ enum Meta : int {
     META_FLAGS = 1,      // FLAG_* bits of the current step
     META_STEPS_DONE = 2, // completed steps since the host last cleared it
     META_MULTI_LEAVES = 3, // multi-body leaves queued by the current build
-    META_CELLS_NEEDED = 4, // most accepted cells of one wavefront (learned far field)
-    META_SPILL = 5,        // spill chunks taken by the current step (learned far field)
+    META_CELLS_NEEDED = 4, // most accepted cells of one group (learned far field)
     META_CELLS_TOTAL = 32, // accepted cells of all groups (learned far field), u64 in 32..33
     META_QHEAD = 64,       // traversal work queue: entries pushed
     META_QTAIL = 96,       // traversal work queue: entries taken
@@ -141,8 +140,7 @@ struct Buffers {
     float* pair_t;         // [pair_cap][2] targets
     uint32_t* sample_off;  // [N + 2] per-sample pair counts, then offsets; [N + 1] total
     // learned far field in the step: per group of 64 bodies
-    uint32_t* cell_slot;   // [groups + spill_chunks][cell_cap] tree slots of the accepted
-                           // cells: one region per group, then the spill chunks
+    uint32_t* cell_slot;   // [groups][cell_cap] tree slots of the accepted cells
     uint32_t* cell_cnt;    // [groups] finished subgroups
     SubRec* subs;          // [groups][64]
     float4* body_w;        // [N] {offset from the subgroup centre (x, y), subgroup, -}
@@ -185,7 +183,6 @@ struct nb_ctx {
     uint32_t train_parts = 0;       // partial-gradient rows allocated
     uint32_t train_step = 0;        // Adam step counter
     uint32_t cell_cap = 0;          // accepted-cell region per group
-    uint32_t spill_chunks = 0;      // spare regions a full wavefront moves on to
     uint32_t queue_cap = 0;         // traversal work-queue entries
     uint64_t last_cells = 0;        // accepted cells of the last nb_compute_forces
     // asynchronous snapshots: two slots, used by tickets alternately
@@ -209,7 +206,6 @@ void launch_bbox(nb_ctx* c, int src);
 void launch_morton(nb_ctx* c, int src);
 void launch_radix_sort(nb_ctx* c);
 void launch_gather(nb_ctx* c, int src);
-uint64_t cell_slots_total(const nb_ctx* c);  // entries of cell_slot, spill chunks included
 void launch_unpermute(nb_ctx* c, double2* out);
 void launch_scan_u32(nb_ctx* c, uint32_t* data, uint32_t count, uint32_t* total_out);
 

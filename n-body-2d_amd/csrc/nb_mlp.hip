@@ -353,7 +353,7 @@ constexpr int kApplyWaves = 4;
 template <bool KICK>
 __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     const uint32_t* __restrict__ cell_slot, const uint32_t* __restrict__ cell_cnt,
-    uint64_t slots, const SubRec* __restrict__ subs, const float4* __restrict__ body_w,
+    uint32_t cell_cap, const SubRec* __restrict__ subs, const float4* __restrict__ body_w,
     double2* __restrict__ acc, double2* __restrict__ vel, uint32_t n,
     const StepConsts* __restrict__ k, uint32_t* __restrict__ meta,
     const NodeRec* __restrict__ nodes, const double* __restrict__ box,
@@ -382,6 +382,7 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     MlpWeights w;
     mlp_load_weights(w, tiles, l);
     const int nsub = (int)min(cell_cnt[g], 64u);
+    const uint64_t slots = (uint64_t)groups * cell_cap;
     const int col = l & 15;
     // The records of the next batch of cells are loaded while the current
     // batch runs through the network (raw fields, converted when used).
@@ -513,8 +514,7 @@ void launch_mlp_apply(nb_ctx* c, int src, bool kick, bool count_step) {
     const int count = count_step ? 1 : 0;
 #define NB_APPLY(K)                                                                            \
     hipLaunchKernelGGL((k_mlp_apply<K>), dim3(blocks), dim3(64 * kApplyWaves), 0, c->stream,   \
-                       c->b.cell_slot, c->b.cell_cnt, cell_slots_total(c), c->b.subs,          \
-                       c->b.body_w,                                                            \
+                       c->b.cell_slot, c->b.cell_cnt, c->cell_cap, c->b.subs, c->b.body_w,     \
                        c->b.acc, c->b.vel[src], n, c->b.consts, c->b.meta, c->b.nodes,         \
                        c->b.box, c->b.mlp_tiles, c->b.mlp_bias, count, groups, c->b.grp_first, \
                        c->b.grp_info)
@@ -655,7 +655,6 @@ void free_mlp(nb_ctx* c) {
     b.subs = nullptr;
     b.body_w = nullptr;
     c->cell_cap = 0;
-    c->spill_chunks = 0;
     b.mlp_w = b.mlp_m = b.mlp_v = b.mlp_gpart = b.mlp_loss = b.pair_x = b.pair_t = nullptr;
     b.mlp_tiles = nullptr;
     b.mlp_bias = nullptr;
@@ -665,37 +664,22 @@ void free_mlp(nb_ctx* c) {
 
 int prepare_mlp(nb_ctx* c) { return ensure_mlp(c); }
 
-// Spill chunks beside the per-group regions: a quarter as many (at least
-// 64), taken by wavefronts whose region fills up within a step.
-static uint32_t spill_chunks_for(uint64_t groups) {
-    return (uint32_t)std::max<uint64_t>(64, groups / 4);
-}
-
-uint64_t cell_slots_total(const nb_ctx* c) {
-    return ((uint64_t)c->ngroups + c->spill_chunks) * c->cell_cap;
-}
-
 int alloc_cells(nb_ctx* c, uint32_t cap) {
     Buffers& b = c->b;
     const uint64_t groups = c->ngroups;
-    const uint32_t spill = spill_chunks_for(groups);
-    if ((groups + spill) * cap > 0xffffffffull)  // subgroup ranges are 32-bit
-        return mlp_fail(c, NB_ERR_OUT_OF_MEMORY, "accepted-cell regions exceed 2^32 entries");
     if (b.cell_slot) (void)hipFree(b.cell_slot);
     b.cell_slot = nullptr;
-    if (c->cell_cap) c->device_bytes -= cell_slots_total(c) * sizeof(uint32_t);
+    if (c->cell_cap) c->device_bytes -= groups * c->cell_cap * sizeof(uint32_t);
     c->cell_cap = 0;
-    c->spill_chunks = 0;
-    MLP_HIP(c, hipMalloc((void**)&b.cell_slot, (groups + spill) * cap * sizeof(uint32_t)));
+    MLP_HIP(c, hipMalloc((void**)&b.cell_slot, groups * cap * sizeof(uint32_t)));
     if (!b.cell_cnt) {
         MLP_HIP(c, hipMalloc((void**)&b.cell_cnt, groups * sizeof(uint32_t)));
         MLP_HIP(c, hipMalloc((void**)&b.subs, groups * 64 * sizeof(SubRec)));
         MLP_HIP(c, hipMalloc((void**)&b.body_w, c->n * sizeof(float4)));
         c->device_bytes += groups * (sizeof(uint32_t) + 64 * sizeof(SubRec)) + c->n * sizeof(float4);
     }
-    c->device_bytes += (groups + spill) * cap * sizeof(uint32_t);
+    c->device_bytes += groups * cap * sizeof(uint32_t);
     c->cell_cap = cap;
-    c->spill_chunks = spill;
     return NB_OK;
 }
 
@@ -891,12 +875,13 @@ int nb_mlp_grad(nb_ctx* c, const float* features, const float* targets, uint64_t
 
 int nb_debug_far_field(nb_ctx* c, uint32_t* nsub, double* sub_centre, uint32_t* sub_range,
                        uint32_t* cell_slots, int32_t* body_sub, float* body_off,
-                       uint32_t* cell_cap_out) {
+                       uint32_t* cell_cap_out, uint32_t* groups_out) {
     if (!c) return NB_ERR_INVALID_ARG;
     if (!c->b.cell_slot)
         return mlp_fail(c, NB_ERR_NOT_SUPPORTED, "no learned far field in this context");
     const uint64_t n = c->n, groups = c->ngroups;
     if (cell_cap_out) *cell_cap_out = c->cell_cap;
+    if (groups_out) *groups_out = (uint32_t)groups;
     if (nsub) MLP_HIP(c, scopy(c, nsub, c->b.cell_cnt, groups * 4, hipMemcpyDeviceToHost));
     if (sub_centre || sub_range) {
         std::vector<SubRec> h(groups * 64);
@@ -914,7 +899,7 @@ int nb_debug_far_field(nb_ctx* c, uint32_t* nsub, double* sub_centre, uint32_t* 
         }
     }
     if (cell_slots)
-        MLP_HIP(c, scopy(c, cell_slots, c->b.cell_slot, cell_slots_total(c) * 4,
+        MLP_HIP(c, scopy(c, cell_slots, c->b.cell_slot, groups * c->cell_cap * 4,
                          hipMemcpyDeviceToHost));
     if (body_sub || body_off) {
         std::vector<float4> h(n);

@@ -100,7 +100,6 @@ __global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ pa
         meta[META_MULTI_LEAVES] = 0;  // the tree build of this step queues from 0
         meta[META_QHEAD] = 0;         // the traversal work queue of this step starts empty
         meta[META_QTAIL] = 0;
-        meta[META_SPILL] = 0;
         // flags stay set for the rest of the batch of steps
         if (!isfinite(xmin) || !isfinite(xmax) || !isfinite(ymin) || !isfinite(ymax))
             meta[META_FLAGS] |= (uint32_t)FLAG_NONFINITE;
@@ -166,7 +165,6 @@ __global__ __launch_bounds__(kBboxThreads) void k_bbox_sys(const double2* __rest
             meta[META_MULTI_LEAVES] = 0;
             meta[META_QHEAD] = 0;
             meta[META_QTAIL] = 0;
-            meta[META_SPILL] = 0;
         }
         if (!isfinite(xmin) || !isfinite(xmax) || !isfinite(ymin) || !isfinite(ymax))
             atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_NONFINITE);

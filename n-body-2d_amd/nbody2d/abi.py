@@ -120,7 +120,8 @@ def lib() -> C.CDLL:
                                    C.POINTER(C.c_float)]
         L.nb_mlp_grad.argtypes = [P, P, P, C.c_uint64, P, C.POINTER(C.c_float)]
         L.nb_mlp_forward.argtypes = [P, P, C.c_uint64, P]
-        L.nb_debug_far_field.argtypes = [P, P, P, P, P, P, P, C.POINTER(C.c_uint32)]
+        L.nb_debug_far_field.argtypes = [P, P, P, P, P, P, P, C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_uint32)]
         L.nb_autotune.argtypes = [P, C.c_uint32, C.POINTER(Tuning)]
         L.nb_snapshot_begin.argtypes = [P, C.POINTER(C.c_uint64)]
         L.nb_snapshot_query.argtypes = [P, C.c_uint64, C.POINTER(C.c_int)]
@@ -335,19 +336,19 @@ class Context:
 
 
     def debug_far_field(self):
-        cap = C.c_uint32()
+        cap, ng = C.c_uint32(), C.c_uint32()
         self._check(lib().nb_debug_far_field(self._h, None, None, None, None, None, None,
-                                             C.byref(cap)))
-        groups = (self.n + 63) // 64
+                                             C.byref(cap), C.byref(ng)))
+        groups = ng.value
         nsub = np.empty(groups, dtype=np.uint32)
         centre = np.empty((groups, 64, 2))
         rng = np.empty((groups, 64, 2), dtype=np.uint32)
-        spill = max(64, groups // 4)  # spill regions after the per-group ones
-        slots = np.empty((groups + spill, cap.value), dtype=np.uint32)
+        slots = np.empty((groups, cap.value), dtype=np.uint32)
         bsub = np.empty(self.n, dtype=np.int32)
         boff = np.empty((self.n, 2), dtype=np.float32)
         self._check(lib().nb_debug_far_field(self._h, _ptr(nsub), _ptr(centre), _ptr(rng),
-                                             _ptr(slots), _ptr(bsub), _ptr(boff), C.byref(cap)))
+                                             _ptr(slots), _ptr(bsub), _ptr(boff), C.byref(cap),
+                                             C.byref(ng)))
         return nsub, centre, rng, slots, bsub, boff
 
 

@@ -282,39 +282,3 @@ def test_region_growth_keeps_learned_forces():
         pb, vb = b.state()
     assert np.array_equal(pa, pb) and np.array_equal(va, vb)
 
-
-def test_spill_regions_keep_learned_forces():
-    """A wavefront whose region fills up within a step moves its subgroup to a
-    spill region and goes on there: forces and steps equal those of a context
-    whose regions never fill, and no region is regrown (every subgroup fits
-    one region)."""
-    m, p, v = ics.exp_disk(16384, seed=31)
-    w = (np.random.default_rng(5).normal(size=4866) * 0.05).astype(np.float32)
-    kw = dict(theta=0.8, softening=EPS, traversal=TRAV_GROUP, self_guard=1,
-              far_field=FAR_LEARNED)
-    with _ctx(m, p, v, **kw) as a:
-        a.mlp_set_weights(w)
-        fa, _, _ = a.forces()
-        nsub, _, rng_, _, _, _ = a.debug_far_field()
-    used = np.arange(64)[None, :] < nsub[:, None]
-    counts = rng_[..., 1][used]
-    per_group = (rng_[..., 1] * used).sum(1)
-    cap = int(counts.max())
-    print(f"[gpu] subgroups {used.sum()} max cells {cap}, group total max {per_group.max()}")
-    with _ctx(m, p, v, cell_capacity=cap, **kw) as a, _ctx(m, p, v, **kw) as b:
-        a.mlp_set_weights(w)
-        b.mlp_set_weights(w)
-        bytes0 = a.info().device_bytes
-        fb, _, _ = a.forces()
-        assert a.info().device_bytes == bytes0  # nothing regrown
-        nsub, _, rng_, _, _, _ = a.debug_far_field()
-        starts = rng_[..., 0][np.arange(64)[None, :] < nsub[:, None]]
-        spilled = int((starts >= len(nsub) * cap).sum())
-        print(f"[gpu] region {cap}: {spilled} subgroups in spill regions")
-        assert spilled > 0
-        assert np.array_equal(fa, fb)
-        a.step(3)
-        b.step(3)
-        pa, va = a.state()
-        pb, vb = b.state()
-    assert np.array_equal(pa, pb) and np.array_equal(va, vb)
