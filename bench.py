@@ -63,6 +63,8 @@ TRAIN_BATCH = 16384
 TRAIN_SCHEDULE = ((1.0, 2e-3), (0.5, 5e-4), (0.5, 1e-4))
 # Plain legs timed to find the plain theta at a learned leg's error.
 EQUAL_ERROR_THETAS = (0.4, 0.35, 0.3, 0.25, 0.2, 0.15, 0.12, 0.1)
+# Ensemble network: pairs from this many systems spread over the size range.
+ENS_TRAIN_SYSTEMS = 16
 # A second learned point, at the error that the 1e-3 target asks for.
 THETA_LEARNED_LOW = 0.5
 METRIC = "N=1M body-steps/s: plain BH vs learned far-field BH at equal force error"
@@ -454,14 +456,27 @@ def run_ensemble(nbody2d, ics, args) -> dict:
     vel = np.vstack([s[2] for s in systems])
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
     n = len(mass)
-    # network: trained once, on the largest system
-    big = int(np.argmax(sizes))
+    # network: trained once, on pairs pooled from ENS_TRAIN_SYSTEMS systems
+    # spread over the size range (a network trained on the largest system
+    # alone did worse than the quadrupole term on the small ones)
+    order = np.argsort(sizes)
+    picks = order[np.linspace(0, len(sizes) - 1, min(ENS_TRAIN_SYSTEMS, len(sizes))).astype(int)]
     t0 = time.perf_counter()
+    xs, ts = [], []
+    per_sys = TRAIN_PAIRS // len(picks)
+    for s in picks:
+        with nbody2d.Context(*systems[s], G=1.0, dt=DT, theta=theta, softening=EPS) as t:
+            t.mlp_generate(theta, int(sizes[s]), per_sys, seed=7)
+            x, tt, _, _ = t.mlp_pairs()
+            xs.append(x)
+            ts.append(tt)
+    big = int(np.argmax(sizes))
     with nbody2d.Context(*systems[big], G=1.0, dt=DT, theta=theta, softening=EPS) as t:
         t.mlp_init(1)
-        pairs = t.mlp_generate(theta, 8192, 1 << 22, seed=7)
-        t.mlp_train(4000, 4096, 2e-3, seed=1)
-        loss = t.mlp_train(2000, 4096, 4e-4, seed=2)
+        t.mlp_set_pairs(np.concatenate(xs), np.concatenate(ts))
+        pairs = int(sum(len(x) for x in xs))
+        for i, (frac, lr) in enumerate(TRAIN_SCHEDULE):
+            loss = t.mlp_train(max(1, int(args.train_steps * frac)), TRAIN_BATCH, lr, seed=1 + i)
         w = t.mlp_weights()
     train_s = time.perf_counter() - t0
     kw = dict(G=1.0, dt=DT, theta=theta, softening=EPS, traversal=abi.TRAV_GROUP,
@@ -476,16 +491,22 @@ def run_ensemble(nbody2d, ics, args) -> dict:
         tn = b.autotune(2)
         tune = {"group_budget": tn.group_budget, "settings": tn.candidates,
                 "best_force_ms": tn.best_ms, "default_force_ms": tn.default_ms}
-    # force error on 8 seeded systems against the FP64 direct sum
+    # force error on 8 seeded systems against the FP64 direct sum, with the
+    # trained network and with zero weights (the quadrupole term alone)
     fb, _, _ = b.forces()
+    b.mlp_set_weights(np.zeros_like(w))
+    fz, _, _ = b.forces()
+    b.mlp_set_weights(w)
     rng = np.random.default_rng(1)
-    errs = []
+    errs, errs_q = [], []
     for s in rng.choice(len(sizes), 8, replace=False):
         lo, hi = int(off[s]), int(off[s + 1])
         ref = nbody2d.direct_sum_f64(pos[lo:hi], mass[lo:hi], 1.0, EPS, np.arange(hi - lo))
-        e = np.linalg.norm(fb[lo:hi] - ref, axis=1) / np.linalg.norm(ref, axis=1)
-        errs.append(e)
+        norm = np.linalg.norm(ref, axis=1)
+        errs.append(np.linalg.norm(fb[lo:hi] - ref, axis=1) / norm)
+        errs_q.append(np.linalg.norm(fz[lo:hi] - ref, axis=1) / norm)
     e = np.concatenate(errs)
+    eq = np.concatenate(errs_q)
     b.step(args.warmup)
     t0 = time.perf_counter()
     b.step(args.steps)
@@ -526,11 +547,13 @@ def run_ensemble(nbody2d, ics, args) -> dict:
         "force_error": {"rms": float(np.sqrt(np.mean(e ** 2))), "max": float(e.max()),
                         "sample": f"all bodies of 8 seeded systems ({len(e)})",
                         "judge": "GPU FP64 direct sum", "state": "initial"},
+        "force_error_quadrupole_only": {"rms": float(np.sqrt(np.mean(eq ** 2))),
+                                        "max": float(eq.max())},
         "phase_ms_per_step_eager": per,
         "far_field_cells_per_step": int(info.far_field_cells),
         "autotune": tune,
         "training": {"pairs": pairs, "loss": loss, "seconds": train_s,
-                     "system": int(sizes[big])},
+                     "systems": [int(sizes[s]) for s in picks]},
         "roofline": None,
         "cpu_baseline": None,
     }

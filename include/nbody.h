@@ -253,10 +253,10 @@ int nb_autotune(nb_ctx* ctx, uint32_t repeats, nb_tuning* out);
  * is the monopole of every accepted cell (BarnesHutCuda.cu:461-471). The
  * correction of an accepted cell, a_exact - a_monopole, is approximated by
  * the cell's softened quadrupole term, computed from its second moments, plus
- * a network (8 -> 64 -> 64 -> 2, ReLU, BF16 weights and activations on MFMA,
- * FP32 accumulation) trained on the rest, from the cell's offset,
- * size/distance, mass fraction and second moments; a context with far_field
- * = NB_FAR_LEARNED adds it for every accepted internal cell.
+ * s/R times a network (8 -> 64 -> 64 -> 2, ReLU, BF16 weights and activations
+ * on MFMA, FP32 accumulation) trained on the rest over s/R, from the cell's
+ * offset, size/distance, mass fraction and second moments; a context with
+ * far_field = NB_FAR_LEARNED adds it for every accepted internal cell.
  * Parameters are FP32 in the order W1[64][8], b1[64], W2[64][64], b2[64],
  * W3[2][64], b3[2] (NB_MLP_PARAMS values); a new context holds zeros (the
  * quadrupole term alone).
@@ -276,7 +276,8 @@ int nb_mlp_init(nb_ctx* ctx, uint64_t seed);
  * every accepted internal cell of at most 8,192 bodies, up to `max_pairs`.
  * Each pair gets its 8 inputs and its target, the exact FP64 Plummer
  * acceleration of the cell's bodies minus the cell's monopole and minus its
- * softened quadrupole term, in the network's output units.
+ * softened quadrupole term, in the network's output units, over s/R (the
+ * input f2).
  * *pairs_out = pairs held. */
 int nb_mlp_generate(nb_ctx* ctx, double theta, uint32_t sample_bodies, uint64_t max_pairs,
                     uint64_t seed, uint64_t* pairs_out);

@@ -29,9 +29,11 @@
 // correction a_exact - a_monopole of the Plummer kernel along (u, w):
 // M s^2 r / R^5, the scale of the softened quadrupole term, which in these
 // units is q = (3/2)(5 rho f4 - 3 f4 - f6, -2 f5), rho = r^2/R^2. The
-// correction applied is q + (y0, y1): q is computed exactly and the network
-// output (y0, y1) is trained on the rest (octupole and higher orders, which
-// the second moments only hint at).
+// correction applied is q + f2 (y0, y1): q is computed exactly and the
+// network output (y0, y1) is trained on the rest over f2 = s/R (octupole and
+// higher orders, of order s/R in these units, which the second moments only
+// hint at), so every target is of order one and a far cell's share of the
+// network's error is small.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -9,7 +9,7 @@
 // is the exact FP64 Plummer acceleration of the cell's bodies at the sample
 // body minus the cell's monopole, along (u, w) in units of M s^2 r / R^5
 // (the network's output frame, nb_mlp.h), less the softened quadrupole term
-// that the apply adds exactly.
+// that the apply adds exactly, over s/R.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -136,10 +136,11 @@ __global__ __launch_bounds__(256) void k_mlp_targets(
                                              ux, uy, qd);
 #pragma unroll
             for (int i = 0; i < 8; i++) X[q * 8 + i] = f[i];
-            // exact - monopole along (u, w), w = (-uy, ux), less the quadrupole term
+            // exact - monopole along (u, w), w = (-uy, ux), less the quadrupole
+            // term, over s/R (the octupole's scale in these units)
             const double cx = ax - dx * w, cy = ay - dy * w;
-            T[q * 2 + 0] = (float)((cx * ux + cy * uy) / (double)scale - (double)qd[0]);
-            T[q * 2 + 1] = (float)((cy * ux - cx * uy) / (double)scale - (double)qd[1]);
+            T[q * 2 + 0] = (float)(((cx * ux + cy * uy) / (double)scale - (double)qd[0]) / (double)f[2]);
+            T[q * 2 + 1] = (float)(((cy * ux - cx * uy) / (double)scale - (double)qd[1]) / (double)f[2]);
         }
     }
 }
@@ -365,7 +366,7 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     __shared__ float su[kApplyWaves][6 * 64];        // [C_x, C_y, K00, K01, K10, K11][subgroup]
     __shared__ __attribute__((aligned(16))) float sf[kApplyWaves][64 * 8];  // cell inputs
     __shared__ float2 sr[kApplyWaves][64];           // cell scale * u
-    __shared__ float2 sq[kApplyWaves][64];           // quadrupole term, output units
+    __shared__ float4 sq[kApplyWaves][64];           // quadrupole term, output units; s/R
     for (int i = threadIdx.x; i < kMlpBiasLen; i += 64 * kApplyWaves) sb[i] = bias[i];
     if (count_step && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
     __syncthreads();
@@ -441,7 +442,7 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
             *reinterpret_cast<float4*>(f_all + 8 * l) = make_float4(f[0], f[1], f[2], f[3]);
             *reinterpret_cast<float4*>(f_all + 8 * l + 4) = make_float4(f[4], f[5], f[6], f[7]);
             sr[wv][l] = make_float2(scale * ux, scale * uy);
-            sq[wv][l] = make_float2(qd[0], qd[1]);
+            sq[wv][l] = make_float4(qd[0], qd[1], f[2], 0.f);
             __builtin_amdgcn_wave_barrier();
             f32x4 x[4], y[4];
 #pragma unroll
@@ -459,9 +460,10 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const float2 su_ = sr[wv][(16 * q + l) & 63];
-                const float2 qq = sq[wv][(16 * q + l) & 63];
+                const float4 qq = sq[wv][(16 * q + l) & 63];
                 if (l < 16) {
-                    const float yu = y[q].x + qq.x, yw = y[q].y + qq.y;
+                    // correction = quadrupole term + (s/R) * network output
+                    const float yu = fmaf(y[q].x, qq.z, qq.x), yw = fmaf(y[q].y, qq.z, qq.y);
                     cx += yu * su_.x - yw * su_.y;
                     cy += yu * su_.y + yw * su_.x;
                 }

@@ -131,9 +131,10 @@ def test_training_pairs_match_oracle():
         corr = O.cell_correction(sp, sm, n["bstart"], n["bend"], sp[body[q]], EPS * EPS)
         u = np.array([f[0], f[1]])
         rho = (d[q] @ d[q]) / (d[q] @ d[q] + eps2)
-        # the target leaves out the quadrupole term, which the apply adds exactly
+        # the target leaves out the quadrupole term, which the apply adds
+        # exactly, and is taken over s/R = f2
         want = (np.array([corr @ u, corr @ np.array([-u[1], u[0]])]) / scale
-                - O.mlp_quad_units(f, rho))
+                - O.mlp_quad_units(f, rho)) / f[2]
         np.testing.assert_allclose(t[q], want, rtol=1e-4,
                                    atol=1e-5 * max(1.0, np.abs(want).max()))
 
@@ -210,8 +211,9 @@ def test_learned_graph_and_eager_steps_agree():
 def test_far_field_correction_matches_oracle():
     """Per body, learned minus monopole equals the oracle's sum over the
     subgroup's accepted cells of the correction at the subgroup centre (the
-    softened quadrupole term plus the network's output), minus the summed
-    softened-quadrupole Jacobians times the body's offset from that centre."""
+    softened quadrupole term plus s/R times the network's output), minus the
+    summed softened-quadrupole Jacobians times the body's offset from that
+    centre."""
     m, p, v = ics.exp_disk(4096, seed=12)
     w = (np.random.default_rng(3).normal(size=4866) * 0.1).astype(np.float32)
     kw = dict(theta=0.8, softening=EPS, traversal=TRAV_GROUP, self_guard=1)
@@ -249,8 +251,8 @@ def test_far_field_correction_matches_oracle():
             C = np.zeros(2)
             if ct:
                 y = O.mlp_forward(w.astype(np.float64), np.array(X, dtype=np.float32))
-                for yy, sc, u, qq in zip(y.astype(np.float64), S, U, Q):
-                    yy = yy + qq
+                for yy, sc, u, qq, ff in zip(y.astype(np.float64), S, U, Q, X):
+                    yy = qq + ff[2] * yy
                     C += sc * (yy[0] * u + yy[1] * np.array([-u[1], u[0]]))
             members = np.nonzero((bsub == k) & (np.arange(len(bsub)) // 64 == g))[0]
             for i in members:
