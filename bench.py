@@ -137,7 +137,7 @@ def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
     acc, inter, total = ctx.forces()
     rms, mx = ics.force_errors(acc[t], ref)
     leg["force_error"] = {"rms": rms, "max": mx, "sample": int(len(t)), "state": "initial",
-                          "judge": "GPU FP64 direct sum"}
+                          "reference": "GPU FP64 direct sum"}
     info = ctx.info()
     leg["interactions"] = total
     leg["interactions_per_body"] = total / n
@@ -377,7 +377,7 @@ def run_allpairs(nbody2d, ics, args) -> dict:
                                f"drift-then-kick, FP32 LDS tiles + Kahan",
                    "n_bodies": n},
         "force_error": {"rms": rms, "max": mx, "sample": int(len(t)),
-                        "judge": "GPU FP64 direct sum"},
+                        "reference": "GPU FP64 direct sum"},
         "autotune": tune,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
@@ -429,7 +429,7 @@ def run_two_disks(nbody2d, ics, args) -> dict:
         "config": {"workload": f"two_disks N={n}, theta={args.theta_plain}, eps={EPS}, "
                                f"dt={DT}, plain group Barnes-Hut", "n_bodies": n},
         "force_error": {"rms": e0[0], "max": e0[1], "sample": int(len(t)), "state": "initial",
-                        "judge": "GPU FP64 direct sum"},
+                        "reference": "GPU FP64 direct sum"},
         "force_error_after_steps": {"rms": e1[0], "max": e1[1],
                                     "steps": args.warmup + args.steps},
         "interactions_per_body": total / n,
@@ -546,7 +546,7 @@ def run_ensemble(nbody2d, ics, args) -> dict:
         "batched_over_looped": wall_l / wall_b,
         "force_error": {"rms": float(np.sqrt(np.mean(e ** 2))), "max": float(e.max()),
                         "sample": f"all bodies of 8 seeded systems ({len(e)})",
-                        "judge": "GPU FP64 direct sum", "state": "initial"},
+                        "reference": "GPU FP64 direct sum", "state": "initial"},
         "force_error_quadrupole_only": {"rms": float(np.sqrt(np.mean(eq ** 2))),
                                         "max": float(eq.max())},
         "phase_ms_per_step_eager": per,

@@ -223,7 +223,7 @@ int nb_debug_build(nb_ctx* ctx, double* box4, uint64_t* sorted_keys, uint32_t* o
 /* Copies the tree built by the last step or nb_debug_build. */
 int nb_debug_tree(nb_ctx* ctx, nb_node* nodes, uint64_t cap);
 
-/* Stateless FP64 direct sum on the GPU, the judge of force errors:
+/* Stateless FP64 direct sum on the GPU, the reference for force errors:
  * acc[t] = G * sum_{j != targets[t]} m_j (r_j - r_t) / (|r|^2 + eps^2)^{3/2}. */
 int nb_direct_sum_f64(uint64_t n, const double* pos_xy, const double* mass, double G,
                       double softening, uint64_t m, const int64_t* targets, double* acc_xy);

@@ -5,7 +5,7 @@
 //                 BarnesHutCuda.cu:382-486, re-designed for wave64)
 //   k_allpairs    LDS-tiled direct sum (AllPairsSim, AllPairs.cpp:38-49)
 //   k_drift       x += v dt (update_positions, BarnesHutCuda.cu:489-495)
-//   k_direct64    FP64 direct sum, the judge of force errors
+//   k_direct64    FP64 direct sum, the reference for force errors
 //
 // Precision: the cell offset dx = com - x, its square d^2 = fma(dx, dx, dy*dy)
 // and the opening test wsq < theta^2 d^2 are evaluated in FP64 on the FP64
@@ -745,7 +745,7 @@ void launch_drift(nb_ctx* c, int src, bool check, bool count_step) {
 }
 
 // ---------------------------------------------------------------------------
-// FP64 direct sum (judge): thread per target, sources through LDS in FP64,
+// FP64 direct sum (the force-error reference): thread per target, sources through LDS in FP64,
 // j range split across blockIdx.y, Kahan-compensated FP64 sums, IEEE sqrt and
 // division.
 // ---------------------------------------------------------------------------

@@ -3,7 +3,7 @@
 ``build_hip()`` compiles the gfx950 HIP library ``libnbody2d.so`` (kernels +
 C ABI) into this directory; ``build_host()`` compiles the C++ host adapter
 (``libnbody2d_host.so`` and the headless driver ``nbody2d-run``) against it.
-Both always recompile. ``build_oracle()`` builds the CPU checker under
+Both always recompile. ``build_oracle()`` builds the CPU oracle under
 ``oracle/`` (test infrastructure, not linked by the product).
 """
 from __future__ import annotations
@@ -78,7 +78,7 @@ def build_host() -> list[Path]:
 
 
 def build_oracle() -> None:
-    """CPU checker (tests/ and bench.py's cpu_baseline leg only)."""
+    """CPU oracle (tests/ and bench.py's cpu_baseline leg only)."""
     _run(["make", "-B", "-C", str(ORACLE)])
 
 

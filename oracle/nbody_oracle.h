@@ -14,8 +14,8 @@
  *      level-synchronous quadtree topology (all four child slots materialised,
  *      level cap 30), bottom-up mass/COM, per-body DFS force with the side MAC,
  *      kick then drift.
- *   C. FP64 direct sum with compensated (Neumaier) accumulation, the judge of
- *      every force error.
+ *   C. FP64 direct sum with compensated (Neumaier) accumulation, the
+ *      reference for every force error.
  * Parity status of each function is listed in the header comment of
  * nbody_oracle.c.
  */

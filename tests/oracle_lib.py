@@ -1,6 +1,6 @@
 """ctypes access to the CPU oracle (oracle/liboracle.so) for the tests.
 
-Test infrastructure: the checker the HIP path is compared with.
+Test infrastructure: the oracle the HIP path is compared with.
 """
 from __future__ import annotations
 
