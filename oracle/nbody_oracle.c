@@ -860,7 +860,7 @@ int or_model_step(uint64_t n, double* pos, double* vel, double* mass, uint32_t* 
 }
 
 /* =====================================================================
- * Part C — FP64 direct sum (the judge)
+ * Part C — FP64 direct sum (the force-error reference)
  * ===================================================================== */
 
 typedef struct {
