@@ -1,7 +1,7 @@
 /*
  * nbody_oracle.c — CPU restatement of the reference's 2-D N-body step.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for tests/ and the timed CPU baseline
+ * TEST INFRASTRUCTURE ONLY: the oracle for tests/ and the timed CPU baseline
  * of bench.py. The product (HIP library + C++ host adapter) never links it.
  *
  * Built with -O2 -ffp-contract=off (no fast-math, no reassociation) so every
@@ -932,7 +932,7 @@ void or_direct_sum(uint64_t n, const double* pos, const double* mass, double eps
 }
 
 /* =====================================================================
- * Part D — far-field MLP, FP64 (checker of the GPU forward/backward)
+ * Part D — far-field MLP, FP64 (oracle of the GPU forward/backward)
  * ===================================================================== */
 
 static double bf16_round(double v) {

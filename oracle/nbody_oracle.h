@@ -2,7 +2,7 @@
  * nbody_oracle.h — CPU restatement of the reference's 2-D N-body step.
  *
  * TEST INFRASTRUCTURE ONLY. Nothing in the product (the HIP library, the C++
- * host adapter) links or calls this code. It is used by tests/ as the checker
+ * host adapter) links or calls this code. It is used by tests/ as the oracle
  * and by bench.py's cpu_baseline leg as the timed CPU baseline.
  *
  * Three parts:
