@@ -601,6 +601,7 @@ int nb_step(nb_ctx* c, uint64_t k) {
         return NB_OK;
     }
     uint64_t remaining = k;
+    int regrowths = 0;  // each grows a buffer past the need that halted a step
     while (remaining > 0) {
         int rc = clear_meta(c);
         if (rc) return rc;
@@ -638,6 +639,7 @@ int nb_step(nb_ctx* c, uint64_t k) {
         if (flags & FLAG_NONFINITE) return fail(c, NB_ERR_NUMERIC, "non-finite body positions");
         if (flags & FLAG_STACK) return fail(c, NB_ERR_OVERFLOW, "traversal stack overflow");
         if (flags & (FLAG_TREE_CAPACITY | FLAG_CELL_CAPACITY)) {
+            if (++regrowths > 32) return fail(c, NB_ERR_HIP, "buffer capacity did not converge");
             rc = grow_for(c, flags);
             if (rc) return rc;
             if (is_dk(c) && remaining > 0) {
