@@ -381,19 +381,25 @@ __global__ __launch_bounds__(64) void k_traverse_group(
             uint32_t listed = 0, steps = 0;
             bool own_multi = false, split = false;
             if (l == 0) s_stack[0] = root;  // the root of this group's system
-            while (sp > 0) {
-                steps++;
+            // Each step pops up to 64 stack entries and loads their records; the
+            // next step's records are loaded before the list is flushed, so
+            // their latency overlaps the list's evaluation.
+            bool have = false;
+            int32_t node = 0;
+            NodeRec r;
+            auto pop = [&]() -> bool {  // false: the stack would overflow
                 const int take = max(1, min(min(64, sp), (kGStack - kGReserve - sp) / 3));
-                if (sp + 3 > kGStack) {  // unreachable: see kGReserve
-                    overflow = true;
-                    break;
-                }
+                if (sp + 3 > kGStack) return false;  // unreachable: see kGReserve
                 const int base = sp - take;
-                const bool have = l < take;
-                const int32_t node = have ? s_stack[base + l] : 0;
+                have = l < take;
+                node = have ? s_stack[base + l] : 0;
                 sp = base;
-                NodeRec r;
                 if (have) r = nodes[node];  // one 64-byte record: four independent loads
+                return true;
+            };
+            overflow = !pop();
+            while (!overflow) {
+                steps++;
                 const int32_t c0 = have ? r.c0 : -2;
                 const double dxc = have ? r.cx - ox : 0.0, dyc = have ? r.cy - oy : 0.0;
                 bool add = false, open = false;
@@ -458,11 +464,17 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                     split = true;
                     break;
                 }
+                const bool more = sp > 0;
+                if (more && !pop()) {
+                    overflow = true;
+                    break;
+                }
                 if (cnt > kGFlush) {
                     eval_list<ZERO_EPS>(s_list, cnt, bx, by, eps2f, t);
                     listed += cnt;
                     cnt = 0;
                 }
+                if (!more) break;
             }
             if (overflow) break;
             if (split) {
