@@ -97,3 +97,21 @@ def test_batch_of_many_uneven_systems_matches_spot_checks():
             a.step(2)
             pa, va = a.state()
         assert np.array_equal(pb[lo:hi], pa) and np.array_equal(vb[lo:hi], va), s
+
+
+def test_batch_far_field_bookkeeping_covers_every_group():
+    """Groups never span two systems, so a batch has more of them than
+    ceil(n/64): the far-field bookkeeping reports its own group count, and
+    every finished subgroup's cell range lies inside the cell regions."""
+    kw = dict(theta=0.6, softening=EPS, traversal=TRAV_GROUP, far_field=FAR_LEARNED)
+    sizes = [2048, 300, 65, 5000, 1, 130]
+    systems, m, p, v, off = _systems(sizes, seed=400)
+    with nbody2d.Context(m, p, v, offsets=off, **kw) as b:
+        b.forces()
+        nsub, centre, rng_, slots, bsub, boff = b.debug_far_field()
+    groups = sum((k + 63) // 64 for k in sizes)
+    assert len(nsub) == groups > (sum(sizes) + 63) // 64
+    used = np.arange(64)[None, :] < nsub[:, None]
+    start, count = rng_[..., 0][used].astype(np.int64), rng_[..., 1][used].astype(np.int64)
+    assert np.all(start + count <= slots.size)
+    assert np.all(bsub >= 0) and np.all(bsub < 64)
