@@ -93,6 +93,80 @@ def test_tree_topology_and_com(name):
         assert np.all(np.abs(gn[f] - rn[f]) <= 1e-5 * scale + 1e-30)
 
 
+def _nonempty_sorted(nodes, mask):
+    """Non-empty nodes ordered by (first body, level), with their child masks."""
+    sel = nodes["bstart"] >= 0
+    n, msk = nodes[sel], mask[sel]
+    o = np.lexsort((n["level"], n["bstart"]))
+    return n[o], msk[o]
+
+
+def _oracle_mask(r):
+    mask = np.zeros(len(r), dtype=np.int64)
+    internal = r["c0"] >= 0
+    for q in range(4):
+        mask[internal] |= (r["bstart"][r["c0"][internal] + q] >= 0).astype(np.int64) << q
+    return mask
+
+
+def _assert_same_tree(g, r):
+    """GPU nodes g equal oracle nodes r: topology, mass and COM bit for bit."""
+    gn, gm = _nonempty_sorted(g, np.where(g["c0"] >= 1, g["child_mask"], 0))
+    rn, rm = _nonempty_sorted(r, _oracle_mask(r))
+    assert len(gn) == len(rn)
+    for f in ("level", "bstart", "bend"):
+        assert np.array_equal(gn[f], rn[f]), f
+    assert np.array_equal(gm, rm)
+    for f in ("mass", "cx", "cy"):
+        assert np.array_equal(gn[f], rn[f]), f
+    scale = np.maximum(rn["ixx"] + rn["iyy"], 1e-300)
+    for f in ("ixx", "ixy", "iyy"):
+        assert np.all(np.abs(gn[f] - rn[f]) <= 1e-5 * scale + 1e-30), f
+
+
+def test_tree_com_large_disk():
+    """262,144 bodies: ~190,000 internal nodes over 23 levels."""
+    m, p, v = ics.exp_disk(1 << 18, seed=5)
+    with _ctx(m, p, v) as c:
+        _, _, _, cnt = c.debug_build()
+        g = c.debug_tree(cnt)
+    _, keys, perm, root_wsq = O.prepare(p, m)
+    _assert_same_tree(g, O.tree_build(keys[perm], p[perm], m[perm], root_wsq))
+
+
+def test_tree_com_many_systems_deep_chains():
+    """260 systems, each a cluster 1e-6 across plus one far body: every system
+    has a ~20-level chain of single-child nodes above a cluster of ~600
+    internal nodes, with multi-body leaves at level 30."""
+    systems = []
+    for s in range(260):
+        mc, pc, _ = ics.exp_disk(800, seed=1000 + s)
+        pc = pc * 1e-6 + np.array([0.25, -0.5]) * (s % 3)
+        ms = np.concatenate([mc, [1.0]])
+        ps = np.vstack([pc, [[1.0 + 0.01 * s, 1.0]]])
+        systems.append((ms, ps))
+    sizes = [len(ms) for ms, _ in systems]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    m = np.concatenate([ms for ms, _ in systems])
+    p = np.vstack([ps for _, ps in systems])
+    with _ctx(m, p, np.zeros_like(p), offsets=off, traversal=nbody2d.abi.TRAV_GROUP) as c:
+        _, _, _, cnt = c.debug_build()
+        g = c.debug_tree(cnt)
+    parts = []
+    for s, (ms, ps) in enumerate(systems):
+        _, keys, perm, root_wsq = O.prepare(ps, ms)
+        r = O.tree_build(keys[perm], ps[perm], ms[perm], root_wsq)
+        lo = int(off[s])
+        r["c0"] = np.where(r["c0"] >= 0, r["c0"] + sum(len(x) for x in parts), r["c0"])
+        for f in ("bstart", "bend"):
+            r[f] = np.where(r[f] >= 0, r[f] + lo, r[f])
+        parts.append(r)
+    r = np.concatenate(parts)
+    levels = r["level"][r["c0"] >= 0]
+    assert levels.max() >= 20
+    _assert_same_tree(g, r)
+
+
 @pytest.mark.parametrize("name", ["disk4k", "uniform3k", "dups", "two", "ragged65"])
 @pytest.mark.parametrize("theta,eps,guard", [(0.5, 0.01, 0), (0.0, 0.01, 0), (0.8, 0.01, 0),
                                              (0.8, 0.01, 1), (0.3, 0.0, 0)])
