@@ -961,12 +961,31 @@ int nb_debug_tree(nb_ctx* c, nb_node* nodes, uint64_t cap) {
     if (cap < count) return fail(c, NB_ERR_INVALID_ARG, "cap smaller than the node count");
     std::vector<NodeRec> h(count);
     NB_HIP(c, hipMemcpy(h.data(), c->b.nodes, count * sizeof(NodeRec), hipMemcpyDeviceToHost));
+    // Empty child slots are not written on the device (their parent's mask
+    // marks them); they are reported here as empty nodes. A parent's slot
+    // precedes its children's, so one forward pass finds them.
     std::vector<int32_t> level(count, 0);
+    std::vector<uint8_t> live(count, 0);
+    std::vector<double> empty_wsq(count, 0.0);
+    for (uint64_t s = 0; s < c->nsys && s < count; s++) live[s] = 1;
     for (uint64_t s = 0; s < count; s++) {
+        nb_node& o = nodes[s];
+        if (!live[s]) {
+            o = nb_node{};
+            o.level = level[s];
+            o.bstart = -1;
+            o.bend = -1;
+            o.c0 = -2;
+            o.wsq = empty_wsq[s];
+            continue;
+        }
         const NodeRec& r = h[s];
         if (r.c0 >= 1 && (uint64_t)r.c0 + 3 < count)
-            for (int q = 0; q < 4; q++) level[r.c0 + q] = level[s] + 1;
-        nb_node& o = nodes[s];
+            for (int q = 0; q < 4; q++) {
+                level[r.c0 + q] = level[s] + 1;
+                live[r.c0 + q] = (uint8_t)((r.mask >> q) & 1);
+                empty_wsq[r.c0 + q] = r.wsq * 0.25;
+            }
         o.level = level[s];
         o.bstart = r.bs;
         o.bend = r.be;

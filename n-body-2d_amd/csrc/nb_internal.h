@@ -69,7 +69,7 @@ struct __attribute__((aligned(64))) NodeRec {
     double cx, cy;   // centre of mass
     double m;        // mass
     double wsq;      // squared cell side (BarnesHutCuda.cu:216-217)
-    int32_t c0;      // first child slot (internal), -1 leaf, -2 empty
+    int32_t c0;      // first child slot (internal), -1 leaf (empty slots are not written)
     int32_t mask;    // internal: bit q set = child c0+q holds bodies
     int32_t bs, be;  // body range [bs, be] in sorted order
     float ixx, ixy, iyy;  // second moments about the COM

@@ -3,8 +3,10 @@
 //
 // Same tree as the reference's level-by-level build (BarnesHutCuda.cu:153-240,
 // 713-777): a node at level L is a 2L-bit key prefix; it is internal when it
-// holds >= 2 bodies and L < 30, and an internal node materialises all four
-// child slots (empty ones included). The reference discovers the tree top-down
+// holds >= 2 bodies and L < 30, and an internal node owns four child slots;
+// its occupancy mask marks the non-empty ones, and every reader reaches a
+// child through that mask, so empty slots are never written (40 % of the
+// slots of a disk: the tree's largest write). The reference discovers the tree top-down
 // with one launch and one host round trip per level; here it is read directly
 // off the sorted keys:
 //   delta_i = number of leading equal base-4 digits of keys i and i+1 (<= 30)
@@ -43,21 +45,6 @@ __global__ __launch_bounds__(256) void k_tree_delta(const uint64_t* __restrict__
     const int hi = d < kMaxLevel - 1 ? d : kMaxLevel - 1;
     delta[i] = (int8_t)d;
     cnt[i] = hi > prev ? (uint32_t)(hi - prev) : 0u;
-}
-
-__device__ __forceinline__ void write_empty(NodeRec* r, double wsq) {
-    r->cx = 0.0;
-    r->cy = 0.0;
-    r->m = 0.0;
-    r->wsq = wsq;
-    r->c0 = -2;
-    r->mask = 0;
-    r->bs = -1;
-    r->be = -1;
-    r->ixx = 0.f;
-    r->ixy = 0.f;
-    r->iyy = 0.f;
-    r->mf = 0.f;
 }
 
 // Builds internal node p = (first body a, level L): its body range, the four
@@ -124,10 +111,7 @@ __device__ void build_node(uint32_t p, const uint64_t* __restrict__ keys, uint32
     for (int q = 0; q < 4; q++) {
         const uint32_t cs = bq[q], ce1 = bq[q + 1];
         NodeRec* r = &nodes[nsys + 4ull * p + q];
-        if (cs >= ce1) {
-            write_empty(r, cwsq);
-            continue;
-        }
+        if (cs >= ce1) continue;  // empty: bit q of the mask stays clear
         mask |= 1u << q;
         const uint32_t ce = ce1 - 1;
         if (cs == ce || L + 1 == kMaxLevel) {
