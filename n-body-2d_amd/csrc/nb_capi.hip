@@ -63,7 +63,7 @@ bool is_dk(const nb_ctx* c) { return c->p.integrator == NB_INTEG_DK; }
 
 uint64_t tree_bytes(uint64_t icap, uint64_t nsys) {
     const uint64_t ng = (icap + 255) / 256;
-    return (nsys + 4 * icap) * sizeof(NodeRec) + icap * (3 * sizeof(uint32_t) + 2) +
+    return (nsys + 4 * icap) * sizeof(NodeRec) + icap * (2 * sizeof(uint32_t) + 2 + sizeof(uint4)) +
            kMaxLevel * ng * sizeof(uint32_t);
 }
 

@@ -120,7 +120,7 @@ struct Buffers {
     uint32_t* node_a;      // [icap] first body of internal node p
     uint32_t* multi_list;  // [N/2 + 1] slots of multi-body leaves
     uint32_t* lvl_cnt;     // [kMaxLevel * ceil(icap/256)] level counts, then offsets
-    uint32_t* by_level;    // [icap] internal nodes sorted by level
+    uint4* by_level;       // [icap] internal nodes sorted by level: {id, slot, mask, 0}
     double2* acc;          // [N] accelerations (nb_compute_forces) / partial sums
     uint32_t* inter;       // [N] force terms per body
     double2* ap_part;      // all-pairs partial sums [split][N]

@@ -369,11 +369,15 @@ __global__ __launch_bounds__(256) void k_level_hist(const uint32_t* __restrict__
     if (threadIdx.x < kMaxLevel) lvl_cnt[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
 
+// Each entry carries what the level's reduction needs besides the child
+// records (node id, own slot, occupancy mask), so those are one load away.
 __global__ __launch_bounds__(256) void k_level_scatter(const uint32_t* __restrict__ meta,
                                                        uint64_t icap,
                                                        const uint8_t* __restrict__ inode_level,
+                                                       const uint8_t* __restrict__ inode_mask,
+                                                       const uint32_t* __restrict__ inode_slot,
                                                        const uint32_t* __restrict__ lvl_off,
-                                                       uint32_t* __restrict__ by_level) {
+                                                       uint4* __restrict__ by_level) {
     __shared__ uint32_t h[kMaxLevel];
     if (threadIdx.x < kMaxLevel) h[threadIdx.x] = 0;
     __syncthreads();
@@ -383,7 +387,8 @@ __global__ __launch_bounds__(256) void k_level_scatter(const uint32_t* __restric
     if (p >= I) return;
     const uint32_t L = inode_level[p];
     const uint32_t r = atomicAdd(&h[L], 1u);
-    by_level[lvl_off[L * gridDim.x + blockIdx.x] + r] = p;
+    by_level[lvl_off[L * gridDim.x + blockIdx.x] + r] =
+        make_uint4(p, inode_slot[p], (uint32_t)inode_mask[p], 0u);
 }
 
 // The internal nodes of one level: segment [lvl_off[L * nblk], lvl_off[(L+1) * nblk])
@@ -391,23 +396,21 @@ __global__ __launch_bounds__(256) void k_level_scatter(const uint32_t* __restric
 __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __restrict__ meta,
                                                    uint64_t icap, uint32_t nblk, uint32_t nsys,
                                                    const uint32_t* __restrict__ lvl_off,
-                                                   const uint32_t* __restrict__ by_level,
-                                                   const uint8_t* __restrict__ inode_mask,
-                                                   const uint32_t* __restrict__ inode_slot,
+                                                   const uint4* __restrict__ by_level,
                                                    NodeRec* __restrict__ nodes) {
-    if (tree_halted(meta, icap)) return;
-    const uint32_t I = meta[META_INTERNAL];
+    // the segment bounds are loaded alongside the halt check (in bounds either way)
     const uint32_t b = lvl_off[(uint64_t)L * nblk];
-    const uint32_t e = L + 1 < kMaxLevel ? lvl_off[(uint64_t)(L + 1) * nblk] : I;
+    const uint32_t e = L + 1 < kMaxLevel ? lvl_off[(uint64_t)(L + 1) * nblk] : meta[META_INTERNAL];
+    if (tree_halted(meta, icap)) return;
     for (uint32_t k = b + blockIdx.x * 256 + threadIdx.x; k < e; k += gridDim.x * 256) {
-        const uint32_t p = by_level[k];
-        const uint32_t mask = inode_mask[p];
+        const uint4 en = by_level[k];  // node id, own slot, occupancy mask
+        const uint32_t p = en.x, mask = en.z;
         // an empty slot holds exact zeros: skip its load, combine() adds the same zeros
         Com ch[4];
 #pragma unroll
         for (int q = 0; q < 4; q++)
             ch[q] = (mask >> q) & 1u ? load_com(&nodes[nsys + 4ull * p + q]) : Com{0.0, 0.0, 0.0, 0.f, 0.f, 0.f};
-        store_com(&nodes[inode_slot[p]], combine(ch), mask);
+        store_com(&nodes[en.y], combine(ch), mask);
     }
 }
 
@@ -436,14 +439,15 @@ void launch_tree_build(nb_ctx* c, int src) {
                        (uint64_t)c->icap, c->b.inode_level, c->b.lvl_cnt);
     launch_scan_u32(c, c->b.lvl_cnt, (uint32_t)(kMaxLevel * ng), nullptr);
     hipLaunchKernelGGL(k_level_scatter, dim3(ng), dim3(256), 0, c->stream, c->b.meta,
-                       (uint64_t)c->icap, c->b.inode_level, c->b.lvl_cnt, c->b.by_level);
+                       (uint64_t)c->icap, c->b.inode_level, c->b.inode_mask, c->b.inode_slot,
+                       c->b.lvl_cnt, c->b.by_level);
     // One launch per level, deepest first (a level's children are all done
     // when its launch starts).
     const uint32_t cg = ng < 512 ? ng : 512;
     for (int L = kMaxLevel - 1; L >= 0; L--) {
         hipLaunchKernelGGL(k_com_level, dim3(cg), dim3(256), 0, c->stream, L, c->b.meta,
                            (uint64_t)c->icap, ng, c->nsys, c->b.lvl_cnt, c->b.by_level,
-                           c->b.inode_mask, c->b.inode_slot, c->b.nodes);
+                           c->b.nodes);
     }
 }
 
