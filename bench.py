@@ -142,6 +142,7 @@ def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
     leg["interactions"] = total
     leg["interactions_per_body"] = total / n
     leg["tree_nodes"] = int(info.tree_nodes)
+    leg["internal_nodes"] = int(info.internal_nodes)
     if learned:
         leg["far_field_cells"] = int(info.far_field_cells)
         leg["far_field_cells_per_group"] = info.far_field_cells / ((n + 63) // 64)
@@ -265,11 +266,14 @@ def equal_error(points, learned) -> dict:
 
 def hbm_sort_build(leg, n: int) -> dict:
     """Sort (+ gather) and tree build against HBM peak: algorithmic bytes per
-    step over the phase's eager hipEvent time."""
+    step over the phase's eager hipEvent time. The build writes one record per
+    non-empty slot (internal nodes + leaves, at most internal + n); empty child
+    slots are never written."""
     ph = leg["phase_ms_per_step_eager"]
     out = {}
+    slots = leg["internal_nodes"] + n
     for name, nbytes in (("sort", SORT_BYTES_PER_BODY * n),
-                         ("build", BUILD_BYTES_PER_BODY * n + NODE_BYTES * leg["tree_nodes"])):
+                         ("build", BUILD_BYTES_PER_BODY * n + NODE_BYTES * slots)):
         tbs = nbytes / (ph[name] * 1e-3) / 1e12
         out[name] = {"bytes": nbytes, "ms": ph[name], "achieved_tb_s": tbs,
                      "peak_tb_s": HBM_PEAK_TBS, "frac": tbs / HBM_PEAK_TBS}
