@@ -15,6 +15,7 @@ import bench  # noqa: E402
 def _leg(**kw):
     leg = {"far_field": "monopole", "interactions": 700 * 2**20,
            "interactions_per_step_timed": 7.5e8, "tree_nodes": 3_000_000,
+           "internal_nodes": 750_000,
            "phase_ms_per_step_eager": {"sort": 0.25, "build": 0.4, "force": 0.6}}
     leg.update(kw)
     return leg
@@ -43,7 +44,8 @@ def test_sort_build_hbm_fractions():
     n = 1 << 20
     h = bench.hbm_sort_build(_leg(), n)
     assert h["sort"]["bytes"] == (8 * 24 + 92) * n
-    assert h["build"]["bytes"] == 32 * n + 64 * 3_000_000
+    # one 64-byte record per non-empty slot: at most internal nodes + bodies
+    assert h["build"]["bytes"] == 32 * n + 64 * (750_000 + n)
     assert h["sort"]["achieved_tb_s"] == pytest.approx(h["sort"]["bytes"] / 0.25e-3 / 1e12)
     assert h["build"]["frac"] == pytest.approx(h["build"]["achieved_tb_s"] / 8.0)
 
