@@ -6,9 +6,9 @@
 // holds >= 2 bodies and L < 30, and an internal node owns four child slots;
 // its occupancy mask marks the non-empty ones, and every reader reaches a
 // child through that mask, so empty slots are never written (40 % of the
-// slots of a disk: the tree's largest write). The reference discovers the tree top-down
-// with one launch and one host round trip per level; here it is read directly
-// off the sorted keys:
+// slots of a disk: the tree's largest write). The reference discovers the
+// tree top-down with one launch and one host round trip per level; here it
+// is read directly off the sorted keys:
 //   delta_i = number of leading equal base-4 digits of keys i and i+1 (<= 30)
 //   pair i owns the internal nodes at levels (delta_{i-1}, min(delta_i, 29)]
 //   (those are exactly the internal nodes whose first body is i)
