@@ -33,7 +33,7 @@ typedef struct nb_ctx nb_ctx; /* opaque context */
 
 enum nb_status {
     NB_OK = 0,
-    NB_ERR_INVALID_ARG = 1,   /* bad parameter (range checks of Config.cpp:283-312) */
+    NB_ERR_INVALID_ARG = 1,   /* bad parameter (range checks of Config.cpp:224-253) */
     NB_ERR_HIP = 2,           /* a HIP runtime call failed */
     NB_ERR_OUT_OF_MEMORY = 3,
     NB_ERR_OVERFLOW = 4,      /* traversal stack overflow reported by the kernel */
@@ -42,10 +42,34 @@ enum nb_status {
     NB_ERR_NUMERIC = 7        /* non-finite positions found while building the tree */
 };
 
-/* Force evaluation. BARNES_HUT follows BarnesHutCuda (square Morton cells,
- * side MAC, Plummer kernel); ALL_PAIRS is the direct sum of AllPairsSim
- * (AllPairs.cpp:38-49) with the Plummer kernel. */
+/* Force evaluation. BARNES_HUT is a quadtree walk (the cells, opening test
+ * and force law below pick the reference path it reproduces); ALL_PAIRS is
+ * the direct sum of AllPairsSim (AllPairs.cpp:38-49). */
 enum nb_algorithm { NB_ALGO_BARNES_HUT = 0, NB_ALGO_ALL_PAIRS = 1 };
+
+/* Pair force law. PLUMMER: G m M d / (d^2 + eps^2)^{3/2}, the CUDA kernel's
+ * (BarnesHutCuda.cu:427-431). REF_CPU: G m M / (d^2 + eps^2) along d / |d|,
+ * Simulation::force and body_to_quad_force of the CPU paths
+ * (Simulation.cpp:74-80, BarnesHut.cpp:130-135). The two agree at eps = 0. */
+enum nb_force_kernel { NB_FORCE_PLUMMER = 0, NB_FORCE_REF_CPU = 1 };
+
+/* Opening test of an internal cell at distance d from its centre of mass.
+ * SIDE: side^2 < theta^2 d^2 with d^2 = fma(dx, dx, dy * dy)
+ * (BarnesHutCuda.cu:413-415, 461-462). DIAG: (w^2 + h^2) / d^2 < theta^2 with
+ * d^2 = dx * dx + dy * dy, the cell's diagonal (BarnesHut.cpp:113-115); on a
+ * square cell DIAG at theta equals SIDE at theta / sqrt(2). */
+enum nb_mac { NB_MAC_SIDE = 0, NB_MAC_DIAG = 1 };
+
+/* Quadtree cells. SQUARE: the true bounding box squared about its shorter
+ * axis, split on 64-bit Morton digits (BarnesHutCuda.cu:54-132, 648-680).
+ * REF_CPU: the reference CPU quadtree (Quadtree.cpp:9-152): the box of
+ * get_universe_boundaries, whose `else if` lets body 0 never raise the
+ * maximum, kept rectangular, each cell split at its centre computed as the
+ * reference does (position + size / 2, halved sizes), centres of mass summed
+ * in the reference's order; leaves below level 30 hold one body, as in the
+ * reference (it has no depth cap: bodies closer than 2^-30 of the box stay
+ * together in one level-30 leaf here). Needs mac = DIAG and one system. */
+enum nb_cells { NB_CELLS_SQUARE = 0, NB_CELLS_REF_CPU = 1 };
 
 /* Update order. KD = kick then drift (BarnesHut.cpp:78-79, BarnesHutCuda.cpp:35-36);
  * DK = drift then kick (AllPairs.cpp:25-26). */
@@ -74,8 +98,8 @@ enum nb_traversal { NB_TRAV_EXACT = 0, NB_TRAV_GROUP = 1 };
 typedef struct nb_params {
     double G;           /* gravitational constant; the reference hard-codes
                            6.67430e-11 (Constants.hpp:17) */
-    double dt;          /* "timestep" (Config.cpp:89) */
-    double theta;       /* "theta" (Config.cpp:92); MAC: side^2 < theta^2 d^2 */
+    double dt;          /* "timestep" (Config.cpp:30) */
+    double theta;       /* "theta" (Config.cpp:33); opening test: enum nb_mac */
     double softening;   /* absolute Plummer epsilon; the reference derives it from
                            "softening_factor" (Simulation.cpp:82-126) */
     int32_t algorithm;  /* enum nb_algorithm */
@@ -94,7 +118,10 @@ typedef struct nb_params {
                               of a first walk at creation, at least 4096); a
                               step that needs more halts, the regions grow (at
                               least double) and the step runs again */
-    int32_t reserved[4];
+    int32_t force_kernel; /* enum nb_force_kernel */
+    int32_t mac;          /* enum nb_mac */
+    int32_t cells;        /* enum nb_cells */
+    int32_t reserved;
 } nb_params;
 
 typedef struct nb_info {
@@ -135,7 +162,8 @@ typedef struct nb_node {
 } nb_node;
 
 /* Fills *p with the defaults: G=1, dt=1e-3, theta=0.5, softening=0.01,
- * Barnes-Hut, KD, no self guard, monopole, graphs on, EXACT traversal. */
+ * Barnes-Hut, KD, no self guard, monopole, graphs on, EXACT traversal,
+ * Plummer force, SIDE test, SQUARE cells (the reference CUDA path). */
 int nb_params_default(nb_params* p);
 
 /* Replaces: the BarnesHutCuda constructor + init_device_resources
@@ -219,6 +247,13 @@ const char* nb_status_string(int status);
  * tree slot count. Any output pointer may be NULL. */
 int nb_debug_build(nb_ctx* ctx, double* box4, uint64_t* sorted_keys, uint32_t* orig_index,
                    uint64_t* node_count);
+
+/* Test hook: the context's next `count` device allocations fail as if the
+ * device were out of memory (regrowths of the node pool go through them).
+ * After a failed regrowth the context keeps its state for nb_get_state and
+ * snapshots, and every call that would run a step returns
+ * NB_ERR_OUT_OF_MEMORY. */
+int nb_debug_fail_alloc(nb_ctx* ctx, uint32_t count);
 
 /* Copies the tree built by the last step or nb_debug_build. */
 int nb_debug_tree(nb_ctx* ctx, nb_node* nodes, uint64_t cap);

@@ -47,6 +47,10 @@ template <class T>
 hipError_t dalloc(nb_ctx* c, T** p, uint64_t count) {
     uint64_t bytes = count * sizeof(T);
     if (bytes == 0) bytes = 256;
+    if (c->fail_allocs > 0) {  // nb_debug_fail_alloc
+        c->fail_allocs--;
+        return hipErrorOutOfMemory;
+    }
     const hipError_t e = hipMalloc((void**)p, bytes);
     if (e == hipSuccess) c->device_bytes += bytes;
     return e;
@@ -145,6 +149,8 @@ void free_all(nb_ctx* c) {
     dfree(b.sort_tot);
     dfree(b.box);
     dfree(b.bbox_part);
+    dfree(b.ref_pos);
+    dfree(b.ref_part);
     dfree(b.delta);
     dfree(b.cnt);
     dfree(b.scan_tmp);
@@ -188,9 +194,23 @@ int check_params(const nb_params* p, std::string* why) {
     if (p->cell_capacity < 0) return *why = "cell_capacity must be >= 0", NB_ERR_INVALID_ARG;
     if (p->far_field != NB_FAR_MONOPOLE && p->far_field != NB_FAR_LEARNED)
         return *why = "unknown far_field", NB_ERR_INVALID_ARG;
+    if (p->force_kernel != NB_FORCE_PLUMMER && p->force_kernel != NB_FORCE_REF_CPU)
+        return *why = "unknown force_kernel", NB_ERR_INVALID_ARG;
+    if (p->mac != NB_MAC_SIDE && p->mac != NB_MAC_DIAG)
+        return *why = "unknown mac", NB_ERR_INVALID_ARG;
+    if (p->cells != NB_CELLS_SQUARE && p->cells != NB_CELLS_REF_CPU)
+        return *why = "unknown cells", NB_ERR_INVALID_ARG;
+    if (p->cells == NB_CELLS_REF_CPU && p->mac != NB_MAC_DIAG)
+        return *why = "cells=REF_CPU needs mac=DIAG (its cells are rectangles)",
+               NB_ERR_NOT_SUPPORTED;
     if (p->far_field == NB_FAR_LEARNED &&
         (p->algorithm != NB_ALGO_BARNES_HUT || p->traversal != NB_TRAV_GROUP))
         return *why = "far_field=LEARNED needs algorithm=BARNES_HUT and traversal=GROUP",
+               NB_ERR_NOT_SUPPORTED;
+    if (p->far_field == NB_FAR_LEARNED &&
+        (p->force_kernel != NB_FORCE_PLUMMER || p->mac != NB_MAC_SIDE ||
+         p->cells != NB_CELLS_SQUARE))
+        return *why = "far_field=LEARNED needs the Plummer force, the SIDE test and SQUARE cells",
                NB_ERR_NOT_SUPPORTED;
     return NB_OK;
 }
@@ -312,20 +332,36 @@ int read_meta(nb_ctx* c) {
     return NB_OK;
 }
 
-// Grows whichever buffer the halted step overflowed.
+// A context whose regrowth failed: the step graphs are gone and buffers
+// they used are freed or missing, so nothing may launch on it any more.
+int mark_dead(nb_ctx* c, int rc) {
+    c->dead = true;
+    c->dead_msg = "context unusable after a failed buffer regrowth: " + c->err;
+    return rc;
+}
+
+int check_alive(nb_ctx* c) {
+    return c->dead ? fail(c, NB_ERR_OUT_OF_MEMORY, c->dead_msg) : NB_OK;
+}
+
+// Grows whichever buffer the halted step overflowed. The captured graphs
+// point at the buffers about to be freed, so they go first: a failure below
+// must not leave a graph that a later step could replay.
 int grow_for(nb_ctx* c, uint32_t flags) {
+    destroy_graphs(c);
     if (flags & FLAG_TREE_CAPACITY) {
         const uint64_t icap = c->last_internal + c->last_internal / 4 + 1024;
         int rc = alloc_tree(c, icap);
-        if (rc) return rc;
+        if (rc) return mark_dead(c, rc);
     }
     if (flags & FLAG_CELL_CAPACITY) {
         // at least double: the regions must grow for the step to converge
         const uint64_t need = c->host_meta[META_CELLS_NEEDED];
         const uint64_t cap = std::max<uint64_t>(2 * need + 64, 2ull * c->cell_cap);
-        if (cap > 0xffffffffull) return fail(c, NB_ERR_OUT_OF_MEMORY, "accepted-cell region");
+        if (cap > 0xffffffffull)
+            return mark_dead(c, fail(c, NB_ERR_OUT_OF_MEMORY, "accepted-cell region"));
         int rc = alloc_cells(c, (uint32_t)cap);
-        if (rc) return rc;
+        if (rc) return mark_dead(c, rc);
     }
     return build_graphs(c);
 }
@@ -334,6 +370,7 @@ int grow_for(nb_ctx* c, uint32_t flags) {
 // if requested, evaluates the forces without a kick. Grows the node pool
 // until the tree fits.
 int build_and_force(nb_ctx* c, bool forces) {
+    if (int rc = check_alive(c)) return rc;
     for (int attempt = 0; attempt < 4; attempt++) {
         int rc = clear_meta(c);
         if (rc) return rc;
@@ -409,6 +446,8 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
         if (p->algorithm != NB_ALGO_BARNES_HUT || p->traversal != NB_TRAV_GROUP)
             return fail(nullptr, NB_ERR_NOT_SUPPORTED,
                         "several systems need algorithm=BARNES_HUT and traversal=GROUP");
+        if (p->cells == NB_CELLS_REF_CPU)
+            return fail(nullptr, NB_ERR_NOT_SUPPORTED, "cells=REF_CPU holds one system");
         if (nsys > 65536) return fail(nullptr, NB_ERR_INVALID_ARG, "at most 65536 systems");
     }
     std::vector<uint32_t> off(nsys + 1);
@@ -497,6 +536,11 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
     NB_TRY(hipMemcpy(b.sys_off, off.data(), (nsys + 1) * sizeof(uint32_t),
                      hipMemcpyHostToDevice));
     NB_TRY(dalloc(c, &b.bbox_part, 4ull * c->bbox_blocks));
+    if (bh && p->cells == NB_CELLS_REF_CPU) {
+        c->ref_blocks = (uint32_t)std::max<uint64_t>((n + kRefBoxTile - 1) / kRefBoxTile, 1);
+        NB_TRY(dalloc(c, &b.ref_pos, n));
+        NB_TRY(dalloc(c, &b.ref_part, c->ref_blocks));
+    }
     NB_TRY(dalloc(c, &b.delta, bh ? n : 0));
     NB_TRY(dalloc(c, &b.cnt, bh ? n : 0));
     NB_TRY(dalloc(c, &b.scan_tmp, scan_blocks));
@@ -595,6 +639,7 @@ int nb_create_batch(const nb_params* p, uint32_t nsys, const uint64_t* offsets,
 
 int nb_step(nb_ctx* c, uint64_t k) {
     if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
+    if (int rc = check_alive(c)) return rc;
     if (c->n == 0) {
         c->iteration += k;
         c->sim_time += (double)k * c->dt;
@@ -778,6 +823,12 @@ int nb_compute_forces(nb_ctx* c, double* acc_xy, uint32_t* interactions, uint64_
     return NB_OK;
 }
 
+int nb_debug_fail_alloc(nb_ctx* c, uint32_t count) {
+    if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
+    c->fail_allocs = count;
+    return NB_OK;
+}
+
 int nb_set_dt(nb_ctx* c, double dt) {
     if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
     if (!std::isfinite(dt) || dt <= 0.0) return fail(c, NB_ERR_INVALID_ARG, "dt must be > 0");
@@ -815,6 +866,7 @@ int nb_get_timing(nb_ctx* c, double* phase_ms, int nphase, uint64_t* steps_timed
 
 int nb_autotune(nb_ctx* c, uint32_t repeats, nb_tuning* out) {
     if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
+    if (int rc = check_alive(c)) return rc;
     if (out) std::memset(out, 0, sizeof(*out));
     if (c->n == 0) return NB_OK;
     if (repeats == 0) repeats = 1;
@@ -1002,6 +1054,22 @@ int nb_debug_tree(nb_ctx* c, nb_node* nodes, uint64_t cap) {
     return NB_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The j split of the direct sum for m targets over n sources.
+uint32_t direct_sum_split(uint64_t n, uint64_t m) {
+    const uint32_t gx = (uint32_t)((m + 255) / 256);
+    uint32_t split = std::max<uint32_t>(1, std::min<uint32_t>((2048 + gx - 1) / gx, 256));
+    while (split > 1 && (n + split - 1) / split < 256) split--;
+    return split;
+}
+
+}  // namespace
+
+extern "C" {
+
 int nb_direct_sum_f64(uint64_t n, const double* pos_xy, const double* mass, double G,
                       double softening, uint64_t m, const int64_t* targets, double* acc_xy) {
     if ((n && (!pos_xy || !mass)) || (m && (!targets || !acc_xy)))
@@ -1015,9 +1083,7 @@ int nb_direct_sum_f64(uint64_t n, const double* pos_xy, const double* mass, doub
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(nullptr, NB_ERR_NO_DEVICE, "no HIP device");
-    const uint32_t gx = (uint32_t)((m + 255) / 256);
-    uint32_t split = std::max<uint32_t>(1, std::min<uint32_t>((2048 + gx - 1) / gx, 256));
-    while (split > 1 && (n + split - 1) / split < 256) split--;
+    const uint32_t split = direct_sum_split(n, m);
     double2 *dpos = nullptr, *dpart = nullptr;
     double* dmass = nullptr;
     int64_t* dt = nullptr;

@@ -7,12 +7,17 @@
 //   k_drift       x += v dt (update_positions, BarnesHutCuda.cu:489-495)
 //   k_direct64    FP64 direct sum, the reference for force errors
 //
-// Precision: the cell offset dx = com - x, its square d^2 = fma(dx, dx, dy*dy)
-// and the opening test wsq < theta^2 d^2 are evaluated in FP64 on the FP64
-// state, exactly the reference's expressions, so every body accepts the same
-// cells as the reference's per-body FP64 DFS. The interaction itself
-// (d^2 + eps^2, rsqrt, r^-3, m r^-3, the two products) runs in FP32 and is
-// summed per body with Kahan compensation; the kick v += a G dt is FP64.
+// Precision: the cell offset dx = com - x, its square d^2 and the opening
+// test are evaluated in FP64 on the FP64 state with exactly the expressions
+// of the reference path being reproduced, so every body accepts the same
+// cells as that path's per-body FP64 DFS: SIDE (the CUDA path)
+// d^2 = fma(dx, dx, dy*dy), wsq < theta^2 d^2; DIAG (the CPU path)
+// d^2 = dx*dx + dy*dy, (w^2 + h^2) / d^2 < theta^2, where each node's wsq
+// field holds its size^2 for the test in use. The interaction itself runs in
+// FP32 and is summed per body with Kahan compensation; the kick v += a G dt
+// is FP64. Force laws: PLUMMER m d (d^2 + eps^2)^{-3/2}; REF_CPU
+// m (d / |d|) / (d^2 + eps^2) (Simulation.cpp:74-80), whose zero-distance
+// terms (a body's own entry) are dropped.
 #include "nb_device.h"
 #include "nb_internal.h"
 
@@ -30,15 +35,48 @@ struct Acc {
     uint32_t n = 0;
 };
 
-// One softened monopole term m d / (d^2 + eps^2)^{3/2}.
+// Magnitude factor s of one term s * d (FP32) for the squared distance d2:
+// PLUMMER m / (d2 + eps^2)^{3/2}; REF_CPU m / (|d| (d2 + eps^2)), 0 at d2 = 0.
+template <bool REFF>
+__device__ __forceinline__ float term_scale(float d2, float m, float eps2f) {
+    if (REFF) {
+        const float s = m * __builtin_amdgcn_rsqf(d2) * __builtin_amdgcn_rcpf(d2 + eps2f);
+        return d2 > 0.f ? s : 0.f;
+    }
+    const float rs = __builtin_amdgcn_rsqf(d2 + eps2f);
+    return m * (rs * rs * rs);
+}
+
+// One monopole term of the force law.
+template <bool REFF>
 __device__ __forceinline__ void interact(Acc& a, double dx, double dy, double d2, float m,
                                          float eps2f) {
-    const float r2 = (float)d2 + eps2f;
-    const float rs = __builtin_amdgcn_rsqf(r2);
-    const float s = m * (rs * rs * rs);
+    const float s = term_scale<REFF>((float)d2, m, eps2f);
     kahan_add(a.ax, a.cx, (float)dx * s);
     kahan_add(a.ay, a.cy, (float)dy * s);
     a.n++;
+}
+
+// d^2 = dx*dx + dy*dy rounded as written, without a multiply-add: the
+// reference CPU's lengthSquared (BarnesHut.cpp:113-114).
+__device__ __forceinline__ double dist2_plain(double dx, double dy) {
+#pragma clang fp contract(off)
+    const double a = dx * dx;
+    const double b = dy * dy;
+    return a + b;
+}
+
+// Squared distance and opening test of one path. SIDE: the CUDA path's
+// fma(dx, dx, dy*dy) and wsq < theta^2 d^2 (BarnesHutCuda.cu:413-415, 462);
+// DIAG: the CPU path's plain d^2 and size^2 / d^2 < theta^2 (BarnesHut.cpp:113-115).
+template <bool DIAG>
+__device__ __forceinline__ double dist2(double dx, double dy) {
+    return DIAG ? dist2_plain(dx, dy) : fma(dx, dx, dy * dy);
+}
+
+template <bool DIAG>
+__device__ __forceinline__ bool opening_accepts(double size_sq, double d2, double theta_sq) {
+    return DIAG ? (size_sq / d2 < theta_sq) : (size_sq < theta_sq * d2);
 }
 
 constexpr int kTravThreads = 256;
@@ -53,7 +91,7 @@ constexpr int kTravWaves = kTravThreads / 64;
 // wave fetches each node once (a wave-uniform scalar load).
 // Masked walk of one wavefront (see above). s_node/s_mask: kStackDepth
 // entries of LDS owned by this wave. Returns false on stack overflow.
-template <bool GUARD>
+template <bool GUARD, bool DIAG, bool REFF>
 __device__ __forceinline__ bool exact_walk(const NodeRec* __restrict__ nodes,
                                            const double2* __restrict__ pos,
                                            const double* __restrict__ mass, double2 p,
@@ -74,27 +112,30 @@ __device__ __forceinline__ bool exact_walk(const NodeRec* __restrict__ nodes,
         const int32_t c0 = r->c0, bs = r->bs, be = r->be;
         const bool mine = (m >> l) & 1ull;
         const double dx = rcx - p.x, dy = rcy - p.y;
-        const double d2 = fma(dx, dx, dy * dy);
+        const double d2 = dist2<DIAG>(dx, dy);
         if (c0 < 0) {
-            // Leaf (empty slots are never pushed). BarnesHutCuda.cu:421-458.
+            // Leaf (empty slots are never pushed). BarnesHutCuda.cu:421-458; a
+            // single-body leaf other than the body itself never sits at its
+            // position (equal positions share one leaf), so the index test is
+            // also the CPU path's position test (BarnesHut.cpp:108).
             if (mine) {
                 if (bs == be) {
-                    if (bs != me) interact(a, dx, dy, d2, r->mf, eps2f);
+                    if (bs != me) interact<REFF>(a, dx, dy, d2, r->mf, eps2f);
                 } else if (me < bs || me > be) {
-                    interact(a, dx, dy, d2, r->mf, eps2f);
+                    interact<REFF>(a, dx, dy, d2, r->mf, eps2f);
                 } else {
                     for (int32_t j = bs; j <= be; j++) {
                         if (j == me) continue;
                         const double2 q = pos[j];
                         const double ex = q.x - p.x, ey = q.y - p.y;
-                        interact(a, ex, ey, fma(ex, ex, ey * ey), (float)mass[j], eps2f);
+                        interact<REFF>(a, ex, ey, dist2<DIAG>(ex, ey), (float)mass[j], eps2f);
                     }
                 }
             }
         } else {
-            bool accept = mine && (r->wsq < theta_sq * d2);
+            bool accept = mine && opening_accepts<DIAG>(r->wsq, d2, theta_sq);
             if (GUARD) accept = accept && !(me >= bs && me <= be);
-            if (accept) interact(a, dx, dy, d2, r->mf, eps2f);
+            if (accept) interact<REFF>(a, dx, dy, d2, r->mf, eps2f);
             const uint64_t open = ballot64(mine && !accept);
             if (open) {
                 if (sp + 4 > kStackDepth) return false;
@@ -113,7 +154,7 @@ __device__ __forceinline__ bool exact_walk(const NodeRec* __restrict__ nodes,
     return true;
 }
 
-template <bool KICK, bool GUARD>
+template <bool KICK, bool GUARD, bool DIAG, bool REFF>
 __global__ __launch_bounds__(kTravThreads) void k_traverse(
     const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
     const double* __restrict__ mass, double2* __restrict__ vel, double2* __restrict__ acc_out,
@@ -130,8 +171,9 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
     if (ballot64(valid) == 0) return;
     const double2 p = valid ? pos[i] : make_double2(0.0, 0.0);
     Acc a;
-    const bool ok = exact_walk<GUARD>(nodes, pos, mass, p, (int32_t)i, valid, k->theta_sq,
-                                      (float)k->eps2, s_node[w], s_mask[w], a);
+    const bool ok = exact_walk<GUARD, DIAG, REFF>(nodes, pos, mass, p, (int32_t)i, valid,
+                                                  k->theta_sq, (float)k->eps2, s_node[w],
+                                                  s_mask[w], a);
     if (!ok && lane_id() == 0) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_STACK);
     if (!valid) return;
     const double ax = (double)a.ax - (double)a.cx;
@@ -195,7 +237,7 @@ constexpr uint32_t kGMinSteps = 96;
 constexpr uint32_t kGWalkSteps = 20;
 constexpr uint32_t kGWavesPerCU = 13;  // LDS-bound residency of k_traverse_group
 
-template <bool ZERO_EPS>
+template <bool ZERO_EPS, bool REFF>
 __device__ __forceinline__ void eval_list(const float4* __restrict__ list, int cnt, float bx,
                                           float by, float eps2f, Acc& a) {
     float px = 0.f, py = 0.f;
@@ -203,10 +245,17 @@ __device__ __forceinline__ void eval_list(const float4* __restrict__ list, int c
     for (int e = 0; e < cnt; e++) {
         const float4 d = list[e];
         const float dx = d.x - bx, dy = d.y - by;
-        const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2f));
-        float rs = __builtin_amdgcn_rsqf(r2);
-        if (ZERO_EPS) rs = r2 > 0.f ? rs : 0.f;
-        const float w = d.z * (rs * rs * rs);
+        float w;
+        if (REFF) {
+            const float d2 = fmaf(dx, dx, dy * dy);
+            const float s = d.z * __builtin_amdgcn_rsqf(d2) * __builtin_amdgcn_rcpf(d2 + eps2f);
+            w = d2 > 0.f ? s : 0.f;  // the body's own entry
+        } else {
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2f));
+            float rs = __builtin_amdgcn_rsqf(r2);
+            if (ZERO_EPS) rs = r2 > 0.f ? rs : 0.f;
+            w = d.z * (rs * rs * rs);
+        }
         px = fmaf(dx, w, px);
         py = fmaf(dy, w, py);
     }
@@ -273,7 +322,7 @@ __device__ __forceinline__ bool queue_pop(uint32_t* meta, uint64_t* queue, uint3
 // per-group counter, each member its subgroup and its offset from that
 // centre, and the raw acceleration (without G) goes to acc_out instead of a
 // kick.
-template <bool KICK, bool ZERO_EPS, bool GUARD, bool LEARN>
+template <bool KICK, bool ZERO_EPS, bool GUARD, bool LEARN, bool REFF>
 __global__ __launch_bounds__(64) void k_traverse_group(
     const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
     const double* __restrict__ mass, double2* __restrict__ vel, double2* __restrict__ acc_out,
@@ -287,6 +336,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     __shared__ __attribute__((aligned(16))) int32_t s_stack[kGStack];
     __shared__ float4 s_list[kGList];
     __shared__ int32_t s_todo[16];
+    __shared__ unsigned long long s_own;  // lanes whose own single-body leaf is listed
     const int l = lane_id();
     const uint32_t home = xcd_remap(blockIdx.x, gridDim.x);
     if (count_step && blockIdx.x == 0 && l == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
@@ -379,7 +429,8 @@ __global__ __launch_bounds__(64) void k_traverse_group(
             int sp = 1, cnt = 0;
             uint32_t ccnt = 0;  // cells of this subgroup written after cbase
             uint32_t listed = 0, steps = 0;
-            bool own_multi = false, split = false;
+            bool split = false;
+            if (l == 0) s_own = 0ull;
             if (l == 0) s_stack[0] = root;  // the root of this group's system
             // Each step pops up to 64 stack entries and loads their records; the
             // next step's records are loaded before the list is flushed, so
@@ -422,6 +473,9 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                 if (add) {
                     const int at = cnt + (int)__popcll(am & lanemask_lt());
                     s_list[at] = make_float4((float)dxc, (float)dyc, r.mf, 0.f);
+                    // a member's own single-body leaf: a zero term, not counted
+                    if (c0 < 0 && bs >= g_first && bs <= g_last)
+                        atomicOr(&s_own, 1ull << (bs - (int32_t)first));
                 }
                 cnt += (int)__popcll(am);
                 if (LEARN) {
@@ -451,12 +505,11 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                     const int src = __ffsll((unsigned long long)mm) - 1;
                     mm &= mm - 1;
                     const int32_t js = __shfl(bs, src, 64), je = __shfl(be, src, 64);
-                    own_multi |= (me >= js && me <= je);
                     for (int32_t j = js; j <= je; j++) {
                         if (j == me) continue;
                         const double2 q = pos[j];
                         const double ex = q.x - p.x, ey = q.y - p.y;
-                        interact(t, ex, ey, fma(ex, ex, ey * ey), (float)mass[j], eps2f);
+                        interact<REFF>(t, ex, ey, fma(ex, ex, ey * ey), (float)mass[j], eps2f);
                     }
                 }
                 if (capped && (listed + (uint32_t)cnt > (uint32_t)budget || steps > max_steps) &&
@@ -470,7 +523,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                     break;
                 }
                 if (cnt > kGFlush) {
-                    eval_list<ZERO_EPS>(s_list, cnt, bx, by, eps2f, t);
+                    eval_list<ZERO_EPS, REFF>(s_list, cnt, bx, by, eps2f, t);
                     listed += cnt;
                     cnt = 0;
                 }
@@ -497,7 +550,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                 continue;
             }
             if (cnt > 0) {
-                eval_list<ZERO_EPS>(s_list, cnt, bx, by, eps2f, t);
+                eval_list<ZERO_EPS, REFF>(s_list, cnt, bx, by, eps2f, t);
                 listed += cnt;
             }
             // the subgroup is finished: its members write their results
@@ -519,8 +572,8 @@ __global__ __launch_bounds__(64) void k_traverse_group(
             if (member) {
                 const double ax = (double)t.ax - (double)t.cx;
                 const double ay = (double)t.ay - (double)t.cy;
-                // a body's own single-body leaf is in its group's list (a zero term)
-                const uint32_t terms = listed - (own_multi ? 0u : 1u) + t.n;
+                // the body's own single-body leaf, when listed, is a zero term
+                const uint32_t terms = listed - (uint32_t)((s_own >> l) & 1ull) + t.n;
                 if (LEARN) {
                     acc_out[i] = make_double2(ax, ay);
                     if (!KICK) inter_out[i] = terms;
@@ -569,65 +622,98 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
         const uint32_t max_steps = (uint32_t)std::min(
             std::max((double)kGMinSteps * grow, (double)(rounds * kGWalkSteps)), 1e9);
         const bool learn = c->p.far_field == NB_FAR_LEARNED;
-#define NB_TRAVG(K, Z, GD, LN)                                                                 \
-    hipLaunchKernelGGL((k_traverse_group<K, Z, GD, LN>), dim3(groups), dim3(64), 0, c->stream, \
-                       c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src], c->b.acc,     \
-                       c->b.inter, n, c->b.consts, c->b.meta, (int)(count_step && !learn),     \
-                       budget, max_steps, c->b.keys[0], c->b.cell_slot, c->b.cell_cnt,         \
-                       c->cell_cap, c->b.subs, c->b.body_w, c->b.queue, c->queue_cap,          \
-                       c->b.grp_first,                                                          \
-                       c->b.grp_info)
-#define NB_TRAVG3(K, Z, GD) if (learn) NB_TRAVG(false, Z, GD, true); else NB_TRAVG(K, Z, GD, false)
-#define NB_TRAVG2(K, Z) if (gd) { NB_TRAVG3(K, Z, true); } else { NB_TRAVG3(K, Z, false); }
-        if (kick) {
-            if (z) { NB_TRAVG2(true, true); } else { NB_TRAVG2(true, false); }
+        const bool reff = c->p.force_kernel == NB_FORCE_REF_CPU;
+#define NB_TRAVG(K, Z, GD, LN, RF)                                                             \
+    hipLaunchKernelGGL((k_traverse_group<K, Z, GD, LN, RF>), dim3(groups), dim3(64), 0,        \
+                       c->stream, c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src],    \
+                       c->b.acc, c->b.inter, n, c->b.consts, c->b.meta,                        \
+                       (int)(count_step && !learn), budget, max_steps, c->b.keys[0],           \
+                       c->b.cell_slot, c->b.cell_cnt, c->cell_cap, c->b.subs, c->b.body_w,     \
+                       c->b.queue, c->queue_cap, c->b.grp_first, c->b.grp_info)
+#define NB_TRAVG_GD(K, Z, LN, RF)                                                              \
+    {                                                                                          \
+        if (gd)                                                                                \
+            NB_TRAVG(K, Z, true, LN, RF);                                                      \
+        else                                                                                   \
+            NB_TRAVG(K, Z, false, LN, RF);                                                     \
+    }
+        // the learned far field runs with the Plummer law (checked at creation);
+        // REF_CPU drops zero-distance terms itself, whatever eps is
+        if (learn) {
+            if (z) NB_TRAVG_GD(false, true, true, false) else NB_TRAVG_GD(false, false, true, false)
+        } else if (reff) {
+            if (kick) NB_TRAVG_GD(true, false, false, true) else NB_TRAVG_GD(false, false, false, true)
+        } else if (kick) {
+            if (z) NB_TRAVG_GD(true, true, false, false) else NB_TRAVG_GD(true, false, false, false)
         } else {
-            if (z) { NB_TRAVG2(false, true); } else { NB_TRAVG2(false, false); }
+            if (z) NB_TRAVG_GD(false, true, false, false) else NB_TRAVG_GD(false, false, false, false)
         }
-#undef NB_TRAVG3
-#undef NB_TRAVG2
+#undef NB_TRAVG_GD
 #undef NB_TRAVG
         return;
     }
     const uint32_t blocks = (n + kTravThreads - 1) / kTravThreads;
     const bool guard = c->p.self_guard != 0;
-#define NB_TRAV(K, G)                                                                          \
-    hipLaunchKernelGGL((k_traverse<K, G>), dim3(blocks), dim3(kTravThreads), 0, c->stream,     \
-                       c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src], c->b.acc,     \
-                       c->b.inter, n, c->b.consts, c->b.meta, (int)count_step)
-    if (kick) {
-        if (guard) NB_TRAV(true, true); else NB_TRAV(true, false);
-    } else {
-        if (guard) NB_TRAV(false, true); else NB_TRAV(false, false);
+    const bool diag = c->p.mac == NB_MAC_DIAG;
+    const bool reff = c->p.force_kernel == NB_FORCE_REF_CPU;
+#define NB_TRAV(K, G, D, R)                                                                    \
+    hipLaunchKernelGGL((k_traverse<K, G, D, R>), dim3(blocks), dim3(kTravThreads), 0,          \
+                       c->stream, c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src],    \
+                       c->b.acc, c->b.inter, n, c->b.consts, c->b.meta, (int)count_step)
+#define NB_TRAV_R(K, G, D)                                                                     \
+    {                                                                                          \
+        if (reff)                                                                              \
+            NB_TRAV(K, G, D, true);                                                            \
+        else                                                                                   \
+            NB_TRAV(K, G, D, false);                                                           \
     }
+#define NB_TRAV_D(K, G)                                                                        \
+    {                                                                                          \
+        if (diag)                                                                              \
+            NB_TRAV_R(K, G, true) else NB_TRAV_R(K, G, false)                                  \
+    }
+#define NB_TRAV_G(K)                                                                           \
+    {                                                                                          \
+        if (guard)                                                                             \
+            NB_TRAV_D(K, true) else NB_TRAV_D(K, false)                                        \
+    }
+    if (kick) NB_TRAV_G(true) else NB_TRAV_G(false)
+#undef NB_TRAV_G
+#undef NB_TRAV_D
+#undef NB_TRAV_R
 #undef NB_TRAV
 }
 
 // ---------------------------------------------------------------------------
 // All-pairs: each thread owns R bodies, the workgroup streams 256-body source
-// tiles through LDS as FP32 {x, y, m}; the j range is split across
-// blockIdx.y so a 262,144-body launch still fills 256 CUs. Each 256-term tile
-// is summed plainly, and tile sums are added to the body's running total with
-// Kahan compensation.
+// tiles through LDS; the j range is split across blockIdx.y so a
+// 262,144-body launch still fills 256 CUs. Positions stay FP64 (in LDS and
+// registers) and each offset dx = x_j - x_i is one FP64 subtraction rounded
+// once to FP32: accurate to half an FP32 ulp of dx itself however close two
+// bodies are (rounding the absolute positions to FP32 first leaves an error
+// of ~6e-8 |x| / |dx|, which at eps = 0 dominates close pairs). Without
+// packed FP32 an FP64 add issues at the FP32 rate, so this costs two VALU
+// operations per pair (the two conversions) over the FP32 subtraction. Each
+// 256-term tile is summed plainly, and tile sums are added to the body's
+// running total with Kahan compensation.
 // ---------------------------------------------------------------------------
 constexpr int kApThreads = 256;
 
-template <bool ZERO_EPS, int kApR>
+template <bool ZERO_EPS, bool REFF, int kApR>
 __global__ __launch_bounds__(kApThreads) void k_allpairs(const double2* __restrict__ pos,
                                                          const double* __restrict__ mass,
                                                          uint32_t n, uint32_t chunk,
                                                          const StepConsts* __restrict__ k,
                                                          double2* __restrict__ part) {
-    __shared__ float4 tile[kApThreads];
+    __shared__ double2 tile[kApThreads];
+    __shared__ float tmass[kApThreads];
     const uint32_t ib = blockIdx.x * (kApThreads * kApR) + threadIdx.x;
-    float xi[kApR], yi[kApR];
+    double2 pi[kApR];
     float ax[kApR], ay[kApR], cx[kApR], cy[kApR];
 #pragma unroll
     for (int r = 0; r < kApR; r++) {
         const uint32_t i = ib + r * kApThreads;
-        const double2 p = i < n ? pos[i] : make_double2(0.0, 0.0);
-        xi[r] = (float)p.x;
-        yi[r] = (float)p.y;
+        pi[r] = i < n ? pos[i] : make_double2(0.0, 0.0);
         ax[r] = ay[r] = cx[r] = cy[r] = 0.f;
     }
     const float eps2f = (float)k->eps2;
@@ -636,10 +722,11 @@ __global__ __launch_bounds__(kApThreads) void k_allpairs(const double2* __restri
     for (uint32_t jt = j0; jt < j1; jt += kApThreads) {
         const uint32_t j = jt + threadIdx.x;
         if (j < j1) {
-            const double2 q = pos[j];
-            tile[threadIdx.x] = make_float4((float)q.x, (float)q.y, (float)mass[j], 0.f);
+            tile[threadIdx.x] = pos[j];
+            tmass[threadIdx.x] = (float)mass[j];
         } else {
-            tile[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+            tile[threadIdx.x] = make_double2(0.0, 0.0);
+            tmass[threadIdx.x] = 0.f;
         }
         __syncthreads();
         float px[kApR], py[kApR];
@@ -647,14 +734,24 @@ __global__ __launch_bounds__(kApThreads) void k_allpairs(const double2* __restri
         for (int r = 0; r < kApR; r++) px[r] = py[r] = 0.f;
 #pragma unroll 8
         for (int jj = 0; jj < kApThreads; jj++) {
-            const float4 s = tile[jj];
+            const double2 s = tile[jj];
+            const float mj = tmass[jj];
 #pragma unroll
             for (int r = 0; r < kApR; r++) {
-                const float dx = s.x - xi[r], dy = s.y - yi[r];
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2f));
-                float rs = __builtin_amdgcn_rsqf(r2);
-                if (ZERO_EPS) rs = r2 > 0.f ? rs : 0.f;
-                const float w = s.z * (rs * rs * rs);
+                const float dx = (float)(s.x - pi[r].x);
+                const float dy = (float)(s.y - pi[r].y);
+                float w;
+                if (REFF) {  // Simulation::force (Simulation.cpp:74-80)
+                    const float d2 = fmaf(dx, dx, dy * dy);
+                    const float t =
+                        mj * __builtin_amdgcn_rsqf(d2) * __builtin_amdgcn_rcpf(d2 + eps2f);
+                    w = d2 > 0.f ? t : 0.f;  // the body itself
+                } else {
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2f));
+                    float rs = __builtin_amdgcn_rsqf(r2);
+                    if (ZERO_EPS) rs = r2 > 0.f ? rs : 0.f;
+                    w = mj * (rs * rs * rs);
+                }
                 px[r] = fmaf(dx, w, px[r]);
                 py[r] = fmaf(dy, w, py[r]);
             }
@@ -711,10 +808,20 @@ void launch_allpairs(nb_ctx* c, int src, bool kick, bool count_step) {
     uint32_t chunk = (n + split - 1) / split;
     chunk = (chunk + kApThreads - 1) / kApThreads * kApThreads;
     const bool z = !(c->p.softening > 0.0);
-#define NB_AP(Z, RR)                                                                           \
-    hipLaunchKernelGGL((k_allpairs<Z, RR>), dim3(ib, split), dim3(kApThreads), 0, c->stream,   \
-                       c->b.pos[src], c->b.mass[src], n, chunk, c->b.consts, c->b.ap_part)
-#define NB_AP2(RR) if (z) NB_AP(true, RR); else NB_AP(false, RR)
+    const bool reff = c->p.force_kernel == NB_FORCE_REF_CPU;
+#define NB_AP(Z, RF, RR)                                                                       \
+    hipLaunchKernelGGL((k_allpairs<Z, RF, RR>), dim3(ib, split), dim3(kApThreads), 0,          \
+                       c->stream, c->b.pos[src], c->b.mass[src], n, chunk, c->b.consts,        \
+                       c->b.ap_part)
+#define NB_AP2(RR)                                                                             \
+    {                                                                                          \
+        if (reff)                                                                              \
+            NB_AP(false, true, RR);                                                            \
+        else if (z)                                                                            \
+            NB_AP(true, false, RR);                                                            \
+        else                                                                                   \
+            NB_AP(false, false, RR);                                                           \
+    }
     if (R == 2) {
         NB_AP2(2);
     } else if (R == 8) {
@@ -768,7 +875,8 @@ __global__ __launch_bounds__(256) void k_direct64(const double2* __restrict__ po
                                                   double2* __restrict__ part) {
     __shared__ double3 tile[256];
     const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    const int64_t ti = t < m ? tgt[t] : -1;
+    int64_t ti = t < m ? tgt[t] : -1;
+    if (ti >= (int64_t)n) ti = -1;  // out of range: never read (finished as NaN)
     const double2 p = ti >= 0 ? pos[ti] : make_double2(0.0, 0.0);
     double sx = 0.0, sy = 0.0, cx = 0.0, cy = 0.0;
     const uint32_t j0 = blockIdx.y * chunk, j1 = min(j0 + chunk, n);

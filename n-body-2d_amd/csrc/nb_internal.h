@@ -33,6 +33,7 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;         // keys per thread per tile
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kBoxStride = 8;          // doubles of box data per system
+constexpr int kRefBoxTile = 2048;      // bodies per block of the REF_CPU box scan
 // Group traversal: list entries after which a group is split and walked again
 // (nb_params.group_budget = 0). Splits restart walks from the root, so a low
 // budget costs more than it saves once a simulation has evolved: on a
@@ -103,13 +104,16 @@ struct Buffers {
     uint32_t* vals[2];
     uint32_t* sort_hist;   // [bins][tiles]
     uint32_t* sort_tot;    // [bins]
-    double* box;           // per system [8]: x_min, x_max, y_min, y_max, root_wsq
+    double* box;           // per system [8]: x_min, x_max, y_min, y_max, root size^2 of
+                           //     the opening test (side^2, or w^2 + h^2 for DIAG)
     uint32_t* sys_of;      // [N] system of the body at position i (positions of a
                            //     system never change: sorting permutes inside it)
     uint32_t* sys_off;     // [nsys + 1] first position of each system
     uint32_t* grp_first;   // [ngroups] first body of traversal group g
     uint32_t* grp_info;    // [ngroups] (system << 8) | bodies (1..64)
     double* bbox_part;     // [blocks][4]
+    double2* ref_pos;      // [N] positions in the caller's order (REF_CPU cells' box)
+    double4* ref_part;     // [ref blocks] {min x, min y, max x, max y} per block of it
     int8_t* delta;         // [N] common-prefix digit count of pair (i, i+1)
     uint32_t* cnt;         // [N] internal nodes owned by pair i, then its offset
     uint32_t* scan_tmp;    // block sums of the scans
@@ -161,6 +165,7 @@ struct nb_ctx {
     int cur = 0;                 // which copy holds the state
     uint64_t icap = 0;           // internal-node capacity
     uint32_t bbox_blocks = 0;
+    uint32_t ref_blocks = 0;     // blocks of the REF_CPU box scan (kRefBoxTile bodies each)
     uint32_t sort_tiles = 0;
     uint32_t scan_blocks = 0;
     uint32_t ap_split = 1;
@@ -196,6 +201,12 @@ struct nb_ctx {
     } snap[2];
     hipStream_t copy_stream = nullptr;
     uint64_t snap_next = 1;
+    // A failed regrowth leaves device buffers missing: the context then only
+    // serves its state (nb_get_state, snapshots) and every call that would
+    // run a step returns this error.
+    bool dead = false;
+    std::string dead_msg;
+    uint32_t fail_allocs = 0;       // test hook: allocations left to fail
     std::string err;
 };
 

@@ -749,7 +749,8 @@ int nb_mlp_generate(nb_ctx* c, double theta, uint32_t sample_bodies, uint64_t ma
     uint32_t* total = b.sample_off + c->n + 1;
     MLP_HIP(c, hipMemsetAsync(total, 0, sizeof(uint32_t), c->stream));
     const uint32_t waves = std::max<uint32_t>(1, (samples + 63) / 64);
-    const uint32_t cap = (uint32_t)c->pair_cap;
+    // this call's limit: the buffer may be larger from an earlier call
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(c->pair_cap, max_pairs);
     hipLaunchKernelGGL(k_mlp_collect<false>, dim3(waves), dim3(64), 0, c->stream, b.nodes,
                        b.pos[c->cur], (uint32_t)c->n, samples, seed, theta * theta,
                        b.sample_off, cap, b.pair_body, b.pair_slot);
@@ -760,11 +761,11 @@ int nb_mlp_generate(nb_ctx* c, double theta, uint32_t sample_bodies, uint64_t ma
     hipLaunchKernelGGL(k_mlp_targets, dim3(2048), dim3(256), 0, c->stream, b.nodes,
                        b.pos[c->cur], b.mass[c->cur], b.box,
                        c->p.softening * c->p.softening, b.pair_body, b.pair_slot, total,
-                       (uint64_t)c->pair_cap, b.pair_x, b.pair_t);
+                       (uint64_t)cap, b.pair_x, b.pair_t);
     MLP_HIP(c, hipGetLastError());
     uint32_t cnt = 0;
     MLP_HIP(c, scopy(c, &cnt, total, sizeof(cnt), hipMemcpyDeviceToHost));
-    c->pair_count = std::min<uint64_t>(cnt, c->pair_cap);
+    c->pair_count = std::min<uint64_t>(cnt, cap);
     if (pairs_out) *pairs_out = c->pair_count;
     return NB_OK;
 }

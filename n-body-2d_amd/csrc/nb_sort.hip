@@ -62,9 +62,16 @@ __global__ __launch_bounds__(kBboxThreads) void k_bbox_partial(const double2* __
     }
 }
 
+// Root size^2 of the opening test: side^2 (SIDE), or w^2 + h^2 = 2 side^2 on
+// a square cell (DIAG; exact, as the reference's lengthSquared of (s, s)).
+__device__ __forceinline__ double root_size_sq(double w, int diag) {
+    const double s2 = w * w;
+    return diag ? s2 + s2 : s2;
+}
+
 __global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ part, uint32_t nparts,
                                                    double* __restrict__ box,
-                                                   uint32_t* __restrict__ meta) {
+                                                   uint32_t* __restrict__ meta, int diag) {
     double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
     for (uint32_t k = threadIdx.x; k < nparts; k += 64) {
         xmin = fmin(xmin, part[4 * k + 0]);
@@ -95,8 +102,7 @@ __global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ pa
         box[1] = xmax;
         box[2] = ymin;
         box[3] = ymax;
-        const double w = xmax - xmin;
-        box[4] = w * w;  // root width_sq (BarnesHutCuda.cu:717)
+        box[4] = root_size_sq(xmax - xmin, diag);  // root width_sq (BarnesHutCuda.cu:717)
         meta[META_MULTI_LEAVES] = 0;  // the tree build of this step queues from 0
         meta[META_QHEAD] = 0;         // the traversal work queue of this step starts empty
         meta[META_QTAIL] = 0;
@@ -110,7 +116,8 @@ __global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ pa
 __global__ __launch_bounds__(kBboxThreads) void k_bbox_sys(const double2* __restrict__ pos,
                                                            const uint32_t* __restrict__ sys_off,
                                                            double* __restrict__ box,
-                                                           uint32_t* __restrict__ meta) {
+                                                           uint32_t* __restrict__ meta,
+                                                           int diag) {
     const uint32_t s = blockIdx.x;
     const uint32_t a = sys_off[s], e = sys_off[s + 1];
     double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
@@ -159,8 +166,7 @@ __global__ __launch_bounds__(kBboxThreads) void k_bbox_sys(const double2* __rest
         bx[1] = xmax;
         bx[2] = ymin;
         bx[3] = ymax;
-        const double wd = xmax - xmin;
-        bx[4] = wd * wd;
+        bx[4] = root_size_sq(xmax - xmin, diag);
         if (s == 0) {
             meta[META_MULTI_LEAVES] = 0;
             meta[META_QHEAD] = 0;
@@ -171,16 +177,223 @@ __global__ __launch_bounds__(kBboxThreads) void k_bbox_sys(const double2* __rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// Reference CPU box (get_universe_boundaries, Quadtree.cpp:9-31), scanned in
+// the caller's body order as the reference does. The minimum is the true
+// one; body i can raise the maximum only when it did not lower the minimum
+// (the `else if`), i.e. when !(x_i < min(DBL_MAX, x_0 .. x_{i-1})), so body 0
+// never does (unless its x is DBL_MAX). The box stays rectangular:
+// {min, max - min}; a maximum no body raised stays -DBL_MAX, as there.
+//   k_refbox_scatter  positions into the caller's order (4 B index + 16 B
+//                     read + 16 B written per body)
+//   k_refbox_min      per tile of kRefBoxTile bodies: its minima
+//   k_refbox_max      per tile: the running minimum entering each body (the
+//                     tiles before it, then an in-tile scan) and the maximum
+//                     of the bodies that pass the test
+//   k_refbox_final    one wave: the box and its root size^2 w^2 + h^2
+// ---------------------------------------------------------------------------
+constexpr int kRefThreads = 256;
+constexpr int kRefItems = kRefBoxTile / kRefThreads;
+
+__global__ __launch_bounds__(256) void k_refbox_scatter(const double2* __restrict__ pos,
+                                                        const uint32_t* __restrict__ orig,
+                                                        uint32_t n, double2* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    out[orig[i]] = pos[i];
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmin(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__global__ __launch_bounds__(kRefThreads) void k_refbox_min(const double2* __restrict__ p,
+                                                            uint32_t n,
+                                                            double4* __restrict__ part) {
+    double mx = DBL_MAX, my = DBL_MAX;
+    const uint64_t base = (uint64_t)blockIdx.x * kRefBoxTile;
+#pragma unroll
+    for (int r = 0; r < kRefItems; r++) {
+        const uint64_t o = base + (uint64_t)r * kRefThreads + threadIdx.x;
+        if (o < n) {
+            const double2 q = p[o];
+            mx = fmin(mx, q.x);
+            my = fmin(my, q.y);
+        }
+    }
+    mx = wave_min(mx);
+    my = wave_min(my);
+    __shared__ double s[2][kRefThreads / 64];
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        s[0][w] = mx;
+        s[1][w] = my;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kRefThreads / 64; k++) {
+            mx = fmin(mx, s[0][k]);
+            my = fmin(my, s[1][k]);
+        }
+        part[blockIdx.x] = make_double4(mx, my, -DBL_MAX, -DBL_MAX);
+    }
+}
+
+// Exclusive min-scan of one double over the workgroup (thread order).
+__device__ __forceinline__ double block_excl_min(double v, double* smem) {
+    const int l = lane_id(), w = threadIdx.x >> 6;
+    double inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double o = __shfl_up(inc, d, 64);
+        if (l >= d) inc = fmin(inc, o);
+    }
+    if (l == 63) smem[w] = inc;
+    __syncthreads();
+    double before = DBL_MAX;  // waves before this one
+    for (int k = 0; k < w; k++) before = fmin(before, smem[k]);
+    double ex = __shfl_up(inc, 1, 64);
+    if (l == 0) ex = DBL_MAX;
+    __syncthreads();  // smem may be reused by the caller
+    return fmin(before, ex);
+}
+
+__global__ __launch_bounds__(kRefThreads) void k_refbox_max(const double2* __restrict__ p,
+                                                            uint32_t n,
+                                                            double4* __restrict__ part) {
+    __shared__ double s[2][kRefThreads / 64];
+    __shared__ double sr[4][kRefThreads / 64];
+    const int t = threadIdx.x, w = t >> 6;
+    // running minima entering this tile: the minima of every tile before it
+    double in_x = DBL_MAX, in_y = DBL_MAX;
+    for (uint32_t k = t; k < blockIdx.x; k += kRefThreads) {
+        const double4 q = part[k];
+        in_x = fmin(in_x, q.x);
+        in_y = fmin(in_y, q.y);
+    }
+    in_x = wave_min(in_x);
+    in_y = wave_min(in_y);
+    if (lane_id() == 0) {
+        s[0][w] = in_x;
+        s[1][w] = in_y;
+    }
+    __syncthreads();
+    for (int k = 0; k < kRefThreads / 64; k++) {
+        in_x = fmin(in_x, s[0][k]);
+        in_y = fmin(in_y, s[1][k]);
+    }
+    // thread t owns bodies base + t * kRefItems .. + kRefItems - 1 (caller order)
+    const uint64_t base = (uint64_t)blockIdx.x * kRefBoxTile + (uint64_t)t * kRefItems;
+    double2 q[kRefItems];
+    double tx = DBL_MAX, ty = DBL_MAX;
+#pragma unroll
+    for (int r = 0; r < kRefItems; r++) {
+        q[r] = base + r < n ? p[base + r] : make_double2(DBL_MAX, DBL_MAX);
+        tx = fmin(tx, q[r].x);
+        ty = fmin(ty, q[r].y);
+    }
+    double run_x = fmin(in_x, block_excl_min(tx, sr[0]));
+    double run_y = fmin(in_y, block_excl_min(ty, sr[1]));
+    double mx = -DBL_MAX, my = -DBL_MAX;
+#pragma unroll
+    for (int r = 0; r < kRefItems; r++) {
+        if (base + r >= n) break;
+        // if (x < min) min = x; else if (x > max) max = x;   (Quadtree.cpp:16-21)
+        if (q[r].x < run_x)
+            run_x = q[r].x;
+        else
+            mx = fmax(mx, q[r].x);
+        if (q[r].y < run_y)
+            run_y = q[r].y;
+        else
+            my = fmax(my, q[r].y);
+    }
+    mx = wave_max(mx);
+    my = wave_max(my);
+    if (lane_id() == 0) {
+        sr[2][w] = mx;
+        sr[3][w] = my;
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int k = 1; k < kRefThreads / 64; k++) {
+            mx = fmax(mx, sr[2][k]);
+            my = fmax(my, sr[3][k]);
+        }
+        // only this tile's maxima: other tiles read its minima meanwhile
+        double* o = reinterpret_cast<double*>(&part[blockIdx.x]);
+        o[2] = mx;
+        o[3] = my;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_refbox_final(const double4* __restrict__ part,
+                                                     uint32_t nparts, double* __restrict__ box,
+                                                     uint32_t* __restrict__ meta) {
+    double xmin = DBL_MAX, ymin = DBL_MAX, xmax = -DBL_MAX, ymax = -DBL_MAX;
+    for (uint32_t k = threadIdx.x; k < nparts; k += 64) {
+        const double4 q = part[k];
+        xmin = fmin(xmin, q.x);
+        ymin = fmin(ymin, q.y);
+        xmax = fmax(xmax, q.z);
+        ymax = fmax(ymax, q.w);
+    }
+    xmin = wave_min(xmin);
+    ymin = wave_min(ymin);
+    xmax = wave_max(xmax);
+    ymax = wave_max(ymax);
+    if (threadIdx.x == 0) {
+        box[0] = xmin;
+        box[1] = xmax;
+        box[2] = ymin;
+        box[3] = ymax;
+        const double w = xmax - xmin, h = ymax - ymin;  // Quadtree.cpp:30
+        {
+            // the root's size.lengthSquared() (BarnesHut.cpp:115), rounded as
+            // written: no multiply-add
+#pragma clang fp contract(off)
+            const double w2 = w * w, h2 = h * h;
+            box[4] = w2 + h2;
+        }
+        meta[META_MULTI_LEAVES] = 0;
+        meta[META_QHEAD] = 0;
+        meta[META_QTAIL] = 0;
+        if (!isfinite(xmin) || !isfinite(ymin) || !isfinite(w) || !isfinite(h))
+            meta[META_FLAGS] |= (uint32_t)FLAG_NONFINITE;
+    }
+}
+
 void launch_bbox(nb_ctx* c, int src) {
+    const int diag = c->p.mac == NB_MAC_DIAG;
+    if (c->p.cells == NB_CELLS_REF_CPU) {
+        const uint32_t n = (uint32_t)c->n;
+        hipLaunchKernelGGL(k_refbox_scatter, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                           c->b.pos[src], c->b.orig[src], n, c->b.ref_pos);
+        hipLaunchKernelGGL(k_refbox_min, dim3(c->ref_blocks), dim3(kRefThreads), 0, c->stream,
+                           c->b.ref_pos, n, c->b.ref_part);
+        hipLaunchKernelGGL(k_refbox_max, dim3(c->ref_blocks), dim3(kRefThreads), 0, c->stream,
+                           c->b.ref_pos, n, c->b.ref_part);
+        hipLaunchKernelGGL(k_refbox_final, dim3(1), dim3(64), 0, c->stream, c->b.ref_part,
+                           c->ref_blocks, c->b.box, c->b.meta);
+        return;
+    }
     if (c->nsys > 1) {
         hipLaunchKernelGGL(k_bbox_sys, dim3(c->nsys), dim3(kBboxThreads), 0, c->stream,
-                           c->b.pos[src], c->b.sys_off, c->b.box, c->b.meta);
+                           c->b.pos[src], c->b.sys_off, c->b.box, c->b.meta, diag);
         return;
     }
     hipLaunchKernelGGL(k_bbox_partial, dim3(c->bbox_blocks), dim3(kBboxThreads), 0, c->stream,
                        c->b.pos[src], (uint32_t)c->n, c->b.bbox_part);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, c->stream, c->b.bbox_part,
-                       c->bbox_blocks, c->b.box, c->b.meta);
+                       c->bbox_blocks, c->b.box, c->b.meta, diag);
 }
 
 // ---------------------------------------------------------------------------
@@ -221,8 +434,46 @@ __global__ __launch_bounds__(256) void k_morton(const double2* __restrict__ pos,
     vals[i] = i;
 }
 
+// Keys of the reference CPU quadtree (REF_CPU cells): the path of each body
+// down fill_tree_recursive's splits (Quadtree.cpp:57-106), one base-4 digit
+// per level for levels 0..29, x in the odd bits as for Morton keys (digit q =
+// right << 1 | bottom, right = !(x < cx), bottom = !(y < cy)). The centre and
+// the halved size of every level are computed as the reference computes them
+// (Rect::getCenter = position + size / 2; children at the centre or the
+// parent's position with size / 2), so a body's digits are its reference
+// child slots; the 4 low key bits stay zero. HBM traffic: 16 B read + 12 B
+// written per body (the descent is ~200 FP64 operations in registers).
+__global__ __launch_bounds__(256) void k_morton_ref(const double2* __restrict__ pos, uint32_t n,
+                                                    const double* __restrict__ box,
+                                                    uint64_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double px = box[0], py = box[2];
+    double sx = box[1] - box[0], sy = box[3] - box[2];
+    const double2 p = pos[i];
+    uint64_t key = 0;
+#pragma unroll 6
+    for (int L = 0; L < kMaxLevel; L++) {
+        const double cx = px + sx / 2.0, cy = py + sy / 2.0;
+        const bool right = !(p.x < cx), bottom = !(p.y < cy);
+        key |= (uint64_t)((right ? 2u : 0u) | (bottom ? 1u : 0u)) << (62 - 2 * L);
+        px = right ? cx : px;
+        py = bottom ? cy : py;
+        sx = sx / 2.0;
+        sy = sy / 2.0;
+    }
+    keys[i] = key;
+    vals[i] = i;
+}
+
 void launch_morton(nb_ctx* c, int src) {
     const uint32_t n = (uint32_t)c->n;
+    if (c->p.cells == NB_CELLS_REF_CPU) {
+        hipLaunchKernelGGL(k_morton_ref, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                           c->b.pos[src], n, c->b.box, c->b.keys[0], c->b.vals[0]);
+        return;
+    }
     hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->b.pos[src], n,
                        c->b.box, c->b.sys_of, c->b.keys[0], c->b.vals[0]);
 }

@@ -335,6 +335,23 @@ __device__ __forceinline__ Com combine(const Com ch[4]) {
     return o;
 }
 
+// The reference CPU quadtree's order (Quadtree.cpp:121-128): mass summed
+// TL + TR + BL + BR, the COM numerator TL, BR, BL, TR as products and sums
+// without contraction, then divided by the mass. Child slot q = right << 1 |
+// bottom, so TL, TR, BL, BR are q = 0, 2, 1, 3.
+__device__ __forceinline__ Com combine_ref(const Com ch[4]) {
+#pragma clang fp contract(off)
+    const Com &tl = ch[0], &tr = ch[2], &bl = ch[1], &br = ch[3];
+    Com o = combine(ch);  // second moments (the reference has none)
+    const double M = tl.m + tr.m + bl.m + br.m;
+    const double sx = tl.cx * tl.m + br.cx * br.m + bl.cx * bl.m + tr.cx * tr.m;
+    const double sy = tl.cy * tl.m + br.cy * br.m + bl.cy * bl.m + tr.cy * tr.m;
+    o.m = M;
+    o.cx = sx / M;
+    o.cy = sy / M;
+    return o;
+}
+
 __device__ __forceinline__ void store_com(NodeRec* __restrict__ r, const Com& o, uint32_t mask) {
     r->cx = o.cx;
     r->cy = o.cy;
@@ -392,7 +409,8 @@ __global__ __launch_bounds__(256) void k_level_scatter(const uint32_t* __restric
 }
 
 // The internal nodes of one level: segment [lvl_off[L * nblk], lvl_off[(L+1) * nblk])
-// of by_level (the last level ends at I).
+// of by_level (the last level ends at I). REF: the reference CPU order.
+template <bool REF>
 __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __restrict__ meta,
                                                    uint64_t icap, uint32_t nblk, uint32_t nsys,
                                                    const uint32_t* __restrict__ lvl_off,
@@ -410,7 +428,7 @@ __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __rest
 #pragma unroll
         for (int q = 0; q < 4; q++)
             ch[q] = (mask >> q) & 1u ? load_com(&nodes[nsys + 4ull * p + q]) : Com{0.0, 0.0, 0.0, 0.f, 0.f, 0.f};
-        store_com(&nodes[en.y], combine(ch), mask);
+        store_com(&nodes[en.y], REF ? combine_ref(ch) : combine(ch), mask);
     }
 }
 
@@ -444,10 +462,16 @@ void launch_tree_build(nb_ctx* c, int src) {
     // One launch per level, deepest first (a level's children are all done
     // when its launch starts).
     const uint32_t cg = ng < 512 ? ng : 512;
+    const bool ref = c->p.cells == NB_CELLS_REF_CPU;
     for (int L = kMaxLevel - 1; L >= 0; L--) {
-        hipLaunchKernelGGL(k_com_level, dim3(cg), dim3(256), 0, c->stream, L, c->b.meta,
-                           (uint64_t)c->icap, ng, c->nsys, c->b.lvl_cnt, c->b.by_level,
-                           c->b.nodes);
+        if (ref)
+            hipLaunchKernelGGL(k_com_level<true>, dim3(cg), dim3(256), 0, c->stream, L,
+                               c->b.meta, (uint64_t)c->icap, ng, c->nsys, c->b.lvl_cnt,
+                               c->b.by_level, c->b.nodes);
+        else
+            hipLaunchKernelGGL(k_com_level<false>, dim3(cg), dim3(256), 0, c->stream, L,
+                               c->b.meta, (uint64_t)c->icap, ng, c->nsys, c->b.lvl_cnt,
+                               c->b.by_level, c->b.nodes);
     }
 }
 

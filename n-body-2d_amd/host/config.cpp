@@ -1,6 +1,6 @@
 // config.cpp — config.json reader (Config.cpp:17-64) with a small JSON parser
 // (the reference uses nlohmann_json, absent here), and the range checks of
-// Config::Simulation::validate (Config.cpp:283-312).
+// Config::Simulation::validate (Config.cpp:224-253).
 #include <cctype>
 #include <cmath>
 #include <cstdlib>

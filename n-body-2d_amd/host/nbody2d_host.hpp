@@ -61,7 +61,7 @@ struct SimConfig {
     int traversal = NB_TRAV_EXACT;
     uint64_t softening_seed = 0x5eed;  // seed of the sampled epsilon branch
 
-    // Range checks of Config::Simulation::validate (Config.cpp:283-312);
+    // Range checks of Config::Simulation::validate (Config.cpp:224-253);
     // appends messages to *errors and returns false on any failure.
     bool validate(std::vector<std::string>* errors);
 };
@@ -147,9 +147,13 @@ private:
     void check(int status, const char* what) const;
 };
 
-// create_sim (main.cpp:26-38). Every algorithm string maps to the GPU:
-// "all pairs" -> direct sum with drift-then-kick; "barnes-hut" and
-// "barnes-hut gpu" -> Barnes-Hut with kick-then-drift.
+// create_sim (main.cpp:26-38). Every algorithm string maps to the GPU with
+// the physics of the reference path it names: "all pairs" -> direct sum,
+// drift then kick, the CPU force law (AllPairs.cpp, Simulation.cpp:74-80);
+// "barnes-hut" -> the CPU quadtree (rectangular cells, `else if` box), the
+// diagonal opening test and the CPU force law, kick then drift
+// (BarnesHut.cpp, Quadtree.cpp); "barnes-hut gpu" -> square Morton cells, the
+// side test and the Plummer law, kick then drift (BarnesHutCuda.cu).
 std::unique_ptr<Simulation> create_sim(const SimConfig& cfg, Bodies& bodies);
 
 }  // namespace nbody2d

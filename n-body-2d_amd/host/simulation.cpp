@@ -61,12 +61,23 @@ Simulation::Simulation(const SimConfig& cfg, Bodies& bodies)
     p.theta = cfg.theta;
     p.softening = epsilon_;
     p.traversal = cfg.traversal;
-    if (cfg.simtype == SimType::ALL_PAIRS) {
+    switch (cfg.simtype) {
+    case SimType::ALL_PAIRS:  // AllPairsSim: drift then kick, Simulation::force
         p.algorithm = NB_ALGO_ALL_PAIRS;
         p.integrator = NB_INTEG_DK;  // AllPairs.cpp:25-26
-    } else {
+        p.force_kernel = NB_FORCE_REF_CPU;  // Simulation.cpp:74-80
+        break;
+    case SimType::BARNES_HUT:  // BarnesHut: the CPU quadtree, diagonal test, CPU force
         p.algorithm = NB_ALGO_BARNES_HUT;
-        p.integrator = NB_INTEG_KD;  // BarnesHut.cpp:78-79, BarnesHutCuda.cpp:35-36
+        p.integrator = NB_INTEG_KD;  // BarnesHut.cpp:78-79
+        p.force_kernel = NB_FORCE_REF_CPU;  // BarnesHut.cpp:130-135
+        p.mac = NB_MAC_DIAG;                // BarnesHut.cpp:113-115
+        p.cells = NB_CELLS_REF_CPU;         // Quadtree.cpp:9-152
+        break;
+    case SimType::BARNES_HUT_GPU:  // BarnesHutCuda: Morton cells, side test, Plummer
+        p.algorithm = NB_ALGO_BARNES_HUT;
+        p.integrator = NB_INTEG_KD;  // BarnesHutCuda.cpp:35-36
+        break;
     }
     const int st = nb_create(&p, bodies.n(), bodies.mass.data(), bodies.pos.data(),
                              bodies.vel.data(), &ctx_);

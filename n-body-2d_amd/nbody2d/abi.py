@@ -29,6 +29,12 @@ FAR_MONOPOLE = 0
 FAR_LEARNED = 1
 TRAV_EXACT = 0
 TRAV_GROUP = 1
+FORCE_PLUMMER = 0
+FORCE_REF_CPU = 1
+MAC_SIDE = 0
+MAC_DIAG = 1
+CELLS_SQUARE = 0
+CELLS_REF_CPU = 1
 PHASES = ("bbox_morton", "sort", "build", "force", "drift", "far_field")
 
 # Every symbol include/nbody.h declares (checked by tests/test_abi_symbols.py).
@@ -39,7 +45,7 @@ EXPORTS = (
     "nb_direct_sum_f64", "nb_mlp_set_weights", "nb_mlp_get_weights", "nb_mlp_init",
     "nb_mlp_generate", "nb_mlp_get_pairs", "nb_mlp_set_pairs", "nb_mlp_train", "nb_mlp_grad",
     "nb_mlp_forward", "nb_debug_far_field", "nb_create_batch", "nb_autotune",
-    "nb_snapshot_begin", "nb_snapshot_query", "nb_snapshot_wait",
+    "nb_snapshot_begin", "nb_snapshot_query", "nb_snapshot_wait", "nb_debug_fail_alloc",
 )
 MLP_PARAMS = 4866
 MLP_INPUTS = 8
@@ -51,7 +57,8 @@ class Params(C.Structure):
         ("algorithm", C.c_int32), ("integrator", C.c_int32), ("self_guard", C.c_int32),
         ("far_field", C.c_int32), ("use_graph", C.c_int32), ("traversal", C.c_int32),
         ("group_budget", C.c_int32), ("cell_capacity", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("force_kernel", C.c_int32), ("mac", C.c_int32), ("cells", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -123,6 +130,7 @@ def lib() -> C.CDLL:
         L.nb_debug_far_field.argtypes = [P, P, P, P, P, P, P, C.POINTER(C.c_uint32),
                                          C.POINTER(C.c_uint32)]
         L.nb_autotune.argtypes = [P, C.c_uint32, C.POINTER(Tuning)]
+        L.nb_debug_fail_alloc.argtypes = [P, C.c_uint32]
         L.nb_snapshot_begin.argtypes = [P, C.POINTER(C.c_uint64)]
         L.nb_snapshot_query.argtypes = [P, C.c_uint64, C.POINTER(C.c_int)]
         L.nb_snapshot_wait.argtypes = [P, C.c_uint64, C.POINTER(C.POINTER(C.c_double)),
@@ -264,6 +272,10 @@ class Context:
         self._check(lib().nb_debug_build(self._h, _ptr(box), _ptr(keys), _ptr(orig),
                                          C.byref(cnt)))
         return box, keys, orig, int(cnt.value)
+
+    def debug_fail_alloc(self, count: int) -> None:
+        """Test hook: the next `count` device allocations of the context fail."""
+        self._check(lib().nb_debug_fail_alloc(self._h, int(count)))
 
     def debug_tree(self, count: int):
         nodes = np.zeros(count, dtype=NODE_DTYPE)

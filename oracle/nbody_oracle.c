@@ -365,7 +365,7 @@ int or_refcpu_bh_run(uint64_t n, double* pos, double* vel, const double* mass, d
                      double dt, double theta, double eps2, int force_kernel, int threads,
                      uint64_t steps, double* seconds_out) {
     if (threads <= 0 || (uint64_t)threads > n)
-        return -1; /* BarnesHut.cpp:45-48 (validated before the division) */
+        return -1; /* BarnesHut.cpp:10-13 (validated before the division) */
     rq_run r;
     memset(&r, 0, sizeof(r));
     r.n = n;

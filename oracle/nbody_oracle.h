@@ -36,7 +36,7 @@ extern "C" {
 enum { OR_FORCE_REF_CPU = 0, OR_FORCE_PLUMMER = 1 };
 
 /* Run `steps` threaded Barnes-Hut steps in place (BarnesHut.cpp:48-88).
- * pos/vel interleaved x,y. Returns 0, or -1 on invalid threads (BarnesHut.cpp:45-48).
+ * pos/vel interleaved x,y. Returns 0, or -1 on invalid threads (BarnesHut.cpp:10-13).
  * *seconds_out (may be NULL) receives the steady-clock wall time of the steps. */
 int or_refcpu_bh_run(uint64_t n, double* pos, double* vel, const double* mass, double G,
                      double dt, double theta, double eps2, int force_kernel, int threads,

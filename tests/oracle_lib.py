@@ -194,6 +194,29 @@ def refcpu_quadtree(pos, mass):
     return tl, bc, rect, com, tm
 
 
+# CPU child slot (TL, TR, BL, BR; Quadtree.cpp:65-68) -> key digit right << 1 | bottom
+REF_SLOT_DIGIT = (0, 2, 1, 3)
+
+
+def refcpu_cells(pos, mass):
+    """Non-empty cells of the reference CPU quadtree keyed by (level, digit
+    path): {(L, prefix): (bodies, mass, (cx, cy), w*w + h*h, is_leaf)}."""
+    tl, bc, rect, com, tm = refcpu_quadtree(pos, mass)
+    out = {}
+    todo = [(0, 0, 0)]
+    while todo:
+        k, lvl, pre = todo.pop()
+        if bc[k] == 0:
+            continue
+        w, h = float(rect[k, 2]), float(rect[k, 3])
+        out[(lvl, pre)] = (int(bc[k]), float(tm[k]), (float(com[k, 0]), float(com[k, 1])),
+                           w * w + h * h, int(tl[k]) == 0)
+        if tl[k]:
+            for s in range(4):
+                todo.append((int(tl[k]) + s, lvl + 1, (pre << 2) | REF_SLOT_DIGIT[s]))
+    return out
+
+
 def refcpu_bh_run(pos, vel, mass, G, dt, theta, eps2, threads, steps, kernel=FORCE_REF_CPU,
                   fast=False):
     p, v, m = c64(pos).copy(), c64(vel).copy(), c64(mass)

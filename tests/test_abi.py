@@ -43,7 +43,8 @@ def test_params_default():
 
 @pytest.mark.parametrize("field,value", [("dt", 0.0), ("dt", float("nan")), ("theta", -1.0),
                                          ("softening", -0.1), ("G", float("inf")),
-                                         ("algorithm", 7), ("integrator", 3), ("far_field", 9)])
+                                         ("algorithm", 7), ("integrator", 3), ("far_field", 9),
+                                         ("force_kernel", 2), ("mac", -1), ("cells", 5)])
 def test_create_rejects_invalid_params(field, value):
     m = np.ones(4)
     x = np.zeros((4, 2))
@@ -59,6 +60,28 @@ def test_learned_far_field_needs_group_barnes_hut(kw):
     x = np.zeros((4, 2))
     with pytest.raises(abi.NBodyError) as e:
         abi.Context(m, x, x, far_field=abi.FAR_LEARNED, **kw)
+    assert e.value.status == abi.NB_ERR_NOT_SUPPORTED
+
+
+@pytest.mark.parametrize("kw", [
+    dict(cells=abi.CELLS_REF_CPU),  # rectangular cells need the diagonal test
+    dict(far_field=abi.FAR_LEARNED, traversal=abi.TRAV_GROUP, force_kernel=abi.FORCE_REF_CPU),
+    dict(far_field=abi.FAR_LEARNED, traversal=abi.TRAV_GROUP, mac=abi.MAC_DIAG),
+])
+def test_unsupported_physics_combinations(kw):
+    m = np.ones(4)
+    x = np.zeros((4, 2))
+    with pytest.raises(abi.NBodyError) as e:
+        abi.Context(m, x, x, **kw)
+    assert e.value.status == abi.NB_ERR_NOT_SUPPORTED
+
+
+def test_reference_cells_hold_one_system():
+    m = np.ones(4)
+    x = np.zeros((4, 2))
+    with pytest.raises(abi.NBodyError) as e:
+        abi.Context(m, x, x, offsets=[0, 2, 4], traversal=abi.TRAV_GROUP,
+                    cells=abi.CELLS_REF_CPU, mac=abi.MAC_DIAG)
     assert e.value.status == abi.NB_ERR_NOT_SUPPORTED
 
 

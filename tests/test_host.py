@@ -135,13 +135,45 @@ def test_driver_matches_library(tmp_path, algorithm, traversal):
     assert stats["iterations"] == 7 and stats["epsilon"] == 0.01
     ids, vals = read_csv(out)
     ap = algorithm == "all pairs"
+    A = nbody2d.abi
     kw = dict(G=1.0, dt=2e-3, theta=0.6, softening=0.01,
-              traversal=nbody2d.abi.TRAV_GROUP if traversal == "group" else nbody2d.abi.TRAV_EXACT,
-              algorithm=nbody2d.abi.ALGO_ALL_PAIRS if ap else nbody2d.abi.ALGO_BARNES_HUT,
-              integrator=nbody2d.abi.INTEG_DK if ap else nbody2d.abi.INTEG_KD)
+              traversal=A.TRAV_GROUP if traversal == "group" else A.TRAV_EXACT,
+              algorithm=A.ALGO_ALL_PAIRS if ap else A.ALGO_BARNES_HUT,
+              integrator=A.INTEG_DK if ap else A.INTEG_KD)
+    if algorithm.lower() != "barnes-hut gpu":  # the CPU paths' physics
+        kw["force_kernel"] = A.FORCE_REF_CPU
+    if algorithm.lower() == "barnes-hut":
+        kw.update(mac=A.MAC_DIAG, cells=A.CELLS_REF_CPU)
     with nbody2d.Context(m, p, v, **kw) as c:
         c.step(7)
         gp, gv = c.state()
+    assert np.array_equal(vals[:, 1:3], gp) and np.array_equal(vals[:, 3:5], gv)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("algorithm", ["Barnes-Hut", "barnes-hut gpu"])
+def test_driver_softening_factor_exact_branch(tmp_path, algorithm):
+    """epsilon from softening_factor (Simulation.cpp:82-126): with n^2 <= 1e6
+    the driver takes the exact branch, factor x (sum of pair distances) / n,
+    equal to the oracle's; the run then equals the library run with that
+    absolute epsilon, bit for bit."""
+    m, p, v = ics.exp_disk(800, seed=5)
+    src, out = tmp_path / "in.csv", tmp_path / "out.csv"
+    write_csv(src, m, p, v)
+    cfg = config(tmp_path, src, out, iterations=5, algorithm=algorithm, G=1.0,
+                 softening_factor=0.01, theta=0.5, timestep=1e-3)
+    print(f"[gpu] driver softening_factor {algorithm}", flush=True)
+    stats = json.loads(run(cfg).stdout)
+    want = O.load().or_plummer_softening_exact(len(p), O.ptr(np.ascontiguousarray(p)), 0.01, 1e6)
+    assert want > 0 and stats["epsilon"] == pytest.approx(want, rel=1e-14)
+    A = nbody2d.abi
+    kw = dict(G=1.0, dt=1e-3, theta=0.5, softening=stats["epsilon"])
+    if algorithm == "Barnes-Hut":
+        kw.update(force_kernel=A.FORCE_REF_CPU, mac=A.MAC_DIAG, cells=A.CELLS_REF_CPU)
+    with nbody2d.Context(m, p, v, **kw) as c:
+        c.step(5)
+        gp, gv = c.state()
+    ids, vals = read_csv(out)
     assert np.array_equal(vals[:, 1:3], gp) and np.array_equal(vals[:, 3:5], gv)
 
 

@@ -186,6 +186,59 @@ def test_tree_matches_reference_cpu_quadtree_on_dyadic_lattice():
             assert np.allclose(c, g[2], rtol=1e-13, atol=1e-12)
 
 
+def mac_probe_case(probe):
+    """Two unit masses A, B alone in the cell [0, 1)^2 (side 1, COM (0.5, 0.5))
+    of the box [0, 4]^2 that two corner bodies fix, and a probe body; body 0
+    is A, so the CPU box's `else if` changes nothing."""
+    pos = np.array([[0.25, 0.25], [0.75, 0.75], [0.0, 4.0], [4.0, 0.0], probe])
+    return pos, np.ones(5)
+
+
+@pytest.mark.parametrize("d,terms", [(2.0, 4), (2.0 * (1 - 1e-12), 4), (2.0 * (1 + 1e-12), 3)])
+def test_side_mac_flips_exactly_at_side_over_theta(d, terms):
+    """CUDA model, wsq < theta^2 d^2: at theta 0.5 the pair's side-1 cell is
+    accepted (one term instead of two) only beyond d = 2; exactly at d = 2
+    the strict test opens it. theta for theta^2, <= for <, or the cell's
+    side for its square would each move the flip."""
+    pos, m = mac_probe_case((0.5 + d, 0.5))
+    acc, inter = O.bh_forces(pos, m, 0.5, 0.0)
+    assert int(inter[4]) == terms
+    if terms == 3:  # the pair as one monopole of mass 2 at (0.5, 0.5)
+        far = 2.0 * (np.array([0.5, 0.5]) - pos[4]) / d ** 3
+        rest = sum((pos[j] - pos[4]) / np.linalg.norm(pos[j] - pos[4]) ** 3 for j in (2, 3))
+        assert np.allclose(acc[4], far + rest, rtol=1e-14)
+
+
+@pytest.mark.parametrize("t,terms", [(2.0, 4), (2.0 * (1 + 1e-12), 3)])
+def test_diag_mac_flips_exactly_at_diagonal_over_theta(t, terms):
+    """Reference CPU, (w^2 + h^2) / d^2 < theta^2: the side-1 cell has
+    w^2 + h^2 = 2, so at theta 0.5 it opens up to d^2 = 8 (probe at COM +
+    (2, 2), exactly representable) and is accepted beyond; the side test
+    would already accept there."""
+    pos, m = mac_probe_case((0.5 + t, 0.5 + t))
+    assert np.array_equal(O.refcpu_bbox(pos), [0.0, 0.0, 4.0, 4.0])
+    acc, inter = O.refcpu_accel(pos, m, 1.0, 0.5, 0.0, O.FORCE_REF_CPU)
+    assert int(inter[4]) == terms
+    side_inter = O.bh_forces(pos, m, 0.5, 0.0)[1]
+    assert int(side_inter[4]) == 3
+
+
+def test_refcpu_cells_key_the_reference_quadtree():
+    """The (level, digit path) view of the CPU quadtree used by the GPU
+    tests: on the probe case every cell is where the geometry puts it."""
+    pos, m = mac_probe_case((2.5, 2.5))
+    cells = O.refcpu_cells(pos, m)
+    assert cells[(0, 0)][:2] == (5, 5.0)
+    assert cells[(1, 0)][0] == 2 and cells[(1, 0)][3] == 8.0      # TL quadrant: the pair
+    assert cells[(2, 0)][2] == (0.5, 0.5) and cells[(2, 0)][3] == 2.0
+    assert cells[(3, 0)][2] == (0.25, 0.25) and cells[(3, 0)][4]   # A: top-left leaf
+    assert cells[(3, 3)][2] == (0.75, 0.75)                         # B: bottom-right
+    assert cells[(1, 1)][2] == (0.0, 4.0)    # x < 2, y >= 2: bottom-left, digit 1
+    assert cells[(1, 2)][2] == (4.0, 0.0)    # x >= 2, y < 2: top-right, digit 2
+    assert cells[(1, 3)][2] == (2.5, 2.5)
+    assert len(cells) == 8
+
+
 def test_refcpu_bbox_else_if_quirk():
     # body 0 holds the largest x: the `else if` never lets it update the max
     pos = np.array([[9.0, 1.0], [1.0, 2.0], [5.0, 3.0]])
