@@ -263,6 +263,23 @@ int nb_debug_tree(nb_ctx* ctx, nb_node* nodes, uint64_t cap);
 int nb_direct_sum_f64(uint64_t n, const double* pos_xy, const double* mass, double G,
                       double softening, uint64_t m, const int64_t* targets, double* acc_xy);
 
+/* No counterpart in the reference. The FP64 direct sum of nb_direct_sum_f64
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): pos_xy,
+ * mass, targets and acc_xy are DEVICE arrays of the current device, the work
+ * is enqueued on `stream` (a hipStream_t; NULL = the default stream) and the
+ * call returns without waiting. It allocates nothing: the partial sums live
+ * in a fixed device array of NB_NOALLOC_SCRATCH_BYTES per GPU (held until the
+ * process ends), so m may be at most NB_NOALLOC_SCRATCH_BYTES / 16; larger m
+ * is NB_ERR_INVALID_ARG. Results equal nb_direct_sum_f64's bit for bit (the
+ * same j split, kernels and summation order; the split's partials always fit
+ * the array). A target outside [0, n) yields NaN. Calls that overlap on one
+ * GPU share the array and race: serialise them (one stream, or events). */
+#define NB_NOALLOC_SCRATCH_BYTES (16u << 20)
+int nb_direct_sum_f64_noalloc(uint64_t n, const double* pos_xy, const double* mass, double G,
+                              double softening, uint64_t m, const int64_t* targets,
+                              double* acc_xy, void* stream);
+
 /* Autotuner: times one force evaluation of the current state (sort, tree and
  * traversal, or the all-pairs kernel; best of `repeats`) for each setting of
  * the context's launch tunables and keeps the fastest -- GROUP traversal:

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1058,6 +1059,35 @@ int nb_debug_tree(nb_ctx* c, nb_node* nodes, uint64_t cap) {
 
 namespace {
 
+// Partial sums of nb_direct_sum_f64_noalloc: [split][m] double2 entries in a
+// fixed device array (16 MiB on every GPU the library runs on, held until
+// the process ends). Host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device.
+constexpr uint64_t kNoAllocScratchEntries = (uint64_t)NB_NOALLOC_SCRATCH_BYTES / sizeof(double2);
+__device__ __align__(256) double2 scratch[kNoAllocScratchEntries];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mtx;
+double2* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch array's address on the current device, looked up once per
+// device with that device current.
+hipError_t scratch_on_current_device(double2** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(g_scratch_mtx);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<double2*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
 // The j split of the direct sum for m targets over n sources.
 uint32_t direct_sum_split(uint64_t n, uint64_t m) {
     const uint32_t gx = (uint32_t)((m + 255) / 256);
@@ -1069,6 +1099,34 @@ uint32_t direct_sum_split(uint64_t n, uint64_t m) {
 }  // namespace
 
 extern "C" {
+
+int nb_direct_sum_f64_noalloc(uint64_t n, const double* pos_xy, const double* mass, double G,
+                              double softening, uint64_t m, const int64_t* targets,
+                              double* acc_xy, void* stream) {
+    if ((n && (!pos_xy || !mass)) || (m && (!targets || !acc_xy)))
+        return fail(nullptr, NB_ERR_INVALID_ARG, "NULL argument");
+    if (n > 0x7fffffffull || m > 0x7fffffffull)
+        return fail(nullptr, NB_ERR_INVALID_ARG, "size exceeds 2^31-1");
+    if (m > kNoAllocScratchEntries)
+        return fail(nullptr, NB_ERR_INVALID_ARG,
+                    "m exceeds the scratch buffer (NB_NOALLOC_SCRATCH_BYTES / 16 targets)");
+    if (m == 0) return NB_OK;
+    double2* part = nullptr;
+    hipError_t e = scratch_on_current_device(&part);
+    if (e != hipSuccess)
+        return fail(nullptr, NB_ERR_HIP, std::string("scratch address: ") + hipGetErrorString(e));
+    uint32_t split = direct_sum_split(n, m);
+    while ((uint64_t)split * m > kNoAllocScratchEntries) split--;  // keeps split >= 1
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    e = direct_sum_f64(n, reinterpret_cast<const double2*>(pos_xy), mass,
+                       softening * softening, m, targets, part, split, s);
+    if (e == hipSuccess)
+        e = direct_sum_f64_finish(part, split, m, n, targets, G,
+                                  reinterpret_cast<double2*>(acc_xy), s);
+    if (e != hipSuccess)
+        return fail(nullptr, NB_ERR_HIP, std::string("direct sum: ") + hipGetErrorString(e));
+    return NB_OK;
+}
 
 int nb_direct_sum_f64(uint64_t n, const double* pos_xy, const double* mass, double G,
                       double softening, uint64_t m, const int64_t* targets, double* acc_xy) {

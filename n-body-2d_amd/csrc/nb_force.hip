@@ -907,6 +907,34 @@ __global__ __launch_bounds__(256) void k_direct64(const double2* __restrict__ po
     if (t < m) part[(uint64_t)blockIdx.y * m + t] = make_double2(sx - cx, sy - cy);
 }
 
+// Sums the split partials of each target in split order, then scales by G:
+// the same additions, in the same order, as the host-side sum of
+// nb_direct_sum_f64. A target outside [0, n) gets NaN.
+__global__ __launch_bounds__(256) void k_direct64_finish(const double2* __restrict__ part,
+                                                         uint32_t split, uint32_t m, uint32_t n,
+                                                         const int64_t* __restrict__ tgt, double G,
+                                                         double2* __restrict__ acc) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= m) return;
+    double ax = 0.0, ay = 0.0;
+    for (uint32_t k = 0; k < split; k++) {
+        const double2 v = part[(uint64_t)k * m + t];
+        ax += v.x;
+        ay += v.y;
+    }
+    const int64_t ti = tgt[t];
+    acc[t] = (ti >= 0 && ti < (int64_t)n) ? make_double2(ax * G, ay * G)
+                                          : make_double2(__longlong_as_double(0x7ff8000000000000ll),
+                                                         __longlong_as_double(0x7ff8000000000000ll));
+}
+
+hipError_t direct_sum_f64_finish(const double2* part, uint32_t split, uint64_t m, uint64_t n,
+                                 const int64_t* targets, double G, double2* acc, hipStream_t s) {
+    hipLaunchKernelGGL(k_direct64_finish, dim3((uint32_t)((m + 255) / 256)), dim3(256), 0, s,
+                       part, split, (uint32_t)m, (uint32_t)n, targets, G, acc);
+    return hipGetLastError();
+}
+
 hipError_t direct_sum_f64(uint64_t n, const double2* pos, const double* mass, double eps2,
                           uint64_t m, const int64_t* targets, double2* part, uint32_t split,
                           hipStream_t s) {

@@ -238,5 +238,7 @@ int build_tree(nb_ctx* c);
 hipError_t direct_sum_f64(uint64_t n, const double2* pos, const double* mass, double eps2,
                           uint64_t m, const int64_t* targets, double2* part, uint32_t split,
                           hipStream_t s);
+hipError_t direct_sum_f64_finish(const double2* part, uint32_t split, uint64_t m, uint64_t n,
+                                 const int64_t* targets, double G, double2* acc, hipStream_t s);
 
 }  // namespace nb

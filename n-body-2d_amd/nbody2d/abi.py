@@ -46,7 +46,9 @@ EXPORTS = (
     "nb_mlp_generate", "nb_mlp_get_pairs", "nb_mlp_set_pairs", "nb_mlp_train", "nb_mlp_grad",
     "nb_mlp_forward", "nb_debug_far_field", "nb_create_batch", "nb_autotune",
     "nb_snapshot_begin", "nb_snapshot_query", "nb_snapshot_wait", "nb_debug_fail_alloc",
+    "nb_direct_sum_f64_noalloc",
 )
+NOALLOC_SCRATCH_BYTES = 16 << 20
 MLP_PARAMS = 4866
 MLP_INPUTS = 8
 
@@ -131,6 +133,8 @@ def lib() -> C.CDLL:
                                          C.POINTER(C.c_uint32)]
         L.nb_autotune.argtypes = [P, C.c_uint32, C.POINTER(Tuning)]
         L.nb_debug_fail_alloc.argtypes = [P, C.c_uint32]
+        L.nb_direct_sum_f64_noalloc.argtypes = [C.c_uint64, P, P, C.c_double, C.c_double,
+                                                C.c_uint64, P, P, P]
         L.nb_snapshot_begin.argtypes = [P, C.POINTER(C.c_uint64)]
         L.nb_snapshot_query.argtypes = [P, C.c_uint64, C.POINTER(C.c_int)]
         L.nb_snapshot_wait.argtypes = [P, C.c_uint64, C.POINTER(C.POINTER(C.c_double)),
