@@ -146,8 +146,9 @@ void free_all(nb_ctx* c) {
         dfree(b.keys[k]);
         dfree(b.vals[k]);
     }
-    dfree(b.sort_hist);
-    dfree(b.sort_tot);
+    dfree(b.sort_glob);
+    dfree(b.sort_status);
+    dfree(b.sort_part);
     dfree(b.box);
     dfree(b.bbox_part);
     dfree(b.ref_pos);
@@ -395,6 +396,7 @@ int build_and_force(nb_ctx* c, bool forces) {
         if (forces) c->last_cells = cells_total(c);
         if (flags & FLAG_NONFINITE) return fail(c, NB_ERR_NUMERIC, "non-finite body positions");
         if (flags & FLAG_STACK) return fail(c, NB_ERR_OVERFLOW, "traversal stack overflow");
+        if (flags & FLAG_SORT) return mark_dead(c, fail(c, NB_ERR_HIP, "radix sort stalled"));
         if (!(flags & (FLAG_TREE_CAPACITY | FLAG_CELL_CAPACITY))) return NB_OK;
         rc = grow_for(c, flags);
         if (rc) return rc;
@@ -510,8 +512,16 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
     // (internal nodes are fewer than 30 per body pair)
     const uint64_t scan_blocks = std::max<uint64_t>((n + 1023) / 1024,
                                                     (uint64_t)kMaxLevel * (30 * n / 256 + 2) / 1024) + 2;
-    NB_TRY(dalloc(c, &b.sort_hist, bh ? (uint64_t)kSortBins * c->sort_tiles : 0));
-    NB_TRY(dalloc(c, &b.sort_tot, kSortBins));
+    // radix sort state: zeroed once, so no published word can match a sort's tag
+    // (tiles + tile groups of 16) published words, and one histogram row per
+    // Morton workgroup (kSortTile keys each)
+    const uint64_t sort_words = (uint64_t)kSortBins * (c->sort_tiles + (c->sort_tiles + 15) / 16);
+    NB_TRY(dalloc(c, &b.sort_glob, kSortGlobWords));
+    NB_TRY(dalloc(c, &b.sort_status, bh ? sort_words : 0));
+    NB_TRY(dalloc(c, &b.sort_part, bh ? (uint64_t)c->sort_tiles * kSortMaxPasses * kSortBins : 0));
+    NB_TRY(hipMemsetAsync(b.sort_glob, 0, kSortGlobWords * sizeof(uint32_t), c->stream));
+    if (bh && c->sort_tiles)
+        NB_TRY(hipMemsetAsync(b.sort_status, 0, sort_words * sizeof(uint64_t), c->stream));
     NB_TRY(dalloc(c, &b.box, (uint64_t)kBoxStride * nsys));
     // systems and traversal groups (<= 64 bodies, never across two systems)
     std::vector<uint32_t> sys_of(n), grp_first, grp_info;
@@ -684,6 +694,7 @@ int nb_step(nb_ctx* c, uint64_t k) {
         remaining -= done;
         if (flags & FLAG_NONFINITE) return fail(c, NB_ERR_NUMERIC, "non-finite body positions");
         if (flags & FLAG_STACK) return fail(c, NB_ERR_OVERFLOW, "traversal stack overflow");
+        if (flags & FLAG_SORT) return mark_dead(c, fail(c, NB_ERR_HIP, "radix sort stalled"));
         if (flags & (FLAG_TREE_CAPACITY | FLAG_CELL_CAPACITY)) {
             if (++regrowths > 32) return fail(c, NB_ERR_HIP, "buffer capacity did not converge");
             rc = grow_for(c, flags);
@@ -889,6 +900,7 @@ int nb_autotune(nb_ctx* c, uint32_t repeats, nb_tuning* out) {
             c->last_internal = c->host_meta[META_INTERNAL];
             if (flags & FLAG_NONFINITE) return fail(c, NB_ERR_NUMERIC, "non-finite positions");
             if (flags & FLAG_STACK) return fail(c, NB_ERR_OVERFLOW, "traversal stack overflow");
+            if (flags & FLAG_SORT) return mark_dead(c, fail(c, NB_ERR_HIP, "radix sort stalled"));
             if (flags & (FLAG_TREE_CAPACITY | FLAG_CELL_CAPACITY)) {
                 rc = grow_for(c, flags);  // then time this setting again
                 if (rc) return rc;

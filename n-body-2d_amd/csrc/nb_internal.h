@@ -32,6 +32,11 @@ constexpr int kSortBins = 1 << kSortBits;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;         // keys per thread per tile
 constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kSortMaxPasses = 10;     // 8 key passes + 2 system-number passes
+// sort_glob words: per-pass digit histograms, per-pass tile counters, epoch
+constexpr int kSortGlobTile = kSortMaxPasses * kSortBins;
+constexpr int kSortGlobEpoch = kSortGlobTile + 16;
+constexpr int kSortGlobWords = kSortGlobEpoch + 16;
 constexpr int kBoxStride = 8;          // doubles of box data per system
 constexpr int kRefBoxTile = 2048;      // bodies per block of the REF_CPU box scan
 // Group traversal: list entries after which a group is split and walked again
@@ -60,9 +65,12 @@ enum Flag : uint32_t {
     FLAG_STACK = 2u,          // traversal stack overflow
     FLAG_NONFINITE = 4u,      // non-finite bounding box
     FLAG_CELL_CAPACITY = 8u,  // a group's accepted cells exceed its region
+    FLAG_SORT = 16u,          // a radix-sort tile waited too long for its predecessors
 };
-// Flags that stop the rest of a step (the host grows the buffer and re-runs).
-constexpr uint32_t kHaltFlags = FLAG_TREE_CAPACITY | FLAG_NONFINITE | FLAG_CELL_CAPACITY;
+// Flags that stop the rest of a step (the host grows the buffer and re-runs,
+// or reports the error).
+constexpr uint32_t kHaltFlags =
+    FLAG_TREE_CAPACITY | FLAG_NONFINITE | FLAG_CELL_CAPACITY | FLAG_SORT;
 
 // One tree slot, 64 bytes so one wave-uniform node fetch is a single 64-byte
 // scalar load.
@@ -102,8 +110,12 @@ struct Buffers {
     uint32_t* orig[2];
     uint64_t* keys[2];
     uint32_t* vals[2];
-    uint32_t* sort_hist;   // [bins][tiles]
-    uint32_t* sort_tot;    // [bins]
+    uint32_t* sort_glob;   // [kSortGlobWords] digit histogram of every pass, the
+                           //     passes' tile counters, the sort epoch
+    uint64_t* sort_status; // [tiles + groups][bins] digit counts / prefixes of the
+                           //     tiles and tile groups published for the
+                           //     look-back (tagged with epoch and pass)
+    uint32_t* sort_part;   // [Morton workgroups][kSortMaxPasses * bins] digit counts
     double* box;           // per system [8]: x_min, x_max, y_min, y_max, root size^2 of
                            //     the opening test (side^2, or w^2 + h^2 for DIAG)
     uint32_t* sys_of;      // [N] system of the body at position i (positions of a

@@ -8,6 +8,8 @@
 // a step has no host round trip.
 #include <float.h>
 
+#include <algorithm>
+
 #include "nb_device.h"
 #include "nb_internal.h"
 
@@ -62,6 +64,14 @@ __global__ __launch_bounds__(kBboxThreads) void k_bbox_partial(const double2* __
     }
 }
 
+// Clears the radix sort's global histograms and tile counters and advances
+// its epoch (see k_os_pass); done by one workgroup of the box kernel that
+// starts every step, ahead of the Morton keys and the sort.
+__device__ __forceinline__ void sort_glob_reset(uint32_t* __restrict__ g, int tid, int nthreads) {
+    for (int k = tid; k < kSortGlobEpoch; k += nthreads) g[k] = 0u;
+    if (tid == 0) g[kSortGlobEpoch] += 1u;
+}
+
 // Root size^2 of the opening test: side^2 (SIDE), or w^2 + h^2 = 2 side^2 on
 // a square cell (DIAG; exact, as the reference's lengthSquared of (s, s)).
 __device__ __forceinline__ double root_size_sq(double w, int diag) {
@@ -71,7 +81,9 @@ __device__ __forceinline__ double root_size_sq(double w, int diag) {
 
 __global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ part, uint32_t nparts,
                                                    double* __restrict__ box,
-                                                   uint32_t* __restrict__ meta, int diag) {
+                                                   uint32_t* __restrict__ meta, int diag,
+                                                   uint32_t* __restrict__ sort_glob) {
+    sort_glob_reset(sort_glob, threadIdx.x, 64);
     double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
     for (uint32_t k = threadIdx.x; k < nparts; k += 64) {
         xmin = fmin(xmin, part[4 * k + 0]);
@@ -117,8 +129,10 @@ __global__ __launch_bounds__(kBboxThreads) void k_bbox_sys(const double2* __rest
                                                            const uint32_t* __restrict__ sys_off,
                                                            double* __restrict__ box,
                                                            uint32_t* __restrict__ meta,
-                                                           int diag) {
+                                                           int diag,
+                                                           uint32_t* __restrict__ sort_glob) {
     const uint32_t s = blockIdx.x;
+    if (s == 0) sort_glob_reset(sort_glob, threadIdx.x, kBboxThreads);
     const uint32_t a = sys_off[s], e = sys_off[s + 1];
     double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
     for (uint32_t i = a + threadIdx.x; i < e; i += kBboxThreads) {
@@ -337,7 +351,9 @@ __global__ __launch_bounds__(kRefThreads) void k_refbox_max(const double2* __res
 
 __global__ __launch_bounds__(64) void k_refbox_final(const double4* __restrict__ part,
                                                      uint32_t nparts, double* __restrict__ box,
-                                                     uint32_t* __restrict__ meta) {
+                                                     uint32_t* __restrict__ meta,
+                                                     uint32_t* __restrict__ sort_glob) {
+    sort_glob_reset(sort_glob, threadIdx.x, 64);
     double xmin = DBL_MAX, ymin = DBL_MAX, xmax = -DBL_MAX, ymax = -DBL_MAX;
     for (uint32_t k = threadIdx.x; k < nparts; k += 64) {
         const double4 q = part[k];
@@ -382,18 +398,108 @@ void launch_bbox(nb_ctx* c, int src) {
         hipLaunchKernelGGL(k_refbox_max, dim3(c->ref_blocks), dim3(kRefThreads), 0, c->stream,
                            c->b.ref_pos, n, c->b.ref_part);
         hipLaunchKernelGGL(k_refbox_final, dim3(1), dim3(64), 0, c->stream, c->b.ref_part,
-                           c->ref_blocks, c->b.box, c->b.meta);
+                           c->ref_blocks, c->b.box, c->b.meta, c->b.sort_glob);
         return;
     }
     if (c->nsys > 1) {
         hipLaunchKernelGGL(k_bbox_sys, dim3(c->nsys), dim3(kBboxThreads), 0, c->stream,
-                           c->b.pos[src], c->b.sys_off, c->b.box, c->b.meta, diag);
+                           c->b.pos[src], c->b.sys_off, c->b.box, c->b.meta, diag,
+                           c->b.sort_glob);
         return;
     }
     hipLaunchKernelGGL(k_bbox_partial, dim3(c->bbox_blocks), dim3(kBboxThreads), 0, c->stream,
                        c->b.pos[src], (uint32_t)c->n, c->b.bbox_part);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, c->stream, c->b.bbox_part,
-                       c->bbox_blocks, c->b.box, c->b.meta, diag);
+                       c->bbox_blocks, c->b.box, c->b.meta, diag, c->b.sort_glob);
+}
+
+// ---------------------------------------------------------------------------
+// Radix-sort digits and their global histograms (see the sort below). The
+// Morton kernels count the digits of every pass while they produce the keys
+// (per workgroup in LDS, one row per workgroup), k_os_reduce adds the rows.
+// ---------------------------------------------------------------------------
+constexpr int kKeyItems = 16;  // keys per thread of the Morton kernels
+constexpr int kKeyTile = 256 * kKeyItems;
+
+// Digit of pass p: key passes take bits [8p, 8p + 8) of the key; the system
+// passes (p >= 8) take bits [8(p - 8), ...) of the system of the key's
+// source position.
+__device__ __forceinline__ uint32_t os_digit(uint64_t key, uint32_t sys, int pass) {
+    return pass < 8 ? (uint32_t)(key >> (8 * pass)) & (kSortBins - 1)
+                    : (sys >> (8 * (pass - 8))) & (kSortBins - 1);
+}
+
+// Lanes of this wave whose digit equals mine (valid lanes only).
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
+    uint64_t peers = ballot64(valid);
+#pragma unroll
+    for (int b = 0; b < kSortBits; b++) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = ballot64(valid && bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+__device__ __forceinline__ void os_hist_clear(uint32_t (*h)[kSortBins]) {
+    for (int k = threadIdx.x; k < kSortMaxPasses * kSortBins; k += blockDim.x) (&h[0][0])[k] = 0u;
+}
+
+// Counts one key per valid lane. The keys of a wave are 64 consecutive ones
+// in the previous step's order, nearly sorted, so their high digits are
+// mostly equal: such a digit is counted with one LDS add for the wave.
+__device__ __forceinline__ void os_hist_add(uint32_t (*h)[kSortBins], uint64_t key, uint32_t sys,
+                                            int npass, bool valid) {
+    const uint64_t active = ballot64(valid);
+    if (active == 0) return;
+    const bool leader = (active & lanemask_lt()) == 0 && valid;
+#pragma unroll
+    for (int p = 0; p < kSortMaxPasses; p++) {
+        if (p >= npass) break;
+        const uint32_t d = os_digit(key, sys, p);
+        // lane 0's digit (a scalar read); the wave counts at once when every
+        // valid lane has it
+        const uint32_t d0 = (uint32_t)readfirst((int)d);
+        if (ballot64(valid && d == d0) == active) {
+            if (leader) atomicAdd(&h[p][d0], (uint32_t)__popcll(active));
+        } else if (valid) {
+            atomicAdd(&h[p][d], 1u);
+        }
+    }
+}
+
+// Writes the workgroup's counts (all bins) to its row of `part`
+// ([blocks][kSortMaxPasses * kSortBins]); k_os_reduce adds the rows up.
+// (Global atomics from every workgroup would serialise on each bin.)
+__device__ __forceinline__ void os_hist_flush(uint32_t (*h)[kSortBins], uint32_t* __restrict__ part,
+                                              int npass) {
+    __syncthreads();
+    uint32_t* row = part + (uint64_t)blockIdx.x * (kSortMaxPasses * kSortBins);
+    for (int k = threadIdx.x; k < npass * kSortBins; k += blockDim.x) row[k] = (&h[0][0])[k];
+}
+
+// Global digit histograms: workgroup (pass p, slice s) adds up the rows
+// b = s, s + kOsSlices, ... of `part` for every digit (one thread per
+// digit), then one atomic per digit into the cleared global counts.
+constexpr int kOsSlices = 16;
+__global__ __launch_bounds__(kSortBins) void k_os_reduce(const uint32_t* __restrict__ part,
+                                                         uint32_t rows,
+                                                         uint32_t* __restrict__ glob) {
+    const int pass = blockIdx.x, d = threadIdx.x;
+    uint32_t sum = 0;
+    uint32_t b = blockIdx.y;
+    for (; b + 7 * kOsSlices < rows; b += 8 * kOsSlices) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            v[j] = part[(uint64_t)(b + j * kOsSlices) * (kSortMaxPasses * kSortBins) +
+                        pass * kSortBins + d];
+#pragma unroll
+        for (int j = 0; j < 8; j++) sum += v[j];
+    }
+    for (; b < rows; b += kOsSlices)
+        sum += part[(uint64_t)b * (kSortMaxPasses * kSortBins) + pass * kSortBins + d];
+    if (sum) atomicAdd(&glob[pass * kSortBins + d], sum);
 }
 
 // ---------------------------------------------------------------------------
@@ -420,18 +526,39 @@ __global__ __launch_bounds__(256) void k_morton(const double2* __restrict__ pos,
                                                 const double* __restrict__ box,
                                                 const uint32_t* __restrict__ sys_of,
                                                 uint64_t* __restrict__ keys,
-                                                uint32_t* __restrict__ vals) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const double* bx = box + (uint64_t)kBoxStride * sys_of[i];
-    const double x_min = bx[0], x_max = bx[1], y_min = bx[2], y_max = bx[3];
-    const double inv_w = 1.0 / (x_max - x_min);
-    const double inv_h = 1.0 / (y_max - y_min);
-    const double2 p = pos[i];
-    const uint32_t ix = quantise(p.x, x_min, inv_w);
-    const uint32_t iy = quantise(p.y, y_min, inv_h);
-    keys[i] = (spread_bits(ix) << 1) | spread_bits(iy);
-    vals[i] = i;
+                                                uint32_t* __restrict__ vals, int npass,
+                                                uint32_t* __restrict__ sort_part) {
+    __shared__ uint32_t h[kSortMaxPasses][kSortBins];
+    os_hist_clear(h);
+    // every load of the workgroup's keys is issued before the first use
+    double2 pp[kKeyItems];
+    uint32_t ss[kKeyItems];
+#pragma unroll
+    for (int r = 0; r < kKeyItems; r++) {
+        const uint32_t i = blockIdx.x * kKeyTile + r * 256 + threadIdx.x;
+        pp[r] = i < n ? pos[i] : make_double2(0.0, 0.0);
+        ss[r] = i < n ? sys_of[i] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kKeyItems; r++) {
+        const uint32_t i = blockIdx.x * kKeyTile + r * 256 + threadIdx.x;
+        const bool valid = i < n;
+        uint64_t key = 0;
+        if (valid) {
+            const double* bx = box + (uint64_t)kBoxStride * ss[r];
+            const double x_min = bx[0], x_max = bx[1], y_min = bx[2], y_max = bx[3];
+            const double inv_w = 1.0 / (x_max - x_min);
+            const double inv_h = 1.0 / (y_max - y_min);
+            const uint32_t ix = quantise(pp[r].x, x_min, inv_w);
+            const uint32_t iy = quantise(pp[r].y, y_min, inv_h);
+            key = (spread_bits(ix) << 1) | spread_bits(iy);
+            keys[i] = key;
+            vals[i] = i;
+        }
+        os_hist_add(h, key, ss[r], npass, valid);
+    }
+    os_hist_flush(h, sort_part, npass);
 }
 
 // Keys of the reference CPU quadtree (REF_CPU cells): the path of each body
@@ -443,15 +570,9 @@ __global__ __launch_bounds__(256) void k_morton(const double2* __restrict__ pos,
 // parent's position with size / 2), so a body's digits are its reference
 // child slots; the 4 low key bits stay zero. HBM traffic: 16 B read + 12 B
 // written per body (the descent is ~200 FP64 operations in registers).
-__global__ __launch_bounds__(256) void k_morton_ref(const double2* __restrict__ pos, uint32_t n,
-                                                    const double* __restrict__ box,
-                                                    uint64_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ vals) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ uint64_t ref_key(double2 p, const double* __restrict__ box) {
     double px = box[0], py = box[2];
     double sx = box[1] - box[0], sy = box[3] - box[2];
-    const double2 p = pos[i];
     uint64_t key = 0;
 #pragma unroll 6
     for (int L = 0; L < kMaxLevel; L++) {
@@ -463,145 +584,282 @@ __global__ __launch_bounds__(256) void k_morton_ref(const double2* __restrict__ 
         sx = sx / 2.0;
         sy = sy / 2.0;
     }
-    keys[i] = key;
-    vals[i] = i;
+    return key;
+}
+
+__global__ __launch_bounds__(256) void k_morton_ref(const double2* __restrict__ pos, uint32_t n,
+                                                    const double* __restrict__ box,
+                                                    uint64_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals,
+                                                    uint32_t* __restrict__ sort_part) {
+    __shared__ uint32_t h[kSortMaxPasses][kSortBins];
+    os_hist_clear(h);
+    double2 pp[kKeyItems];
+#pragma unroll
+    for (int r = 0; r < kKeyItems; r++) {
+        const uint32_t i = blockIdx.x * kKeyTile + r * 256 + threadIdx.x;
+        pp[r] = i < n ? pos[i] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int r = 0; r < kKeyItems; r++) {
+        const uint32_t i = blockIdx.x * kKeyTile + r * 256 + threadIdx.x;
+        const bool valid = i < n;
+        uint64_t key = 0;
+        if (valid) {
+            key = ref_key(pp[r], box);
+            keys[i] = key;
+            vals[i] = i;
+        }
+        os_hist_add(h, key, 0u, 8, valid);
+    }
+    os_hist_flush(h, sort_part, 8);
 }
 
 void launch_morton(nb_ctx* c, int src) {
     const uint32_t n = (uint32_t)c->n;
-    if (c->p.cells == NB_CELLS_REF_CPU) {
-        hipLaunchKernelGGL(k_morton_ref, dim3((n + 255) / 256), dim3(256), 0, c->stream,
-                           c->b.pos[src], n, c->b.box, c->b.keys[0], c->b.vals[0]);
-        return;
-    }
-    hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->b.pos[src], n,
-                       c->b.box, c->b.sys_of, c->b.keys[0], c->b.vals[0]);
+    const uint32_t grid = (n + kKeyTile - 1) / kKeyTile;
+    const int npass = 64 / kSortBits + (c->nsys > 1 ? 2 : 0);
+    if (c->p.cells == NB_CELLS_REF_CPU)
+        hipLaunchKernelGGL(k_morton_ref, dim3(grid), dim3(256), 0, c->stream, c->b.pos[src], n,
+                           c->b.box, c->b.keys[0], c->b.vals[0], c->b.sort_part);
+    else
+        hipLaunchKernelGGL(k_morton, dim3(grid), dim3(256), 0, c->stream, c->b.pos[src], n,
+                           c->b.box, c->b.sys_of, c->b.keys[0], c->b.vals[0], npass,
+                           c->b.sort_part);
+    hipLaunchKernelGGL(k_os_reduce, dim3(npass, kOsSlices), dim3(kSortBins), 0, c->stream,
+                       c->b.sort_part, grid, c->b.sort_glob);
 }
 
 // ---------------------------------------------------------------------------
 // LSD radix sort, 8-bit digits, 8 passes (keys ping-pong keys[0] -> keys[1]
-// -> ... -> keys[0]). Per pass:
-//   hist    per-tile digit histogram (wave-aggregated LDS adds)
-//   scan    one workgroup per digit scans that digit's column across tiles
-//   scatter stable rank inside the tile (8 ballots per key give the lanes of
-//           the wave holding the same digit), then a coalesced-by-digit store
-// HBM traffic per pass: 8 B (hist) + 12 B read + 12 B written per key.
+// -> ... -> keys[0]), one launch per pass (a onesweep sort). The global
+// digit histogram of every pass is counted by the Morton kernel as it makes
+// the keys. Per pass:
+//   k_os_pass  per tile of kSortTile keys: a stable rank of every key
+//              among the tile's keys of its digit (8 ballots give a key's
+//              same-digit lanes); the tile's digit counts are published, and
+//              its exclusive prefix over the earlier tiles is read back from
+//              what they and their groups published (a decoupled look-back,
+//              see below); the tile is reordered by digit in LDS and written
+//              out in runs of equal digits.
+// Published words (u64): flag (1 = count, 2 = inclusive prefix) << 62 |
+// tag << 32 | value, tag = (epoch * 16 + pass) mod 2^30 with an epoch that
+// every sort advances, so words left by earlier sorts never match. The
+// buffers' global histograms and tile counters are cleared by the box kernel
+// that starts every step.
+// HBM traffic per pass: 12 B read + 12 B written per key.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kOsSpinLimit = 1u << 21;
 
-// Lanes of this wave whose digit equals mine (valid lanes only).
-__device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
-    uint64_t peers = ballot64(valid);
+__device__ __forceinline__ uint64_t os_word(uint32_t flag, uint32_t tag, uint32_t v) {
+    return ((uint64_t)flag << 62) | ((uint64_t)tag << 32) | v;
+}
+
+// Exclusive prefix of column `d` over entries 0 .. idx-1 of a published
+// array whose entries are counts (flag 1) or inclusive prefixes (flag 2):
+// walks back from idx - 1, up to 32 entries per round trip, to the nearest
+// inclusive prefix. Returns false if an entry never appears (the wait is
+// cut at kOsSpinLimit rounds).
+__device__ __forceinline__ bool os_lookback(const uint64_t* __restrict__ status, uint32_t idx,
+                                            uint32_t d, uint32_t tag, uint32_t* excl_out) {
+    constexpr int kAhead = 32;
+    uint32_t excl = 0, spins = 0;
+    int64_t k = (int64_t)idx - 1;
+    while (k >= 0) {
+        uint64_t v[kAhead];
 #pragma unroll
-    for (int b = 0; b < kSortBits; b++) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t m = ballot64(valid && bit);
-        peers &= bit ? m : ~m;
+        for (int j = 0; j < kAhead; j++)
+            v[j] = k - j >= 0 ? __hip_atomic_load(status + (uint64_t)(k - j) * kSortBins + d,
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                              : 0ull;
+        bool done = false, stalled = false;
+        int used = 0;
+#pragma unroll
+        for (int j = 0; j < kAhead; j++) {
+            if (done || stalled || k - j < 0) continue;
+            const uint32_t f = (uint32_t)(v[j] >> 62), vt = (uint32_t)(v[j] >> 32) & 0x3fffffffu;
+            if (vt != tag || f == 0u) {
+                stalled = true;  // retry from entry k - j
+                continue;
+            }
+            excl += (uint32_t)v[j];
+            used = j + 1;
+            if (f == 2u) done = true;
+        }
+        if (done) break;
+        k -= used;
+        if (stalled) {
+            if (++spins > kOsSpinLimit) return false;
+            __builtin_amdgcn_s_sleep(1);
+        }
     }
-    return peers;
+    *excl_out = excl;
+    return true;
 }
 
-// SYS: the digit is taken from the system of the key's source position
-// (sys_of[val]) instead of the key, for the passes that keep systems apart.
-template <bool SYS>
-__device__ __forceinline__ uint32_t sort_digit(uint64_t key, uint32_t val, int shift,
-                                               const uint32_t* __restrict__ sys_of) {
-    if (SYS) return (sys_of[val] >> shift) & (kSortBins - 1);
-    return (uint32_t)(key >> shift) & (kSortBins - 1);
-}
-
-template <bool SYS>
-__global__ __launch_bounds__(kSortThreads) void k_rs_hist(const uint64_t* __restrict__ keys,
-                                                          const uint32_t* __restrict__ vals,
-                                                          const uint32_t* __restrict__ sys_of,
-                                                          uint32_t n, int shift,
-                                                          uint32_t* __restrict__ hist,
-                                                          uint32_t tiles) {
-    __shared__ uint32_t h[kSortBins];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-#pragma unroll 4
-    for (int r = 0; r < kSortItems; r++) {
-        const uint64_t idx = base + (uint64_t)r * kSortThreads + threadIdx.x;
-        const bool valid = idx < n;
-        const uint32_t d =
-            valid ? sort_digit<SYS>(SYS ? 0ull : keys[idx], SYS ? vals[idx] : 0u, shift, sys_of)
-                  : 0u;
-        const uint64_t peers = digit_peers(d, valid);
-        if (valid && (peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+// Sum of the counts (flag 1) of entries [first, first + cnt) of column d,
+// cnt <= kOsGroup, waiting (bounded) for entries not yet published.
+constexpr int kOsGroup = 16;  // tiles per look-back group
+__device__ __forceinline__ bool os_sum_counts(const uint64_t* __restrict__ status, uint32_t first,
+                                              uint32_t cnt, uint32_t d, uint32_t tag,
+                                              uint32_t* out) {
+    uint32_t sum = 0, have = 0, spins = 0;
+    while (have < cnt) {
+        uint64_t v[kOsGroup];
+#pragma unroll
+        for (int j = 0; j < kOsGroup; j++)
+            v[j] = have + j < cnt
+                       ? __hip_atomic_load(status + (uint64_t)(first + have + j) * kSortBins + d,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                       : 0ull;
+        uint32_t got = 0;
+        bool stop = false;
+#pragma unroll
+        for (int j = 0; j < kOsGroup; j++) {
+            if (stop || have + j >= cnt) continue;
+            const uint32_t f = (uint32_t)(v[j] >> 62), vt = (uint32_t)(v[j] >> 32) & 0x3fffffffu;
+            if (vt != tag || f == 0u) {
+                stop = true;
+                continue;
+            }
+            sum += (uint32_t)v[j];
+            got++;
+        }
+        have += got;
+        if (have < cnt) {
+            if (++spins > kOsSpinLimit) return false;
+            __builtin_amdgcn_s_sleep(1);
+        }
     }
-    __syncthreads();
-    hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+    *out = sum;
+    return true;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_rs_scan(uint32_t* __restrict__ hist,
-                                                          uint32_t tiles,
-                                                          uint32_t* __restrict__ tot) {
-    __shared__ uint32_t s[kSortThreads / 64 + 1];
-    uint32_t* col = hist + (uint64_t)blockIdx.x * tiles;
-    uint32_t carry = 0;
-    for (uint32_t b0 = 0; b0 < tiles; b0 += kSortThreads) {
-        const uint32_t k = b0 + threadIdx.x;
-        const uint32_t v = k < tiles ? col[k] : 0u;
-        uint32_t sum;
-        const uint32_t ex = block_excl_scan<kSortThreads>(v, s, &sum);
-        if (k < tiles) col[k] = carry + ex;
-        carry += sum;
-    }
-    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
-}
+// One pass: 512 threads (8 waves) per tile of kSortTile keys, tile =
+// workgroup index. Workgroups are dispatched in index order (per XCD), so
+// the lowest unfinished tile is always running and every wait below ends;
+// the bounded spins only guard against that assumption failing, and then
+// flag the step (FLAG_SORT) instead of hanging.
+// Prefix of a tile: its own count is published at once; the counts of the
+// earlier tiles of its group of kOsGroup are summed (one round trip), and
+// the groups before it are looked back over (counts / inclusive prefixes
+// published by each group's last tile): about two round trips after the
+// counts appear, however many tiles there are.
+constexpr int kOsThreads = 512;
+constexpr int kOsItems = kSortTile / kOsThreads;
+static_assert(kOsThreads * kOsItems == kSortTile, "tile size");
 
 template <bool SYS>
-__global__ __launch_bounds__(kSortThreads) void k_rs_scatter(
+__global__ __launch_bounds__(kOsThreads) void k_os_pass(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, const uint32_t* __restrict__ sys_of, uint32_t n, int shift,
-    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ tot, uint32_t tiles) {
-    constexpr int NW = kSortThreads / 64;
-    __shared__ uint32_t run[kSortBins];      // next output position of each digit
-    __shared__ uint32_t wcnt[NW][kSortBins]; // keys of each digit per wave, this round
-    __shared__ uint32_t s[NW + 1];
-    const int t = threadIdx.x, w = t >> 6;
-    {
-        uint32_t sum;
-        const uint32_t ex = block_excl_scan<kSortThreads>(tot[t], s, &sum);
-        run[t] = ex + hist[(uint64_t)t * tiles + blockIdx.x];
-#pragma unroll
-        for (int k = 0; k < NW; k++) wcnt[k][t] = 0;
-    }
-    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-    uint64_t key[kSortItems];
-    uint32_t val[kSortItems];
-#pragma unroll
-    for (int r = 0; r < kSortItems; r++) {
-        const uint64_t idx = base + (uint64_t)r * kSortThreads + t;
-        key[r] = idx < n ? kin[idx] : 0ull;
-        val[r] = idx < n ? vin[idx] : 0u;
-    }
+    uint32_t* __restrict__ vout, const uint32_t* __restrict__ sys_of, uint32_t n, int pass,
+    const uint32_t* __restrict__ glob, uint64_t* __restrict__ status, uint64_t* __restrict__ gstatus,
+    uint32_t* __restrict__ meta) {
+    constexpr int NW = kOsThreads / 64;
+    constexpr int kWaveKeys = kSortTile / NW;  // each wave ranks a contiguous part
+    __shared__ uint64_t s_key[kSortTile];
+    __shared__ uint32_t s_val[kSortTile];
+    __shared__ uint8_t s_dig[kSortTile];
+    __shared__ uint32_t wc[NW][kSortBins];   // per wave: keys of each digit, then its offset
+    __shared__ uint32_t s_off[kSortBins];    // first slot of each digit in the tile
+    __shared__ uint32_t s_dst[kSortBins];    // output position of each digit's first key
+    __shared__ uint32_t s_scan[NW + 1];
+    const int t = threadIdx.x, w = t >> 6, l = lane_id();
+    for (int k = t; k < NW * kSortBins; k += kOsThreads) (&wc[0][0])[k] = 0u;
     __syncthreads();
+    const uint32_t tile = blockIdx.x;
+    const uint64_t base = (uint64_t)tile * kSortTile;
+    const uint32_t nvalid = (uint32_t)min((uint64_t)kSortTile, (uint64_t)n - base);
+    // Wave w ranks keys [w * kWaveKeys, (w + 1) * kWaveKeys) of the tile in
+    // rounds of 64 consecutive keys, against its own digit counters (LDS
+    // accesses of one wave are ordered: no barrier): stable within the wave.
+    uint64_t key[kOsItems];
+    uint32_t val[kOsItems], rank[kOsItems];
+    uint8_t dig[kOsItems];
 #pragma unroll
-    for (int r = 0; r < kSortItems; r++) {
-        const uint64_t idx = base + (uint64_t)r * kSortThreads + t;
-        const bool valid = idx < n;
-        const uint32_t d = valid ? sort_digit<SYS>(key[r], val[r], shift, sys_of) : 0u;
+    for (int r = 0; r < kOsItems; r++) {
+        const uint32_t k = (uint32_t)(w * kWaveKeys + r * 64 + l);
+        const bool valid = k < nvalid;
+        key[r] = valid ? kin[base + k] : 0ull;
+        val[r] = valid ? vin[base + k] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kOsItems; r++) {
+        const uint32_t k = (uint32_t)(w * kWaveKeys + r * 64 + l);
+        const bool valid = k < nvalid;
+        const uint32_t d = valid ? os_digit(key[r], SYS ? sys_of[val[r]] : 0u, pass) : 0u;
+        dig[r] = (uint8_t)d;
         const uint64_t peers = digit_peers(d, valid);
         const uint64_t below = peers & lanemask_lt();
-        if (valid && below == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t dst = run[d] + (uint32_t)__popcll(below);
-            for (int k = 0; k < w; k++) dst += wcnt[k][d];
-            kout[dst] = key[r];
-            vout[dst] = val[r];
-        }
-        __syncthreads();
-        uint32_t add = 0;
+        const uint32_t before = valid ? wc[w][d] : 0u;
+        rank[r] = before + (uint32_t)__popcll(below);
+        if (valid && below == 0) wc[w][d] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    // threads t < 256 own digit t: its count in the tile and each wave's offset
+    const bool owner = t < kSortBins;
+    uint32_t cnt = 0;
+    if (owner) {
 #pragma unroll
         for (int k = 0; k < NW; k++) {
-            add += wcnt[k][t];
-            wcnt[k][t] = 0;
+            const uint32_t c = wc[k][t];
+            wc[k][t] = cnt;
+            cnt += c;
         }
-        run[t] += add;
-        __syncthreads();
+    }
+    uint32_t total;
+    const uint32_t toff = block_excl_scan<kOsThreads>(cnt, s_scan, &total);
+    if (owner) s_off[t] = toff;
+    // the tile's exclusive prefix over the earlier tiles, digit by digit
+    const uint32_t tag = (glob[kSortGlobEpoch] * 16u + (uint32_t)pass) & 0x3fffffffu;
+    const uint32_t g = tile / kOsGroup, j = tile % kOsGroup;
+    uint32_t excl = 0;
+    if (owner) {
+        __hip_atomic_store(status + (uint64_t)tile * kSortBins + t, os_word(1u, tag, cnt),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t within = 0, groups = 0;
+        bool ok = os_sum_counts(status, g * kOsGroup, j, (uint32_t)t, tag, &within);
+        uint64_t* gmine = gstatus + (uint64_t)g * kSortBins + t;
+        const bool last = j == kOsGroup - 1;
+        if (last)  // the group's count (its inclusive prefix for group 0)
+            __hip_atomic_store(gmine, os_word(g == 0 ? 2u : 1u, tag, within + cnt),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (g > 0) ok = ok && os_lookback(gstatus, g, (uint32_t)t, tag, &groups);
+        if (last && g > 0)
+            __hip_atomic_store(gmine, os_word(2u, tag, groups + within + cnt), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        if (!ok) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_SORT);  // never expected
+        excl = groups + within;
+    }
+    uint32_t gtotal;
+    const uint32_t gex =
+        block_excl_scan<kOsThreads>(owner ? glob[pass * kSortBins + t] : 0u, s_scan, &gtotal);
+    if (owner) s_dst[t] = gex + excl;
+    __syncthreads();
+    // the tile in digit order in LDS, then runs of equal digits written out
+#pragma unroll
+    for (int r = 0; r < kOsItems; r++) {
+        const uint32_t k = (uint32_t)(w * kWaveKeys + r * 64 + l);
+        if (k < nvalid) {
+            const uint32_t slot = s_off[dig[r]] + wc[w][dig[r]] + rank[r];
+            s_key[slot] = key[r];
+            s_val[slot] = val[r];
+            s_dig[slot] = dig[r];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kOsItems; r++) {
+        const uint32_t slot = (uint32_t)r * kOsThreads + t;
+        if (slot < nvalid) {
+            const uint32_t d = s_dig[slot];
+            const uint32_t dst = s_dst[d] + (slot - s_off[d]);
+            if (dst < n) {  // always, unless a wait was cut short (FLAG_SORT)
+                kout[dst] = s_key[slot];
+                vout[dst] = s_val[slot];
+            }
+        }
     }
 }
 
@@ -611,30 +869,21 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_scatter(
 void launch_radix_sort(nb_ctx* c) {
     const uint32_t n = (uint32_t)c->n;
     const uint32_t tiles = c->sort_tiles;
-    const int key_passes = 64 / kSortBits, sys_passes = c->nsys > 1 ? 2 : 0;
-    for (int pass = 0; pass < key_passes + sys_passes; pass++) {
-        const bool sys = pass >= key_passes;
-        const int shift = (sys ? pass - key_passes : pass) * kSortBits;
+    const int key_passes = 64 / kSortBits, npass = key_passes + (c->nsys > 1 ? 2 : 0);
+    // the digit histograms were counted by the Morton kernel
+    for (int pass = 0; pass < npass; pass++) {
         const int in = pass & 1, out = in ^ 1;
-        if (sys)
-            hipLaunchKernelGGL(k_rs_hist<true>, dim3(tiles), dim3(kSortThreads), 0, c->stream,
-                               c->b.keys[in], c->b.vals[in], c->b.sys_of, n, shift,
-                               c->b.sort_hist, tiles);
-        else
-            hipLaunchKernelGGL(k_rs_hist<false>, dim3(tiles), dim3(kSortThreads), 0, c->stream,
-                               c->b.keys[in], c->b.vals[in], c->b.sys_of, n, shift,
-                               c->b.sort_hist, tiles);
-        hipLaunchKernelGGL(k_rs_scan, dim3(kSortBins), dim3(kSortThreads), 0, c->stream,
-                           c->b.sort_hist, tiles, c->b.sort_tot);
-        if (sys)
-            hipLaunchKernelGGL(k_rs_scatter<true>, dim3(tiles), dim3(kSortThreads), 0, c->stream,
+        uint64_t* gstatus = c->b.sort_status + (uint64_t)tiles * kSortBins;
+        if (pass >= key_passes)
+            hipLaunchKernelGGL(k_os_pass<true>, dim3(tiles), dim3(kOsThreads), 0, c->stream,
                                c->b.keys[in], c->b.vals[in], c->b.keys[out], c->b.vals[out],
-                               c->b.sys_of, n, shift, c->b.sort_hist, c->b.sort_tot, tiles);
+                               c->b.sys_of, n, pass, c->b.sort_glob, c->b.sort_status, gstatus,
+                               c->b.meta);
         else
-            hipLaunchKernelGGL(k_rs_scatter<false>, dim3(tiles), dim3(kSortThreads), 0,
-                               c->stream, c->b.keys[in], c->b.vals[in], c->b.keys[out],
-                               c->b.vals[out], c->b.sys_of, n, shift, c->b.sort_hist,
-                               c->b.sort_tot, tiles);
+            hipLaunchKernelGGL(k_os_pass<false>, dim3(tiles), dim3(kOsThreads), 0, c->stream,
+                               c->b.keys[in], c->b.vals[in], c->b.keys[out], c->b.vals[out],
+                               c->b.sys_of, n, pass, c->b.sort_glob, c->b.sort_status, gstatus,
+                               c->b.meta);
     }
 }
 
@@ -649,7 +898,8 @@ __global__ __launch_bounds__(256) void k_gather(
     double* __restrict__ mout, uint32_t* __restrict__ oout, uint32_t n) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const uint32_t s = perm[i];
+    uint32_t s = perm[i];
+    if (s >= n) s = i;  // only after a stalled sort (FLAG_SORT), whose step is void
     pout[i] = pin[s];
     vout[i] = vin[s];
     mout[i] = min_[s];
