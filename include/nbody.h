@@ -159,6 +159,8 @@ typedef struct nb_node {
     double mass, cx, cy, wsq;
     float ixx, ixy, iyy;  /* second moments about the COM */
     int32_t child_mask;   /* bit q set = child slot c0+q holds bodies */
+    float t3[4];          /* third moments about the COM {xxx, xxy, xyy, yyy}: kept
+                             for the learned far field (else zero) */
 } nb_node;
 
 /* Fills *p with the defaults: G=1, dt=1e-3, theta=0.5, softening=0.01,
@@ -305,16 +307,17 @@ int nb_autotune(nb_ctx* ctx, uint32_t repeats, nb_tuning* out);
  * is the monopole of every accepted cell (BarnesHutCuda.cu:461-471). The
  * correction of an accepted cell, a_exact - a_monopole, is approximated by
  * the cell's softened quadrupole term, computed from its second moments, plus
- * s/R times a network (8 -> 64 -> 64 -> 2, ReLU, BF16 weights and activations
- * on MFMA, FP32 accumulation) trained on the rest over s/R, from the cell's
- * offset, size/distance, mass fraction and second moments; a context with
- * far_field = NB_FAR_LEARNED adds it for every accepted internal cell.
- * Parameters are FP32 in the order W1[64][8], b1[64], W2[64][64], b2[64],
- * W3[2][64], b3[2] (NB_MLP_PARAMS values); a new context holds zeros (the
- * quadrupole term alone).
+ * s/R times a network (12 -> 64 -> 64 -> 2, ReLU, BF16 weights and
+ * activations on MFMA, FP32 accumulation) trained on the rest over s/R, from
+ * the cell's offset, size/distance, mass fraction, second moments and third
+ * moments (the octupole the rest is led by); a context with far_field =
+ * NB_FAR_LEARNED adds it for every accepted internal cell. Parameters are
+ * FP32 in the order W1[64][12], b1[64], W2[64][64], b2[64], W3[2][64], b3[2]
+ * (NB_MLP_PARAMS values); a new context holds zeros (the quadrupole term
+ * alone).
  * ------------------------------------------------------------------------- */
-#define NB_MLP_PARAMS 4866
-#define NB_MLP_INPUTS 8
+#define NB_MLP_PARAMS 5122
+#define NB_MLP_INPUTS 12
 
 int nb_mlp_set_weights(nb_ctx* ctx, const float* params, uint64_t count);
 int nb_mlp_get_weights(nb_ctx* ctx, float* params, uint64_t count);
@@ -326,7 +329,7 @@ int nb_mlp_init(nb_ctx* ctx, uint64_t seed);
  * tree, walks it from `sample_bodies` seeded bodies with the per-body opening
  * test at `theta` (cells holding the body are never accepted), and keeps
  * every accepted internal cell of at most 8,192 bodies, up to `max_pairs`.
- * Each pair gets its 8 inputs and its target, the exact FP64 Plummer
+ * Each pair gets its NB_MLP_INPUTS inputs and its target, the exact FP64 Plummer
  * acceleration of the cell's bodies minus the cell's monopole and minus its
  * softened quadrupole term, in the network's output units, over s/R (the
  * input f2).
@@ -334,7 +337,7 @@ int nb_mlp_init(nb_ctx* ctx, uint64_t seed);
 int nb_mlp_generate(nb_ctx* ctx, double theta, uint32_t sample_bodies, uint64_t max_pairs,
                     uint64_t seed, uint64_t* pairs_out);
 
-/* Copies up to `cap` held pairs: features [k][8], targets [k][2], the sample
+/* Copies up to `cap` held pairs: features [k][12], targets [k][2], the sample
  * body (index in the tree's sorted order) and the cell's tree slot. Any
  * output may be NULL; *count = pairs held. */
 int nb_mlp_get_pairs(nb_ctx* ctx, float* features, float* targets, uint32_t* body,

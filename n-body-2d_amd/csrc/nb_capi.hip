@@ -66,15 +66,20 @@ void dfree(T*& p) {
 bool is_bh(const nb_ctx* c) { return c->p.algorithm == NB_ALGO_BARNES_HUT; }
 bool is_dk(const nb_ctx* c) { return c->p.integrator == NB_INTEG_DK; }
 
-uint64_t tree_bytes(uint64_t icap, uint64_t nsys) {
+uint64_t tree_bytes(uint64_t icap, uint64_t nsys, bool mom3) {
     const uint64_t ng = (icap + 255) / 256;
-    return (nsys + 4 * icap) * sizeof(NodeRec) + icap * (2 * sizeof(uint32_t) + 2 + sizeof(uint4)) +
-           kMaxLevel * ng * sizeof(uint32_t);
+    return (nsys + 4 * icap) * (sizeof(NodeRec) + (mom3 ? sizeof(float4) : 0)) +
+           icap * (2 * sizeof(uint32_t) + 2 + sizeof(uint4)) + kMaxLevel * ng * sizeof(uint32_t);
 }
 
+// Third moments are kept for the learned far field's inputs only (contexts
+// with far_field = LEARNED, and any context once it generated training pairs).
+bool wants_mom3(const nb_ctx* c) { return c->keep_mom3; }
+
 void free_tree(nb_ctx* c) {
-    if (c->icap) c->device_bytes -= tree_bytes(c->icap, c->nsys);
+    if (c->icap) c->device_bytes -= tree_bytes(c->icap, c->nsys, wants_mom3(c));
     dfree(c->b.nodes);
+    dfree(c->b.mom3);
     dfree(c->b.inode_slot);
     dfree(c->b.inode_level);
     dfree(c->b.inode_mask);
@@ -88,13 +93,14 @@ int alloc_tree(nb_ctx* c, uint64_t icap) {
     free_tree(c);
     const uint64_t before = c->device_bytes;
     NB_HIP(c, dalloc(c, &c->b.nodes, c->nsys + 4 * icap));
+    if (wants_mom3(c)) NB_HIP(c, dalloc(c, &c->b.mom3, c->nsys + 4 * icap));
     NB_HIP(c, dalloc(c, &c->b.inode_slot, icap));
     NB_HIP(c, dalloc(c, &c->b.inode_level, icap));
     NB_HIP(c, dalloc(c, &c->b.inode_mask, icap));
     NB_HIP(c, dalloc(c, &c->b.node_a, icap));
     NB_HIP(c, dalloc(c, &c->b.by_level, icap));
     NB_HIP(c, dalloc(c, &c->b.lvl_cnt, kMaxLevel * ((icap + 255) / 256)));
-    c->device_bytes = before + tree_bytes(icap, c->nsys);
+    c->device_bytes = before + tree_bytes(icap, c->nsys, wants_mom3(c));
     c->icap = icap;
     return NB_OK;
 }
@@ -408,6 +414,19 @@ int build_and_force(nb_ctx* c, bool forces) {
 
 namespace nb {
 int build_tree(nb_ctx* c) { return build_and_force(c, false); }
+
+int ensure_mom3(nb_ctx* c) {
+    if (c->keep_mom3) return NB_OK;
+    // Captured steps hold the tree buffers' addresses (no third moments):
+    // they are recaptured with the new buffer.
+    destroy_graphs(c);
+    if (c->icap) {
+        NB_HIP(c, dalloc(c, &c->b.mom3, c->nsys + 4 * c->icap));
+        c->device_bytes += (c->nsys + 4 * c->icap) * sizeof(float4);
+    }
+    c->keep_mom3 = true;
+    return NB_OK;
+}
 }  // namespace nb
 
 extern "C" {
@@ -472,6 +491,7 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
     c->n = n;
     c->nsys = nsys;
     c->dt = p->dt;
+    c->keep_mom3 = p->far_field == NB_FAR_LEARNED;
     auto bail = [&](int code) {
         g_err = c->err;
         free_all(c);
@@ -1026,6 +1046,9 @@ int nb_debug_tree(nb_ctx* c, nb_node* nodes, uint64_t cap) {
     if (cap < count) return fail(c, NB_ERR_INVALID_ARG, "cap smaller than the node count");
     std::vector<NodeRec> h(count);
     NB_HIP(c, hipMemcpy(h.data(), c->b.nodes, count * sizeof(NodeRec), hipMemcpyDeviceToHost));
+    std::vector<float4> t3(c->b.mom3 ? count : 0);
+    if (c->b.mom3)
+        NB_HIP(c, hipMemcpy(t3.data(), c->b.mom3, count * sizeof(float4), hipMemcpyDeviceToHost));
     // Empty child slots are not written on the device (their parent's mask
     // marks them); they are reported here as empty nodes. A parent's slot
     // precedes its children's, so one forward pass finds them.
@@ -1063,6 +1086,12 @@ int nb_debug_tree(nb_ctx* c, nb_node* nodes, uint64_t cap) {
         o.ixy = r.ixy;
         o.iyy = r.iyy;
         o.child_mask = r.c0 >= 1 ? r.mask : 0;
+        if (!t3.empty()) {
+            o.t3[0] = t3[s].x;
+            o.t3[1] = t3[s].y;
+            o.t3[2] = t3[s].z;
+            o.t3[3] = t3[s].w;
+        }
     }
     return NB_OK;
 }

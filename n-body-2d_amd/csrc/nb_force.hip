@@ -263,6 +263,38 @@ __device__ __forceinline__ void eval_list(const float4* __restrict__ list, int c
     kahan_add(a.ay, a.cy, py);
 }
 
+// The learned far field's list: every entry also carries its second moments
+// about the COM (ixx in the entry's w, ixy and iyy in `quad`; zero for single
+// bodies), and each body adds the cell's softened quadrupole term at its own
+// position to the monopole,
+//   a = m d / R^3 + (3 / (2 R^5)) [(5 (d.I d) / R^2 - tr I) d - 2 I d],
+// d = COM - body, R^2 = |d|^2 + eps^2: 36 flops per entry.
+template <bool ZERO_EPS>
+__device__ __forceinline__ void eval_list_quad(const float4* __restrict__ list,
+                                               const float2* __restrict__ quad, int cnt,
+                                               float bx, float by, float eps2f, Acc& a) {
+    float px = 0.f, py = 0.f;
+#pragma unroll 4
+    for (int e = 0; e < cnt; e++) {
+        const float4 d = list[e];
+        const float2 q = quad[e];
+        const float dx = d.x - bx, dy = d.y - by;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2f));
+        float rs = __builtin_amdgcn_rsqf(r2);
+        if (ZERO_EPS) rs = r2 > 0.f ? rs : 0.f;
+        const float rs2 = rs * rs, rs3 = rs2 * rs, rs5 = rs3 * rs2;
+        const float idx = fmaf(d.w, dx, q.x * dy), idy = fmaf(q.x, dx, q.y * dy);
+        const float S = fmaf(dx, idx, dy * idy);
+        const float cq = 1.5f * rs5;
+        const float cd = fmaf(cq, fmaf(5.f * S, rs2, -(d.w + q.y)), d.z * rs3);
+        const float ci = -2.f * cq;
+        px = fmaf(dx, cd, fmaf(ci, idx, px));
+        py = fmaf(dy, cd, fmaf(ci, idy, py));
+    }
+    kahan_add(a.ax, a.cx, px);
+    kahan_add(a.ay, a.cy, py);
+}
+
 // Work sharing: a subgroup whose list passes the budget is split into pieces
 // of 8 bodies (single bodies below that); all pieces but one go to a queue in device memory (meta[META_QHEAD], meta[META_QTAIL]
 // count pushes and pops; an entry is {group, first lane << 8 | lanes} + 1 in
@@ -315,7 +347,9 @@ __device__ __forceinline__ bool queue_pop(uint32_t* meta, uint64_t* queue, uint3
     return hi != 0;
 }
 
-// LEARN: the walk leaves the far-field network to k_mlp_apply. The tree slot
+// LEARN: each body adds the accepted cells' softened quadrupole terms at its
+// own position (eval_list_quad) and the walk leaves the far-field network to
+// k_mlp_apply. The tree slot
 // of every accepted internal cell is also appended to `cell_slot`, in the
 // region of the group the wavefront was launched for; each finished subgroup
 // k of group g gets subs[g][k] (its box centre and its cell range) from a
@@ -335,6 +369,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     if (meta[META_FLAGS] & kHaltFlags) return;
     __shared__ __attribute__((aligned(16))) int32_t s_stack[kGStack];
     __shared__ float4 s_list[kGList];
+    __shared__ float2 s_quad[LEARN ? kGList : 1];  // LEARN: ixy, iyy of each entry
     __shared__ int32_t s_todo[16];
     __shared__ unsigned long long s_own;  // lanes whose own single-body leaf is listed
     const int l = lane_id();
@@ -472,7 +507,10 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                 const uint64_t am = ballot64(add);
                 if (add) {
                     const int at = cnt + (int)__popcll(am & lanemask_lt());
-                    s_list[at] = make_float4((float)dxc, (float)dyc, r.mf, 0.f);
+                    const bool cell = c0 >= 0;
+                    s_list[at] = make_float4((float)dxc, (float)dyc, r.mf,
+                                             LEARN && cell ? r.ixx : 0.f);
+                    if (LEARN) s_quad[at] = cell ? make_float2(r.ixy, r.iyy) : make_float2(0.f, 0.f);
                     // a member's own single-body leaf: a zero term, not counted
                     if (c0 < 0 && bs >= g_first && bs <= g_last)
                         atomicOr(&s_own, 1ull << (bs - (int32_t)first));
@@ -523,7 +561,10 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                     break;
                 }
                 if (cnt > kGFlush) {
-                    eval_list<ZERO_EPS, REFF>(s_list, cnt, bx, by, eps2f, t);
+                    if (LEARN)
+                        eval_list_quad<ZERO_EPS>(s_list, s_quad, cnt, bx, by, eps2f, t);
+                    else
+                        eval_list<ZERO_EPS, REFF>(s_list, cnt, bx, by, eps2f, t);
                     listed += cnt;
                     cnt = 0;
                 }
@@ -550,7 +591,10 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                 continue;
             }
             if (cnt > 0) {
-                eval_list<ZERO_EPS, REFF>(s_list, cnt, bx, by, eps2f, t);
+                if (LEARN)
+                    eval_list_quad<ZERO_EPS>(s_list, s_quad, cnt, bx, by, eps2f, t);
+                else
+                    eval_list<ZERO_EPS, REFF>(s_list, cnt, bx, by, eps2f, t);
                 listed += cnt;
             }
             // the subgroup is finished: its members write their results

@@ -133,6 +133,8 @@ struct Buffers {
     uint8_t* inode_level;  // [icap]
     uint8_t* inode_mask;   // [icap]
     NodeRec* nodes;        // [1 + 4*icap]
+    float4* mom3;          // [1 + 4*icap] third moments about the COM {T_xxx, T_xxy,
+                           //     T_xyy, T_yyy} of each slot (learned far field only)
     uint32_t* node_a;      // [icap] first body of internal node p
     uint32_t* multi_list;  // [N/2 + 1] slots of multi-body leaves
     uint32_t* lvl_cnt;     // [kMaxLevel * ceil(icap/256)] level counts, then offsets
@@ -218,6 +220,7 @@ struct nb_ctx {
     // run a step returns this error.
     bool dead = false;
     std::string dead_msg;
+    bool keep_mom3 = false;         // the tree keeps third moments (b.mom3)
     uint32_t fail_allocs = 0;       // test hook: allocations left to fail
     std::string err;
 };
@@ -246,6 +249,8 @@ int alloc_cells(nb_ctx* c, uint32_t cap);
 void launch_mlp_apply(nb_ctx* c, int src, bool kick, bool count_step);
 // Builds the tree of the current state (nb_capi.hip).
 int build_tree(nb_ctx* c);
+// Makes the tree keep third moments from the next build on (nb_capi.hip).
+int ensure_mom3(nb_ctx* c);
 
 hipError_t direct_sum_f64(uint64_t n, const double2* pos, const double* mass, double eps2,
                           uint64_t m, const int64_t* targets, double2* part, uint32_t split,

@@ -1,15 +1,16 @@
 // nb_mlp.h — the far-field correction network on MFMA (device side).
 //
-// Network: 8 -> 64 -> 64 -> 2, ReLU, BF16 weights and activations, FP32
-// accumulation. Parameters in the canonical order W1[64][8], b1[64],
-// W2[64][64], b2[64], W3[2][64], b3[2] (4,866 values).
+// Network: 12 -> 64 -> 64 -> 2, ReLU, BF16 weights and activations, FP32
+// accumulation. Parameters in the canonical order W1[64][12], b1[64],
+// W2[64][64], b2[64], W3[2][64], b3[2] (5,122 values).
 //
 // MFMA formulation (D = A B + C, wave64). The weights are A (rows = output
 // neurons), the batch is B (16 evaluation points per tile, one per column);
 // biases are the accumulator's initial value.
 //   layer 1, v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)+v],
 //     B[4(l/16)+v][l%16] (v = 0..3): inputs f0..f3 in lanes 0-15, f4..f7 in
-//     lanes 16-31, zero padding (k = 8..15) in lanes 32-63.
+//     lanes 16-31, f8..f11 in lanes 32-47, zero padding (k = 12..15) in
+//     lanes 48-63.
 //   layers 2, 3, v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)+v],
 //     B[8(l/16)+v][l%16] (v = 0..7).
 //   every layer's D: lane l holds D[4(l/16)+v][l%16] (v = 0..3).
@@ -22,18 +23,21 @@
 //
 // Inputs of a (cell, point) pair, d = COM - point, r = |d|, u = d/r,
 // w = (-u_y, u_x), R^2 = r^2 + eps^2, s = cell side, M = cell mass, I =
-// second moments about the COM, taken in the frame (u, w):
+// second and T = third moments about the COM, taken in the frame (u, w):
 //   f0,f1 = u   f2 = s/R   f3 = log2(M/M_root)/16
-//   f4,f5,f6 = (u.Iu, u.Iw, w.Iw) / (M s^2)   f7 = log2(r/ell)/8
+//   f4,f5,f6 = (I_uu, I_uw, I_ww) / (M s^2)   f7 = log2(r/ell)/8
+//   f8..f11 = (T_uuu, T_uuw, T_uww, T_www) / (M s^3)
 // with ell = eps (or the root cell side when eps = 0). Units of the
 // correction a_exact - a_monopole of the Plummer kernel along (u, w):
 // M s^2 r / R^5, the scale of the softened quadrupole term, which in these
 // units is q = (3/2)(5 rho f4 - 3 f4 - f6, -2 f5), rho = r^2/R^2. The
-// correction applied is q + f2 (y0, y1): q is computed exactly and the
-// network output (y0, y1) is trained on the rest over f2 = s/R (octupole and
-// higher orders, of order s/R in these units, which the second moments only
-// hint at), so every target is of order one and a far cell's share of the
-// network's error is small.
+// network output (y0, y1) is trained on the rest, a_exact - a_monopole - q,
+// over f2 = s/R. The rest is led by the octupole, of order s/R in these
+// units and linear in the third moments f8..f11 (with coefficients that
+// depend on rho), so every target is of order one and a far cell's share of
+// the network's error is small. In a step the walk adds each cell's softened
+// quadrupole term at every body's own position, and f2 (y0, y1) is added at
+// the centre of the body's subgroup.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,11 +45,11 @@
 
 namespace nb {
 
-constexpr int kMlpIn = 8;
+constexpr int kMlpIn = 12;
 constexpr int kMlpHidden = 64;
 constexpr int kMlpOut = 2;
 constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden + kMlpHidden * kMlpHidden +
-                           kMlpHidden + kMlpOut * kMlpHidden + kMlpOut;  // 4866
+                           kMlpHidden + kMlpOut * kMlpHidden + kMlpOut;  // 5122
 // Packed A tiles: layer 1 as 4 x 64 uint2 (4 BF16 per lane), then layer 2
 // (4 output tiles x 2 K blocks) and layer 3 (2 K blocks) as 10 x 64 uint4
 // (8 BF16 per lane).
@@ -128,11 +132,13 @@ __device__ inline bf16x8 pack8_relu(f32x4 a, f32x4 b) {
                                                  relu_bf16x2(pk2(b.z, b.w))));
 }
 
-// The 8 inputs of one (cell, point) pair; returns the output scale and the
-// unit offset (ux, uy) that rotates the output back.
+// The kMlpIn inputs of one (cell, point) pair; returns the output scale and
+// the unit offset (ux, uy) that rotates the output back. t3 = third moments
+// about the COM {T_xxx, T_xxy, T_xyy, T_yyy}.
 __device__ inline float mlp_features(float dx, float dy, float s, float M, float ixx, float ixy,
-                                     float iyy, float eps2, float inv_mroot, float inv_ell,
-                                     float f[8], float& ux, float& uy, float q[2]) {
+                                     float iyy, float4 t3, float eps2, float inv_mroot,
+                                     float inv_ell, float f[kMlpIn], float& ux, float& uy,
+                                     float q[2]) {
     const float r2 = fmaf(dx, dx, dy * dy);
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * inv_r;
@@ -154,37 +160,29 @@ __device__ inline float mlp_features(float dx, float dy, float s, float M, float
     f[5] = uiw * k;
     f[6] = wiw * k;
     f[7] = __log2f(r * inv_ell) * 0.125f;
+    // third moments in the frame (u, w): T_abc u or w in each slot
+    const float k3 = k * __builtin_amdgcn_rcpf(s);
+    const float wx = -uy, wy = ux;
+    // contract one index with u: the 2x2 symmetric matrix A_bc = u_a T_abc
+    const float axx = fmaf(ux, t3.x, uy * t3.y), axy = fmaf(ux, t3.y, uy * t3.z),
+                ayy = fmaf(ux, t3.z, uy * t3.w);
+    // ... or with w: B_bc = w_a T_abc
+    const float bxx = fmaf(wx, t3.x, wy * t3.y), bxy = fmaf(wx, t3.y, wy * t3.z),
+                byy = fmaf(wx, t3.z, wy * t3.w);
+    const float uau = fmaf(ux, fmaf(ux, axx, uy * axy), uy * fmaf(ux, axy, uy * ayy));
+    const float uaw = fmaf(wx, fmaf(ux, axx, uy * axy), wy * fmaf(ux, axy, uy * ayy));
+    const float wbw = fmaf(wx, fmaf(wx, bxx, wy * bxy), wy * fmaf(wx, bxy, wy * byy));
+    const float wbu = fmaf(wx, fmaf(ux, bxx, uy * bxy), wy * fmaf(ux, bxy, uy * byy));
+    f[8] = uau * k3;   // T_uuu
+    f[9] = uaw * k3;   // T_uuw
+    f[10] = wbu * k3;  // T_uww
+    f[11] = wbw * k3;  // T_www
     const float iR2 = inv_R * inv_R;
     // the softened quadrupole term in these units
     const float rho = r2 * iR2;
     q[0] = 1.5f * fmaf(5.f * rho, f[4], -fmaf(3.f, f[4], f[6]));
     q[1] = -3.f * f[5];
     return ms2 * r * (iR2 * iR2 * inv_R);
-}
-
-// Jacobian dQ/dd of the softened quadrupole term Q(d) = (3/(2R^5))
-// [-tr(I) d - 2 I d + 5 (d.Id) d / R^2] (d = COM - point, R^2 = |d|^2 + eps^2),
-// FP32: J = {dQx/ddx, dQx/ddy, dQy/ddx, dQy/ddy}. Moving the point by b
-// changes d by -b, so Q(point + b) ~ Q - J b.
-__device__ inline void quad_jacobian(float dx, float dy, float ixx, float ixy, float iyy,
-                                     float eps2, float J[4]) {
-    const float R2 = fmaf(dx, dx, fmaf(dy, dy, eps2));
-    const float iR = __builtin_amdgcn_rsqf(R2);
-    const float iR2 = iR * iR;
-    const float iR5 = iR2 * iR2 * iR;
-    const float t = ixx + iyy;
-    const float idx = fmaf(ixx, dx, ixy * dy), idy = fmaf(ixy, dx, iyy * dy);
-    const float S = fmaf(dx, idx, dy * idy);
-    const float px = fmaf(5.f * S * iR2, dx, -fmaf(t, dx, 2.f * idx));
-    const float py = fmaf(5.f * S * iR2, dy, -fmaf(t, dy, 2.f * idy));
-    const float k = 1.5f * iR5;
-    const float diag = 5.f * S * iR2 - t;
-    const float a = 5.f * iR2, b = 10.f * iR2, c = 10.f * S * iR2 * iR2;
-    // J_ij = k [ -a d_j P_i - t d_ij - 2 I_ij + b (Id)_j d_i + 5 S R^-2 d_ij - c d_i d_j ]
-    J[0] = k * (-a * dx * px + diag - 2.f * ixx + b * idx * dx - c * dx * dx);
-    J[1] = k * (-a * dy * px - 2.f * ixy + b * idy * dx - c * dx * dy);
-    J[2] = k * (-a * dx * py - 2.f * ixy + b * idx * dy - c * dy * dx);
-    J[3] = k * (-a * dy * py + diag - 2.f * iyy + b * idy * dy - c * dy * dy);
 }
 
 // 1 / ell for the distance input: eps, or the root cell side when eps = 0.
@@ -216,7 +214,8 @@ __device__ inline f32x4 mlp_bias4(const float* sb, int off, int lane) {
 
 // Forward pass of T independent 16-column tiles (interleaved for the MFMA
 // pipeline). x[t] holds this lane's four inputs of tile t (lanes 0-15: f0..f3
-// of column l, lanes 16-31: f4..f7 of column l-16, lanes 32-63: zero).
+// of column l, lanes 16-31: f4..f7 of column l-16, lanes 32-47: f8..f11 of
+// column l-32, lanes 48-63: zero).
 // y[t]: lanes 0-15 hold (y0, y1, 0, 0) of column l.
 template <int T>
 __device__ inline void mlp_forward_tiles(const f32x4* x, const MlpWeights& w, const float* sb,

@@ -5,11 +5,12 @@
 //
 // Pair generation: sample bodies walk the tree with the per-body opening test
 // at theta (cells holding the body itself are never accepted); every accepted
-// internal cell with at most kPairMaxBodies bodies becomes a pair. Its target
-// is the exact FP64 Plummer acceleration of the cell's bodies at the sample
-// body minus the cell's monopole, along (u, w) in units of M s^2 r / R^5
-// (the network's output frame, nb_mlp.h), less the softened quadrupole term
-// that the apply adds exactly, over s/R.
+// internal cell with at most kPairMaxBodies bodies becomes a pair. Its inputs
+// include the cell's third moments (the tree keeps them for the learned far
+// field); its target is the exact FP64 Plummer acceleration of the cell's
+// bodies at the sample body minus the cell's monopole, along (u, w) in units
+// of M s^2 r / R^5 (the network's output frame, nb_mlp.h), less the softened
+// quadrupole term that the apply adds exactly, over s/R.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -98,7 +99,8 @@ __global__ __launch_bounds__(64) void k_mlp_collect(const NodeRec* __restrict__ 
 
 // Features and exact-minus-monopole targets, one wave per pair.
 __global__ __launch_bounds__(256) void k_mlp_targets(
-    const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
+    const NodeRec* __restrict__ nodes, const float4* __restrict__ mom3,
+    const double2* __restrict__ pos,
     const double* __restrict__ mass, const double* __restrict__ box, double eps2,
     const uint32_t* __restrict__ pair_body, const uint32_t* __restrict__ pair_slot,
     const uint32_t* __restrict__ total, uint64_t cap, float* __restrict__ X,
@@ -130,12 +132,12 @@ __global__ __launch_bounds__(256) void k_mlp_targets(
             const double dx = r->cx - x.x, dy = r->cy - x.y;
             const double r2 = dx * dx + dy * dy + eps2;
             const double w = r->m / (r2 * sqrt(r2));
-            float f[8], ux, uy, qd[2];
+            float f[kMlpIn], ux, uy, qd[2];
             const float scale = mlp_features((float)dx, (float)dy, (float)sqrt(r->wsq), r->mf,
-                                             r->ixx, r->ixy, r->iyy, eps2f, inv_mroot, inv_ell, f,
-                                             ux, uy, qd);
+                                             r->ixx, r->ixy, r->iyy, mom3[pair_slot[q]], eps2f,
+                                             inv_mroot, inv_ell, f, ux, uy, qd);
 #pragma unroll
-            for (int i = 0; i < 8; i++) X[q * 8 + i] = f[i];
+            for (int i = 0; i < kMlpIn; i++) X[q * kMlpIn + i] = f[i];
             // exact - monopole along (u, w), w = (-uy, ux), less the quadrupole
             // term, over s/R (the octupole's scale in these units)
             const double cx = ax - dx * w, cy = ay - dy * w;
@@ -152,8 +154,11 @@ __global__ __launch_bounds__(256) void k_mlp_targets(
 // samples writes its partial gradient; k_mlp_adam sums the partials.
 // ---------------------------------------------------------------------------
 struct MlpOffsets {
-    static constexpr int W1 = 0, b1 = 512, W2 = 576, b2 = 4672, W3 = 4736, b3 = 4864;
+    static constexpr int W1 = 0, b1 = W1 + kMlpHidden * kMlpIn, W2 = b1 + kMlpHidden,
+                         b2 = W2 + kMlpHidden * kMlpHidden, W3 = b2 + kMlpHidden,
+                         b3 = W3 + kMlpOut * kMlpHidden;
 };
+static_assert(MlpOffsets::b3 + kMlpOut == kMlpParams, "parameter layout");
 
 __global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ w,
                                                   const float* __restrict__ X,
@@ -182,7 +187,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ w,
             src = explicit_idx ? explicit_idx[b]
                                : hash_u32(seed * 0x9e3779b97f4a7c15ull + (uint64_t)step * 0x100000001b3ull + b) % npairs;
 #pragma unroll
-        for (int k = 0; k < kMlpIn; k++) sx[i][k] = ok ? bf16_round(X[src * 8 + k]) : 0.f;
+        for (int k = 0; k < kMlpIn; k++) sx[i][k] = ok ? bf16_round(X[src * kMlpIn + k]) : 0.f;
         st[i][0] = ok ? T[src * 2 + 0] : 0.f;
         st[i][1] = ok ? T[src * 2 + 1] : 0.f;
     }
@@ -192,7 +197,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ w,
     for (int j = j0; j < j0 + 16; j++) {
         float a = sw[O::b1 + j];
 #pragma unroll
-        for (int k = 0; k < kMlpIn; k++) a = fmaf(sw[O::W1 + j * 8 + k], sx[s][k], a);
+        for (int k = 0; k < kMlpIn; k++) a = fmaf(sw[O::W1 + j * kMlpIn + k], sx[s][k], a);
         sh1[s][j] = bf16_round(fmaxf(a, 0.f));
     }
     __syncthreads();
@@ -257,9 +262,10 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ w,
         sh2[s][k] = sh1[s][k] > 0.f ? g : 0.f;
     }
     __syncthreads();
-    // dW1 (2 outputs per thread), db1
-    for (int q = 0; q < 2; q++) {
-        const int idx = t * 2 + q, j = idx >> 3, k = idx & 7;
+    // dW1 (kMlpIn * 64 / 256 outputs per thread), db1
+    static_assert(kMlpHidden * kMlpIn % 256 == 0, "dW1 split");
+    for (int q = 0; q < kMlpHidden * kMlpIn / 256; q++) {
+        const int idx = t * (kMlpHidden * kMlpIn / 256) + q, j = idx / kMlpIn, k = idx % kMlpIn;
         float a = 0.f;
         for (int ss = 0; ss < kTrainPerBlock; ss++) a = fmaf(sh2[ss][j], sx[ss][k], a);
         gp[O::W1 + idx] = a;
@@ -310,18 +316,18 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X
     const uint64_t batches = (count + 63) / 64;
     const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint64_t waves = (uint64_t)gridDim.x * 4;
-    const int part = l >> 4;  // 0: f0..f3, 1: f4..f7, 2-3: zero padding
+    const int part = l >> 4;  // 0: f0..f3, 1: f4..f7, 2: f8..f11, 3: zero padding
     for (uint64_t bt = wave; bt < batches; bt += waves) {
         f32x4 x[4], y[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const uint64_t row = bt * 64 + 16 * q + (l & 15);
             x[q] = f32x4{0.f, 0.f, 0.f, 0.f};
-            if (row < count && part < 2) {
-                x[q].x = X[row * 8 + part * 4 + 0];
-                x[q].y = X[row * 8 + part * 4 + 1];
-                x[q].z = X[row * 8 + part * 4 + 2];
-                x[q].w = X[row * 8 + part * 4 + 3];
+            if (row < count && part < kMlpIn / 4) {
+                x[q].x = X[row * kMlpIn + part * 4 + 0];
+                x[q].y = X[row * kMlpIn + part * 4 + 1];
+                x[q].z = X[row * kMlpIn + part * 4 + 2];
+                x[q].w = X[row * kMlpIn + part * 4 + 3];
             }
         }
         mlp_forward_tiles<2>(x, w, sb, l, y);
@@ -342,12 +348,12 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X
 // each finished subgroup the accepted cells (tree slots written by the walk,
 // possibly in another group's region when another wavefront walked it)
 // go through the network 64 at a time at the subgroup's box centre (lane l
-// prepares cell l; tile q of the MFMA pass is cells 16q..16q+15). Each lane
-// also adds its cell's softened-quadrupole Jacobian, so a subgroup ends with
-// C = sum of the corrections at the centre and K = sum of the Jacobians; a
-// member at offset b from the centre adds C - K b (first order across the
-// subgroup) to the walk's acceleration and kicks (or stores G a). The sums
-// run in a fixed order.
+// prepares cell l; tile q of the MFMA pass is cells 16q..16q+15). A subgroup
+// ends with C = the sum of the network's terms (s/R times its output, in
+// acceleration units) at the centre; every member adds C to the walk's
+// acceleration, which already holds each cell's softened quadrupole term at
+// the member's own position, and kicks (or stores G a). The sums run in a
+// fixed order.
 // ---------------------------------------------------------------------------
 constexpr int kApplyWaves = 4;
 
@@ -357,16 +363,16 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     uint32_t cell_cap, const SubRec* __restrict__ subs, const float4* __restrict__ body_w,
     double2* __restrict__ acc, double2* __restrict__ vel, uint32_t n,
     const StepConsts* __restrict__ k, uint32_t* __restrict__ meta,
-    const NodeRec* __restrict__ nodes, const double* __restrict__ box,
+    const NodeRec* __restrict__ nodes, const float4* __restrict__ mom3,
+    const double* __restrict__ box,
     const void* __restrict__ tiles, const float* __restrict__ bias, int count_step,
     uint32_t groups, const uint32_t* __restrict__ grp_first,
     const uint32_t* __restrict__ grp_info) {
     if (meta[META_FLAGS] & kHaltFlags) return;
     __shared__ __attribute__((aligned(16))) float sb[kMlpBiasLen];
-    __shared__ float su[kApplyWaves][6 * 64];        // [C_x, C_y, K00, K01, K10, K11][subgroup]
-    __shared__ __attribute__((aligned(16))) float sf[kApplyWaves][64 * 8];  // cell inputs
-    __shared__ float2 sr[kApplyWaves][64];           // cell scale * u
-    __shared__ float4 sq[kApplyWaves][64];           // quadrupole term, output units; s/R
+    __shared__ float su[kApplyWaves][2 * 64];        // [C_x, C_y][subgroup]
+    __shared__ __attribute__((aligned(16))) float sf[kApplyWaves][64 * kMlpIn];  // cell inputs
+    __shared__ float2 sr[kApplyWaves][64];           // cell scale * (s/R) * u
     for (int i = threadIdx.x; i < kMlpBiasLen; i += 64 * kApplyWaves) sb[i] = bias[i];
     if (count_step && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
     __syncthreads();
@@ -390,13 +396,16 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     struct CellIn {
         double cx, cy, wsq;
         float mf, ixx, ixy, iyy;
+        float4 t3;
         bool ok;
     };
     auto load_cells = [&](uint64_t b0, uint64_t end) {
-        CellIn c{0.0, 0.0, 1.0, 1.f, 0.f, 0.f, 0.f, false};
+        CellIn c{0.0, 0.0, 1.0, 1.f, 0.f, 0.f, 0.f, make_float4(0.f, 0.f, 0.f, 0.f), false};
         const uint64_t cell = b0 + l;
         if (cell < end) {
-            const NodeRec* r = nodes + cell_slot[cell];
+            const uint32_t slot = cell_slot[cell];
+            const NodeRec* r = nodes + slot;
+            c.t3 = mom3[slot];
             c.cx = r->cx;
             c.cy = r->cy;
             c.wsq = r->wsq;
@@ -411,14 +420,16 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     for (int ks = 0; ks < nsub; ks++) {
         const SubRec sub = subs[(uint64_t)g * 64 + ks];
         const uint64_t end = min((uint64_t)sub.start + sub.count, slots);
-        float cx = 0.f, cy = 0.f, k00 = 0.f, k01 = 0.f, k10 = 0.f, k11 = 0.f;
+        float cx = 0.f, cy = 0.f;
         CellIn nx = load_cells(sub.start, end);
         for (uint64_t b0 = sub.start; b0 < end; b0 += 64) {
             const CellIn cur = nx;
             if (b0 + 64 < end) nx = load_cells(b0 + 64, end);
             const bool ok = cur.ok;
             float dx = 1.f, dy = 0.f, s = 1.f, M = 1.f, ixx = 0.f, ixy = 0.f, iyy = 0.f;
+            float4 t3 = make_float4(0.f, 0.f, 0.f, 0.f);
             if (ok) {
+                t3 = cur.t3;
                 dx = (float)(cur.cx - sub.ox);
                 dy = (float)(cur.cy - sub.oy);
                 s = sqrtf((float)cur.wsq);
@@ -427,30 +438,23 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
                 ixy = cur.ixy;
                 iyy = cur.iyy;
             }
-            float f[8], ux, uy, J[4], qd[2];
-            float scale = mlp_features(dx, dy, s, M, ixx, ixy, iyy, eps2f, inv_mroot, inv_ell, f,
-                                       ux, uy, qd);
-            quad_jacobian(dx, dy, ixx, ixy, iyy, eps2f, J);
-            if (ok) {
-                k00 += J[0];
-                k01 += J[1];
-                k10 += J[2];
-                k11 += J[3];
-            } else {
-                scale = 0.f;
-            }
-            *reinterpret_cast<float4*>(f_all + 8 * l) = make_float4(f[0], f[1], f[2], f[3]);
-            *reinterpret_cast<float4*>(f_all + 8 * l + 4) = make_float4(f[4], f[5], f[6], f[7]);
+            float f[kMlpIn], ux, uy, qd[2];
+            float scale = mlp_features(dx, dy, s, M, ixx, ixy, iyy, t3, eps2f, inv_mroot,
+                                       inv_ell, f, ux, uy, qd);
+            scale = ok ? scale * f[2] : 0.f;
+#pragma unroll
+            for (int h = 0; h < kMlpIn / 4; h++)
+                *reinterpret_cast<float4*>(f_all + kMlpIn * l + 4 * h) =
+                    make_float4(f[4 * h], f[4 * h + 1], f[4 * h + 2], f[4 * h + 3]);
             sr[wv][l] = make_float2(scale * ux, scale * uy);
-            sq[wv][l] = make_float4(qd[0], qd[1], f[2], 0.f);
             __builtin_amdgcn_wave_barrier();
             f32x4 x[4], y[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 x[q] = f32x4{0.f, 0.f, 0.f, 0.f};
-                if (l < 32) {
+                if (l < 16 * (kMlpIn / 4)) {
                     const float4 v = *reinterpret_cast<const float4*>(
-                        f_all + 8 * (16 * q + col) + 4 * (l >> 4));
+                        f_all + kMlpIn * (16 * q + col) + 4 * (l >> 4));
                     x[q] = f32x4{v.x, v.y, v.z, v.w};
                 }
             }
@@ -460,10 +464,9 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const float2 su_ = sr[wv][(16 * q + l) & 63];
-                const float4 qq = sq[wv][(16 * q + l) & 63];
                 if (l < 16) {
-                    // correction = quadrupole term + (s/R) * network output
-                    const float yu = fmaf(y[q].x, qq.z, qq.x), yw = fmaf(y[q].y, qq.z, qq.y);
+                    // the network's term: (s/R) * output along (u, w)
+                    const float yu = y[q].x, yw = y[q].y;
                     cx += yu * su_.x - yw * su_.y;
                     cy += yu * su_.y + yw * su_.x;
                 }
@@ -474,18 +477,10 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
         for (int d = 1; d < 64; d <<= 1) {
             cx += __shfl_xor(cx, d, 64);
             cy += __shfl_xor(cy, d, 64);
-            k00 += __shfl_xor(k00, d, 64);
-            k01 += __shfl_xor(k01, d, 64);
-            k10 += __shfl_xor(k10, d, 64);
-            k11 += __shfl_xor(k11, d, 64);
         }
         if (l == 0) {
             u[0 * 64 + ks] = cx;
             u[1 * 64 + ks] = cy;
-            u[2 * 64 + ks] = k00;
-            u[3 * 64 + ks] = k01;
-            u[4 * 64 + ks] = k10;
-            u[5 * 64 + ks] = k11;
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -493,9 +488,7 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     const uint32_t i = first + l;
     const float4 bw = body_w[i];
     const int ks = __float_as_int(bw.z) & 63;
-    // correction at offset b: C - K b
-    const float ax = u[0 * 64 + ks] - (u[2 * 64 + ks] * bw.x + u[3 * 64 + ks] * bw.y);
-    const float ay = u[1 * 64 + ks] - (u[4 * 64 + ks] * bw.x + u[5 * 64 + ks] * bw.y);
+    const float ax = u[0 * 64 + ks], ay = u[1 * 64 + ks];
     const double2 a0 = acc[i];
     const double2 a = make_double2(a0.x + (double)ax, a0.y + (double)ay);
     if (KICK) {
@@ -518,8 +511,8 @@ void launch_mlp_apply(nb_ctx* c, int src, bool kick, bool count_step) {
     hipLaunchKernelGGL((k_mlp_apply<K>), dim3(blocks), dim3(64 * kApplyWaves), 0, c->stream,   \
                        c->b.cell_slot, c->b.cell_cnt, c->cell_cap, c->b.subs, c->b.body_w,     \
                        c->b.acc, c->b.vel[src], n, c->b.consts, c->b.meta, c->b.nodes,         \
-                       c->b.box, c->b.mlp_tiles, c->b.mlp_bias, count, groups, c->b.grp_first, \
-                       c->b.grp_info)
+                       c->b.mom3, c->b.box, c->b.mlp_tiles, c->b.mlp_bias, count, groups,      \
+                       c->b.grp_first, c->b.grp_info)
     if (kick) NB_APPLY(true); else NB_APPLY(false);
 #undef NB_APPLY
 }
@@ -563,7 +556,7 @@ void pack_tiles(const float* p, std::vector<uint8_t>& tiles, std::vector<float>&
         for (int l = 0; l < 64; l++)
             for (int v = 0; v < 4; v++) {
                 const int m = l % 16, kk = 4 * (l / 16) + v;
-                const float x = kk < 8 ? p[O::W1 + (16 * t + m) * 8 + kk] : 0.f;
+                const float x = kk < kMlpIn ? p[O::W1 + (16 * t + m) * kMlpIn + kk] : 0.f;
                 t1[(t * 64 + l) * 4 + v] = f32_to_bf16(x);
             }
     for (int t = 0; t < kMlpTiles23; t++)
@@ -623,7 +616,7 @@ int ensure_pairs(nb_ctx* c, uint64_t cap) {
     c->pair_cap = 0;
     MLP_HIP(c, hipMalloc((void**)&b.pair_body, cap * sizeof(uint32_t)));
     MLP_HIP(c, hipMalloc((void**)&b.pair_slot, cap * sizeof(uint32_t)));
-    MLP_HIP(c, hipMalloc((void**)&b.pair_x, cap * 8 * sizeof(float)));
+    MLP_HIP(c, hipMalloc((void**)&b.pair_x, cap * kMlpIn * sizeof(float)));
     MLP_HIP(c, hipMalloc((void**)&b.pair_t, cap * 2 * sizeof(float)));
     c->pair_cap = cap;
     return NB_OK;
@@ -692,7 +685,7 @@ extern "C" {
 int nb_mlp_set_weights(nb_ctx* c, const float* params, uint64_t count) {
     if (!c || !params) return NB_ERR_INVALID_ARG;
     if (count != (uint64_t)kMlpParams)
-        return mlp_fail(c, NB_ERR_INVALID_ARG, "expected 4866 parameters");
+        return mlp_fail(c, NB_ERR_INVALID_ARG, "expected NB_MLP_PARAMS parameters");
     int rc = ensure_mlp(c);
     if (rc) return rc;
     return upload_weights(c, params);
@@ -701,7 +694,7 @@ int nb_mlp_set_weights(nb_ctx* c, const float* params, uint64_t count) {
 int nb_mlp_get_weights(nb_ctx* c, float* params, uint64_t count) {
     if (!c || !params) return NB_ERR_INVALID_ARG;
     if (count != (uint64_t)kMlpParams)
-        return mlp_fail(c, NB_ERR_INVALID_ARG, "expected 4866 parameters");
+        return mlp_fail(c, NB_ERR_INVALID_ARG, "expected NB_MLP_PARAMS parameters");
     int rc = ensure_mlp(c);
     if (rc) return rc;
     MLP_HIP(c, scopy(c, params, c->b.mlp_w, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
@@ -714,7 +707,7 @@ int nb_mlp_init(nb_ctx* c, uint64_t seed) {
     std::mt19937_64 rng(seed);
     std::normal_distribution<float> nd(0.f, 1.f);
     using O = MlpOffsets;
-    for (int i = 0; i < 64 * 8; i++) p[O::W1 + i] = nd(rng) * std::sqrt(2.f / 8.f);
+    for (int i = 0; i < 64 * kMlpIn; i++) p[O::W1 + i] = nd(rng) * std::sqrt(2.f / kMlpIn);
     for (int i = 0; i < 64 * 64; i++) p[O::W2 + i] = nd(rng) * std::sqrt(2.f / 64.f);
     for (int i = 0; i < 2 * 64; i++) p[O::W3 + i] = nd(rng) * std::sqrt(1.f / 64.f);
     int rc = ensure_mlp(c);
@@ -737,7 +730,9 @@ int nb_mlp_generate(nb_ctx* c, double theta, uint32_t sample_bodies, uint64_t ma
         return mlp_fail(c, NB_ERR_INVALID_ARG, "max_pairs must be in [1, 2^32)");
     // The scan of per-sample counts uses the context's scan scratch, sized for n.
     const uint32_t samples = (uint32_t)std::min<uint64_t>(sample_bodies, c->n);
-    int rc = build_tree(c);
+    int rc = ensure_mom3(c);  // the pairs' inputs include the cells' third moments
+    if (rc) return rc;
+    rc = build_tree(c);
     if (rc) return rc;
     rc = ensure_pairs(c, max_pairs);
     if (rc) return rc;
@@ -758,7 +753,7 @@ int nb_mlp_generate(nb_ctx* c, double theta, uint32_t sample_bodies, uint64_t ma
     hipLaunchKernelGGL(k_mlp_collect<true>, dim3(waves), dim3(64), 0, c->stream, b.nodes,
                        b.pos[c->cur], (uint32_t)c->n, samples, seed, theta * theta,
                        b.sample_off, cap, b.pair_body, b.pair_slot);
-    hipLaunchKernelGGL(k_mlp_targets, dim3(2048), dim3(256), 0, c->stream, b.nodes,
+    hipLaunchKernelGGL(k_mlp_targets, dim3(2048), dim3(256), 0, c->stream, b.nodes, b.mom3,
                        b.pos[c->cur], b.mass[c->cur], b.box,
                        c->p.softening * c->p.softening, b.pair_body, b.pair_slot, total,
                        (uint64_t)cap, b.pair_x, b.pair_t);
@@ -781,7 +776,8 @@ int nb_mlp_get_pairs(nb_ctx* c, float* features, float* targets, uint32_t* body,
     if (cell)
         MLP_HIP(c, scopy(c, cell, c->b.pair_slot, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (features)
-        MLP_HIP(c, scopy(c, features, c->b.pair_x, k * 8 * sizeof(float), hipMemcpyDeviceToHost));
+        MLP_HIP(c, scopy(c, features, c->b.pair_x, k * kMlpIn * sizeof(float),
+                         hipMemcpyDeviceToHost));
     if (targets)
         MLP_HIP(c, scopy(c, targets, c->b.pair_t, k * 2 * sizeof(float), hipMemcpyDeviceToHost));
     return NB_OK;
@@ -792,7 +788,7 @@ int nb_mlp_set_pairs(nb_ctx* c, const float* features, const float* targets, uin
     int rc = ensure_pairs(c, count ? count : 1);
     if (rc) return rc;
     if (count) {
-        MLP_HIP(c, scopy(c, c->b.pair_x, features, count * 8 * sizeof(float),
+        MLP_HIP(c, scopy(c, c->b.pair_x, features, count * kMlpIn * sizeof(float),
                              hipMemcpyHostToDevice));
         MLP_HIP(c, scopy(c, c->b.pair_t, targets, count * 2 * sizeof(float),
                              hipMemcpyHostToDevice));
@@ -923,10 +919,10 @@ int nb_mlp_forward(nb_ctx* c, const float* features, uint64_t count, float* out)
     int rc = ensure_mlp(c);
     if (rc || count == 0) return rc;
     float *dx = nullptr, *dy = nullptr;
-    MLP_HIP(c, hipMalloc((void**)&dx, count * 8 * sizeof(float)));
+    MLP_HIP(c, hipMalloc((void**)&dx, count * kMlpIn * sizeof(float)));
     hipError_t e = hipMalloc((void**)&dy, count * 2 * sizeof(float));
     if (e == hipSuccess)
-        e = scopy(c, dx, features, count * 8 * sizeof(float), hipMemcpyHostToDevice);
+        e = scopy(c, dx, features, count * kMlpIn * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         const uint64_t batches = (count + 63) / 64;
         const uint64_t blocks = std::min<uint64_t>((batches + 3) / 4, 2048);

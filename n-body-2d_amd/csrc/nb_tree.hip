@@ -60,7 +60,7 @@ __device__ void build_node(uint32_t p, const uint64_t* __restrict__ keys, uint32
                            NodeRec* __restrict__ nodes, uint32_t* __restrict__ inode_slot,
                            const uint8_t* __restrict__ inode_level,
                            uint8_t* __restrict__ inode_mask, uint32_t* __restrict__ multi_list,
-                           uint32_t* __restrict__ meta) {
+                           uint32_t* __restrict__ meta, float4* __restrict__ mom3) {
     const uint32_t a = node_a[p];
     const int L = inode_level[p];
     const uint32_t sys = sys_of[a];
@@ -130,6 +130,7 @@ __device__ void build_node(uint32_t p, const uint64_t* __restrict__ keys, uint32
                 r->ixx = 0.f;
                 r->ixy = 0.f;
                 r->iyy = 0.f;
+                if (mom3) mom3[nsys + 4ull * p + q] = make_float4(0.f, 0.f, 0.f, 0.f);
             } else {
                 const uint32_t k = atomicAdd(&meta[META_MULTI_LEAVES], 1u);
                 multi_list[k] = (uint32_t)(nsys + 4ull * p + q);
@@ -167,7 +168,8 @@ __global__ __launch_bounds__(256) void k_tree_enum(const int8_t* __restrict__ de
                                                    const double2* __restrict__ pos,
                                                    const double* __restrict__ mass,
                                                    const double* __restrict__ box,
-                                                   NodeRec* __restrict__ nodes) {
+                                                   NodeRec* __restrict__ nodes,
+                                                   float4* __restrict__ mom3) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const uint32_t I = meta[META_INTERNAL];
     // A step after one that halted (any halt flag) only carries the state
@@ -193,6 +195,7 @@ __global__ __launch_bounds__(256) void k_tree_enum(const int8_t* __restrict__ de
         r->bs = (int32_t)i;
         r->be = (int32_t)i;
         r->ixx = r->ixy = r->iyy = 0.f;
+        if (mom3) mom3[sys] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
     if (i + 1 >= n) return;
@@ -214,13 +217,14 @@ __global__ __launch_bounds__(256) void k_tree_nodes(
     const uint32_t* __restrict__ off, const double* __restrict__ box, uint32_t* __restrict__ meta,
     uint64_t icap, const uint32_t* __restrict__ node_a, NodeRec* __restrict__ nodes,
     uint32_t* __restrict__ inode_slot, const uint8_t* __restrict__ inode_level,
-    uint8_t* __restrict__ inode_mask, uint32_t* __restrict__ multi_list) {
+    uint8_t* __restrict__ inode_mask, uint32_t* __restrict__ multi_list,
+    float4* __restrict__ mom3) {
     const uint32_t I = meta[META_INTERNAL];
     if (I > icap || (meta[META_FLAGS] & kHaltFlags)) return;
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     if (p >= I) return;
     build_node(p, keys, nsys, sys_of, sys_off, pos, mass, delta, off, box, node_a, nodes,
-               inode_slot, inode_level, inode_mask, multi_list, meta);
+               inode_slot, inode_level, inode_mask, multi_list, meta, mom3);
 }
 
 // Multi-body leaf COM, one wave per leaf, in the order of compute_leaf_com
@@ -230,7 +234,8 @@ __global__ __launch_bounds__(256) void k_leaf_multi(const double2* __restrict__ 
                                                     const double* __restrict__ mass,
                                                     const uint32_t* __restrict__ multi_list,
                                                     const uint32_t* __restrict__ meta,
-                                                    NodeRec* __restrict__ nodes) {
+                                                    NodeRec* __restrict__ nodes,
+                                                    float4* __restrict__ mom3) {
     const uint32_t count = meta[META_MULTI_LEAVES];
     const uint32_t waves = gridDim.x * 4;
     const int l = lane_id();
@@ -261,20 +266,30 @@ __global__ __launch_bounds__(256) void k_leaf_multi(const double2* __restrict__ 
         }
         m = __shfl(m, 0, 64);
         const double cx = __shfl(x, 0, 64) / m, cy = __shfl(y, 0, 64) / m;
-        double a = 0.0, b = 0.0, c = 0.0;
+        double a = 0.0, b = 0.0, c = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
         for (int32_t i = bs + l; i <= be; i += 64) {
             const double2 p = pos[i];
             const double w = mass[i], dx = p.x - cx, dy = p.y - cy;
             a += w * dx * dx;
             b += w * dx * dy;
             c += w * dy * dy;
+            t0 += w * dx * dx * dx;
+            t1 += w * dx * dx * dy;
+            t2 += w * dx * dy * dy;
+            t3 += w * dy * dy * dy;
         }
 #pragma unroll
         for (int st = 32; st >= 1; st >>= 1) {
             a += __shfl_xor(a, st, 64);
             b += __shfl_xor(b, st, 64);
             c += __shfl_xor(c, st, 64);
+            t0 += __shfl_xor(t0, st, 64);
+            t1 += __shfl_xor(t1, st, 64);
+            t2 += __shfl_xor(t2, st, 64);
+            t3 += __shfl_xor(t3, st, 64);
         }
+        if (l == 0 && mom3)
+            mom3[multi_list[k]] = make_float4((float)t0, (float)t1, (float)t2, (float)t3);
         if (l == 0) {
             r->cx = cx;
             r->cy = cy;
@@ -352,6 +367,24 @@ __device__ __forceinline__ Com combine_ref(const Com ch[4]) {
     return o;
 }
 
+// Third moments about the parent's COM from the children's (parallel axes):
+// with d = child COM - parent COM, m, I, T the child's mass, second and
+// third moments, T_abc += T_abc + d_a I_bc + d_b I_ac + d_c I_ab + m d_a d_b d_c.
+__device__ __forceinline__ float4 combine3(const Com ch[4], const float4 t[4], const Com& o) {
+    double txxx = 0.0, txxy = 0.0, txyy = 0.0, tyyy = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double m = ch[q].m, dx = ch[q].cx - o.cx, dy = ch[q].cy - o.cy;
+        if (m == 0.0) continue;  // an empty slot
+        const double ixx = ch[q].ixx, ixy = ch[q].ixy, iyy = ch[q].iyy;
+        txxx += (double)t[q].x + 3.0 * dx * ixx + m * dx * dx * dx;
+        txxy += (double)t[q].y + 2.0 * dx * ixy + dy * ixx + m * dx * dx * dy;
+        txyy += (double)t[q].z + dx * iyy + 2.0 * dy * ixy + m * dx * dy * dy;
+        tyyy += (double)t[q].w + 3.0 * dy * iyy + m * dy * dy * dy;
+    }
+    return make_float4((float)txxx, (float)txxy, (float)txyy, (float)tyyy);
+}
+
 __device__ __forceinline__ void store_com(NodeRec* __restrict__ r, const Com& o, uint32_t mask) {
     r->cx = o.cx;
     r->cy = o.cy;
@@ -415,7 +448,8 @@ __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __rest
                                                    uint64_t icap, uint32_t nblk, uint32_t nsys,
                                                    const uint32_t* __restrict__ lvl_off,
                                                    const uint4* __restrict__ by_level,
-                                                   NodeRec* __restrict__ nodes) {
+                                                   NodeRec* __restrict__ nodes,
+                                                   float4* __restrict__ mom3) {
     // the segment bounds are loaded alongside the halt check (in bounds either way)
     const uint32_t b = lvl_off[(uint64_t)L * nblk];
     const uint32_t e = L + 1 < kMaxLevel ? lvl_off[(uint64_t)(L + 1) * nblk] : meta[META_INTERNAL];
@@ -428,7 +462,15 @@ __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __rest
 #pragma unroll
         for (int q = 0; q < 4; q++)
             ch[q] = (mask >> q) & 1u ? load_com(&nodes[nsys + 4ull * p + q]) : Com{0.0, 0.0, 0.0, 0.f, 0.f, 0.f};
-        store_com(&nodes[en.y], REF ? combine_ref(ch) : combine(ch), mask);
+        const Com o = REF ? combine_ref(ch) : combine(ch);
+        store_com(&nodes[en.y], o, mask);
+        if (mom3) {
+            float4 t[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                t[q] = (mask >> q) & 1u ? mom3[nsys + 4ull * p + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+            mom3[en.y] = combine3(ch, t, o);
+        }
     }
 }
 
@@ -444,15 +486,16 @@ void launch_tree_build(nb_ctx* c, int src) {
     launch_scan_u32(c, c->b.cnt, pairs, &c->b.meta[META_INTERNAL]);
     hipLaunchKernelGGL(k_tree_enum, dim3(bg), dim3(256), 0, c->stream, c->b.delta, c->b.cnt, n,
                        c->b.sys_of, c->b.sys_off, c->b.meta, (uint64_t)c->icap, c->b.node_a,
-                       c->b.inode_level, c->b.pos[src], c->b.mass[src], c->b.box, c->b.nodes);
+                       c->b.inode_level, c->b.pos[src], c->b.mass[src], c->b.box, c->b.nodes,
+                       c->b.mom3);
     const uint32_t ng = (uint32_t)((c->icap + 255) / 256);
     hipLaunchKernelGGL(k_tree_nodes, dim3(ng), dim3(256), 0, c->stream, c->b.keys[0], c->nsys,
                        c->b.sys_of, c->b.sys_off, c->b.pos[src], c->b.mass[src], c->b.delta,
                        c->b.cnt, c->b.box, c->b.meta, (uint64_t)c->icap, c->b.node_a,
                        c->b.nodes, c->b.inode_slot, c->b.inode_level, c->b.inode_mask,
-                       c->b.multi_list);
+                       c->b.multi_list, c->b.mom3);
     hipLaunchKernelGGL(k_leaf_multi, dim3(64), dim3(256), 0, c->stream, c->b.pos[src],
-                       c->b.mass[src], c->b.multi_list, c->b.meta, c->b.nodes);
+                       c->b.mass[src], c->b.multi_list, c->b.meta, c->b.nodes, c->b.mom3);
     hipLaunchKernelGGL(k_level_hist, dim3(ng), dim3(256), 0, c->stream, c->b.meta,
                        (uint64_t)c->icap, c->b.inode_level, c->b.lvl_cnt);
     launch_scan_u32(c, c->b.lvl_cnt, (uint32_t)(kMaxLevel * ng), nullptr);
@@ -467,11 +510,11 @@ void launch_tree_build(nb_ctx* c, int src) {
         if (ref)
             hipLaunchKernelGGL(k_com_level<true>, dim3(cg), dim3(256), 0, c->stream, L,
                                c->b.meta, (uint64_t)c->icap, ng, c->nsys, c->b.lvl_cnt,
-                               c->b.by_level, c->b.nodes);
+                               c->b.by_level, c->b.nodes, c->b.mom3);
         else
             hipLaunchKernelGGL(k_com_level<false>, dim3(cg), dim3(256), 0, c->stream, L,
                                c->b.meta, (uint64_t)c->icap, ng, c->nsys, c->b.lvl_cnt,
-                               c->b.by_level, c->b.nodes);
+                               c->b.by_level, c->b.nodes, c->b.mom3);
     }
 }
 

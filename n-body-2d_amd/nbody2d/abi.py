@@ -49,8 +49,8 @@ EXPORTS = (
     "nb_direct_sum_f64_noalloc",
 )
 NOALLOC_SCRATCH_BYTES = 16 << 20
-MLP_PARAMS = 4866
-MLP_INPUTS = 8
+MLP_PARAMS = 5122
+MLP_INPUTS = 12
 
 
 class Params(C.Structure):
@@ -84,6 +84,7 @@ NODE_DTYPE = np.dtype([
     ("level", "<i4"), ("bstart", "<i4"), ("bend", "<i4"), ("c0", "<i4"),
     ("mass", "<f8"), ("cx", "<f8"), ("cy", "<f8"), ("wsq", "<f8"),
     ("ixx", "<f4"), ("ixy", "<f4"), ("iyy", "<f4"), ("child_mask", "<i4"),
+    ("t3", "<f4", (4,)),
 ])
 
 _lib = None

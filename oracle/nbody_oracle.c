@@ -27,7 +27,6 @@
  *   Part D (the reference has no counterpart; pinned by mathematics):
  *   or_quadrupole             order of the residual exact - monopole -
  *                             quadrupole against brute-force sums
- *   or_quadrupole_jacobian    central differences of or_quadrupole
  *   or_cell_correction        brute-force pair sums; zero for one body
  *   or_mlp_features           hand-computed values
  *   or_mlp_quad_units         or_quadrupole taken along (u, w) over the
@@ -952,10 +951,12 @@ typedef struct {
     const double *W1, *b1, *W2, *b2, *W3, *b3;
 } mlp_view;
 
+#define OR_MLP_IN 12
+
 static mlp_view mlp_split(const double* p) {
     mlp_view v;
     v.W1 = p;
-    v.b1 = v.W1 + 64 * 8;
+    v.b1 = v.W1 + 64 * OR_MLP_IN;
     v.W2 = v.b1 + 64;
     v.b2 = v.W2 + 64 * 64;
     v.W3 = v.b2 + 64;
@@ -966,11 +967,12 @@ static mlp_view mlp_split(const double* p) {
 /* one sample: returns pre-activations and activations of both hidden layers */
 static void mlp_fwd1(const mlp_view* v, const float* x, int bf16, double* z1, double* h1,
                      double* z2, double* h2, double* y) {
-    double xin[8];
-    for (int k = 0; k < 8; k++) xin[k] = bf16 ? bf16_round(x[k]) : x[k];
+    double xin[OR_MLP_IN];
+    for (int k = 0; k < OR_MLP_IN; k++) xin[k] = bf16 ? bf16_round(x[k]) : x[k];
     for (int j = 0; j < 64; j++) {
         double s = v->b1[j];
-        for (int k = 0; k < 8; k++) s += (bf16 ? bf16_round(v->W1[j * 8 + k]) : v->W1[j * 8 + k]) * xin[k];
+        for (int k = 0; k < OR_MLP_IN; k++)
+            s += (bf16 ? bf16_round(v->W1[j * OR_MLP_IN + k]) : v->W1[j * OR_MLP_IN + k]) * xin[k];
         z1[j] = s;
         h1[j] = s > 0 ? s : 0.0;
         if (bf16) h1[j] = bf16_round(h1[j]);
@@ -993,7 +995,7 @@ double or_mlp_forward(const double* params, uint64_t count, const float* x, floa
     mlp_view v = mlp_split(params);
     double z1[64], h1[64], z2[64], h2[64], out[2], sum = 0.0;
     for (uint64_t s = 0; s < count; s++) {
-        mlp_fwd1(&v, x + 8 * s, bf16, z1, h1, z2, h2, out);
+        mlp_fwd1(&v, x + OR_MLP_IN * s, bf16, z1, h1, z2, h2, out);
         y[2 * s] = (float)out[0];
         y[2 * s + 1] = (float)out[1];
         sum += out[0] + out[1];
@@ -1009,14 +1011,14 @@ double or_mlp_grad(const double* params, uint64_t count, const float* x, const f
     mlp_view v = mlp_split(params);
     memset(grad, 0, sizeof(double) * OR_MLP_PARAMS);
     double* gW1 = grad;
-    double* gb1 = gW1 + 64 * 8;
+    double* gb1 = gW1 + 64 * OR_MLP_IN;
     double* gW2 = gb1 + 64;
     double* gb2 = gW2 + 64 * 64;
     double* gW3 = gb2 + 64;
     double* gb3 = gW3 + 2 * 64;
     double z1[64], h1[64], z2[64], h2[64], y[2], loss = 0.0;
     for (uint64_t s = 0; s < count; s++) {
-        const float* xs = x + 8 * s;
+        const float* xs = x + OR_MLP_IN * s;
         mlp_fwd1(&v, xs, bf16, z1, h1, z2, h2, y);
         double dy[2];
         for (int o = 0; o < 2; o++) {
@@ -1045,7 +1047,8 @@ double or_mlp_grad(const double* params, uint64_t count, const float* x, const f
         }
         for (int j = 0; j < 64; j++) {
             gb1[j] += dh1[j];
-            for (int k = 0; k < 8; k++) gW1[j * 8 + k] += dh1[j] * (bf16 ? bf16_round(xs[k]) : xs[k]);
+            for (int k = 0; k < OR_MLP_IN; k++)
+                gW1[j * OR_MLP_IN + k] += dh1[j] * (bf16 ? bf16_round(xs[k]) : xs[k]);
         }
     }
     return loss / (double)count;
@@ -1067,36 +1070,26 @@ void or_quadrupole(double dx, double dy, double ixx, double ixy, double iyy, dou
     *ay = k * (-tr * dy - 2.0 * idy + 5.0 * dId * dy / R2);
 }
 
-/* Jacobian J[i][j] = dQ_i/dd_j of or_quadrupole, written out from
- * Q_i = (3/2) R^-5 P_i, P = -tr(I) d - 2 I d + 5 S d R^-2, S = d.Id:
- *   J_ij = (3/2) R^-5 [ -5 R^-2 d_j P_i - tr(I) d_ij - 2 I_ij
- *                       + 10 R^-2 (Id)_j d_i + 5 S R^-2 d_ij - 10 S R^-4 d_i d_j ]
- * (pinned by central differences of or_quadrupole in the tests). */
-void or_quadrupole_jacobian(double dx, double dy, double ixx, double ixy, double iyy,
-                            double eps2, double* J) {
-    const double R2 = dx * dx + dy * dy + eps2, iR2 = 1.0 / R2, iR5 = iR2 * iR2 / sqrt(R2);
-    const double t = ixx + iyy;
-    const double d[2] = {dx, dy};
-    const double Id[2] = {ixx * dx + ixy * dy, ixy * dx + iyy * dy};
-    const double I[2][2] = {{ixx, ixy}, {ixy, iyy}};
-    const double S = dx * Id[0] + dy * Id[1];
-    double P[2];
-    for (int i = 0; i < 2; i++) P[i] = -t * d[i] - 2.0 * Id[i] + 5.0 * S * d[i] * iR2;
+/* T(a, b, c) = sum_ijk a_i b_j c_k T_ijk for the symmetric 2-D 3-tensor
+ * t3 = {T_xxx, T_xxy, T_xyy, T_yyy}. */
+static double t3_contract(const double* t3, const double* a, const double* b, const double* c) {
+    const double T[2][2][2] = {{{t3[0], t3[1]}, {t3[1], t3[2]}}, {{t3[1], t3[2]}, {t3[2], t3[3]}}};
+    double sum = 0.0;
     for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) {
-            const double dij = i == j ? 1.0 : 0.0;
-            J[2 * i + j] = 1.5 * iR5 * (-5.0 * iR2 * d[j] * P[i] - t * dij - 2.0 * I[i][j] +
-                                        10.0 * iR2 * Id[j] * d[i] + 5.0 * S * iR2 * dij -
-                                        10.0 * S * iR2 * iR2 * d[i] * d[j]);
-        }
+        for (int j = 0; j < 2; j++)
+            for (int k = 0; k < 2; k++) sum += a[i] * b[j] * c[k] * T[i][j][k];
+    return sum;
 }
 
 /* The network inputs of a (cell, point) pair, FP64; returns the output scale
  * M s^2 r / R^5. d = com - point, u = d/r, w = (-u_y, u_x); the second
- * moments enter in the frame (u, w); ell = eps (or the root side when
- * eps = 0). Outputs and targets are components along (u, w). */
+ * moments (f4..f6, over M s^2) and the third moments t3 (f8..f11: T_uuu,
+ * T_uuw, T_uww, T_www over M s^3) enter in the frame (u, w); ell = eps (or
+ * the root side when eps = 0). Outputs and targets are components along
+ * (u, w). */
 double or_mlp_features(double dx, double dy, double s, double M, double ixx, double ixy,
-                       double iyy, double eps2, double m_root, double ell, double* f) {
+                       double iyy, const double* t3, double eps2, double m_root, double ell,
+                       double* f) {
     const double r2 = dx * dx + dy * dy, r = sqrt(r2), R = sqrt(r2 + eps2);
     const double ux = dx / r, uy = dy / r, wx = -uy, wy = ux;
     const double ms2 = M * s * s;
@@ -1108,6 +1101,11 @@ double or_mlp_features(double dx, double dy, double s, double M, double ixx, dou
     f[5] = (wx * (ixx * ux + ixy * uy) + wy * (ixy * ux + iyy * uy)) / ms2;
     f[6] = (wx * (ixx * wx + ixy * wy) + wy * (ixy * wx + iyy * wy)) / ms2;
     f[7] = log2(r / ell) / 8.0;
+    const double u[2] = {ux, uy}, w[2] = {wx, wy}, ms3 = ms2 * s;
+    f[8] = t3_contract(t3, u, u, u) / ms3;
+    f[9] = t3_contract(t3, u, u, w) / ms3;
+    f[10] = t3_contract(t3, u, w, w) / ms3;
+    f[11] = t3_contract(t3, w, w, w) / ms3;
     return ms2 * r / (R * R * R * R * R);
 }
 

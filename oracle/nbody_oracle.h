@@ -124,12 +124,12 @@ void or_direct_sum(uint64_t n, const double* pos, const double* mass, double eps
 #ifdef __cplusplus
 extern "C" {
 #endif
-/* ---------------- Part D: far-field MLP (8 -> 64 -> 64 -> 2, ReLU) ----------------
- * Parameters in the canonical order W1[64][8], b1[64], W2[64][64], b2[64],
- * W3[2][64], b3[2] (4,866 values). Loss = mean over the batch of the squared
+/* ---------------- Part D: far-field MLP (12 -> 64 -> 64 -> 2, ReLU) ----------------
+ * Parameters in the canonical order W1[64][12], b1[64], W2[64][64], b2[64],
+ * W3[2][64], b3[2] (5,122 values). Loss = mean over the batch of the squared
  * error summed over the 2 outputs. bf16 = 1 rounds weights and hidden
  * activations to bfloat16 (round to nearest even) as the GPU does. */
-#define OR_MLP_PARAMS 4866
+#define OR_MLP_PARAMS 5122
 double or_mlp_forward(const double* params, uint64_t count, const float* x, float* y, int bf16);
 double or_mlp_grad(const double* params, uint64_t count, const float* x, const float* t,
                    double* grad, int bf16);
@@ -138,12 +138,11 @@ double or_mlp_grad(const double* params, uint64_t count, const float* x, const f
  * R^2 = |d|^2 + eps2. */
 void or_quadrupole(double dx, double dy, double ixx, double ixy, double iyy, double eps2,
                    double* ax, double* ay);
-/* Jacobian J = {dQx/ddx, dQx/ddy, dQy/ddx, dQy/ddy} of or_quadrupole. */
-void or_quadrupole_jacobian(double dx, double dy, double ixx, double ixy, double iyy,
-                            double eps2, double* J);
-/* Network inputs f[8] of a (cell, point) pair; returns the output scale. */
+/* Network inputs f[12] of a (cell, point) pair (t3 = third moments {T_xxx,
+ * T_xxy, T_xyy, T_yyy} about the COM); returns the output scale. */
 double or_mlp_features(double dx, double dy, double s, double M, double ixx, double ixy,
-                       double iyy, double eps2, double m_root, double ell, double* f);
+                       double iyy, const double* t3, double eps2, double m_root, double ell,
+                       double* f);
 /* The softened quadrupole term along (u, w) in output units, from the inputs
  * f[8] and rho = r^2 / R^2. */
 void or_mlp_quad_units(const double* f, double rho, double* q);

@@ -58,10 +58,9 @@ def load(fast: bool = False) -> C.CDLL:
     L.or_mlp_grad.argtypes = [_P, u64, _P, _P, _P, i]
     L.or_mlp_grad.restype = d
     L.or_quadrupole.argtypes = [d, d, d, d, d, d, C.POINTER(d), C.POINTER(d)]
-    L.or_mlp_features.argtypes = [d, d, d, d, d, d, d, d, d, d, _P]
+    L.or_mlp_features.argtypes = [d, d, d, d, d, d, d, _P, d, d, d, _P]
     L.or_mlp_features.restype = d
     L.or_cell_correction.argtypes = [_P, _P, i64, i64, d, d, d, _P]
-    L.or_quadrupole_jacobian.argtypes = [d, d, d, d, d, d, _P]
     L.or_mlp_quad_units.argtypes = [_P, d, _P]
     _libs[name] = L
     return L
@@ -235,12 +234,15 @@ def refcpu_allpairs_run(pos, vel, mass, G, dt, eps2, steps, kernel=FORCE_REF_CPU
 
 
 # ---- Part D: far-field network ----
-MLP_PARAMS = 4866
+MLP_PARAMS = 5122
+MLP_IN = 12
+# canonical parameter offsets: W1[64][12], b1[64], W2[64][64], b2[64], W3[2][64], b3[2]
+MLP_OFF = {"W1": 0, "b1": 768, "W2": 832, "b2": 4928, "W3": 4992, "b3": 5120}
 
 
 def mlp_forward(params, x, bf16=True):
     w = c64(params)
-    xs = c64(x, np.float32).reshape(-1, 8)
+    xs = c64(x, np.float32).reshape(-1, MLP_IN)
     y = np.empty((len(xs), 2), dtype=np.float32)
     load().or_mlp_forward(ptr(w), len(xs), ptr(xs), ptr(y), int(bool(bf16)))
     return y
@@ -248,7 +250,7 @@ def mlp_forward(params, x, bf16=True):
 
 def mlp_grad(params, x, t, bf16=True):
     w = c64(params)
-    xs = c64(x, np.float32).reshape(-1, 8)
+    xs = c64(x, np.float32).reshape(-1, MLP_IN)
     ts = c64(t, np.float32).reshape(-1, 2)
     g = np.empty(MLP_PARAMS)
     loss = load().or_mlp_grad(ptr(w), len(xs), ptr(xs), ptr(ts), ptr(g), int(bool(bf16)))
@@ -262,10 +264,13 @@ def quadrupole(d, I, eps2=0.0):
     return np.array([ax.value, ay.value])
 
 
-def mlp_features(d, s, M, I, eps2, m_root, ell):
-    f = np.empty(8)
+def mlp_features(d, s, M, I, eps2, m_root, ell, T=(0.0, 0.0, 0.0, 0.0)):
+    """Network inputs f[12] and output scale; T = third moments {xxx, xxy,
+    xyy, yyy} about the COM."""
+    f = np.empty(MLP_IN)
+    t = c64(T)
     scale = load().or_mlp_features(float(d[0]), float(d[1]), float(s), float(M), float(I[0]),
-                                   float(I[1]), float(I[2]), float(eps2), float(m_root),
+                                   float(I[1]), float(I[2]), ptr(t), float(eps2), float(m_root),
                                    float(ell), ptr(f))
     return f, scale
 
@@ -285,9 +290,3 @@ def cell_correction(pos, mass, bs, be, x, eps2):
                               float(eps2), ptr(out))
     return out
 
-
-def quadrupole_jacobian(d, I, eps2=0.0):
-    J = np.empty(4)
-    load().or_quadrupole_jacobian(float(d[0]), float(d[1]), float(I[0]), float(I[1]),
-                                  float(I[2]), float(eps2), ptr(J))
-    return J.reshape(2, 2)

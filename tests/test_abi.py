@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     assert C.sizeof(abi.Params) == 4 * 8 + 12 * 4
     assert C.sizeof(abi.Info) == 8 * 8
-    assert abi.NODE_DTYPE.itemsize == 4 * 4 + 4 * 8 + 4 * 4
+    assert abi.NODE_DTYPE.itemsize == 4 * 4 + 4 * 8 + 4 * 4 + 4 * 4
 
 
 def test_params_default():

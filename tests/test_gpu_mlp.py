@@ -12,14 +12,18 @@
             the quadrupole term) against the oracle's FP64 restatement on the
             same tree
   step      per body, learned minus monopole equals the oracle's correction
-            (quadrupole term plus network at the subgroup centre, quadrupole
-            Jacobian across the subgroup) on the walk's own cell lists;
+            (each accepted cell's softened quadrupole term at the body, the
+            network at the body's subgroup centre) on the walk's own cell
+            lists;
             growing the accepted-cell regions mid-step changes nothing; graph
             and eager steps agree
+  moments   the tree's third moments about each cell's COM (the network's
+            last four inputs) against a brute-force FP64 sum over the cell's
+            bodies
   gain      a network trained on one disk lowers the force error of another
-            disk at theta = 0.8 against the FP64 direct sum, well below the
-            monopole error at the same theta (the quadrupole term alone, zero
-            weights, is reported beside it)
+            disk against the FP64 direct sum at theta = 0.8 and 0.5: below
+            0.85 times the error of the quadrupole term alone (zero weights),
+            and below half the monopole error
 """
 import numpy as np
 import pytest
@@ -27,7 +31,7 @@ import pytest
 import nbody2d
 import oracle_lib as O
 from nbody2d import ics
-from nbody2d.abi import FAR_LEARNED, FAR_MONOPOLE, TRAV_GROUP
+from nbody2d.abi import FAR_LEARNED, FAR_MONOPOLE, MLP_INPUTS, MLP_PARAMS, TRAV_GROUP
 
 pytestmark = pytest.mark.gpu
 
@@ -41,7 +45,7 @@ def _ctx(m, p, v, **kw):
 
 def _features(rng, k):
     ang = rng.uniform(0, 2 * np.pi, k)
-    x = np.empty((k, 8), dtype=np.float32)
+    x = np.empty((k, MLP_INPUTS), dtype=np.float32)
     x[:, 0], x[:, 1] = np.cos(ang), np.sin(ang)
     x[:, 2] = rng.uniform(0.0, 0.8, k)
     x[:, 3] = rng.uniform(-1.3, 0.0, k)
@@ -49,6 +53,7 @@ def _features(rng, k):
     x[:, 5] = rng.uniform(-0.1, 0.1, k)
     x[:, 6] = rng.uniform(0.0, 0.3, k)
     x[:, 7] = rng.uniform(-0.5, 1.5, k)
+    x[:, 8:12] = rng.uniform(-0.1, 0.1, (k, 4))
     return x
 
 
@@ -60,9 +65,9 @@ def _small_ctx(**kw):
 def test_mfma_forward_known_linear_map():
     from test_oracle_mlp import _linear_net
     rng = np.random.default_rng(2)
-    A = rng.integers(-8, 9, size=(2, 8)) / 4.0
+    A = rng.integers(-8, 9, size=(2, MLP_INPUTS)) / 4.0
     b = np.array([0.5, -1.25])
-    x = (rng.integers(-64, 65, size=(1000, 8)) / 16.0).astype(np.float32)
+    x = (rng.integers(-64, 65, size=(1000, MLP_INPUTS)) / 16.0).astype(np.float32)
     with _small_ctx() as c:
         c.mlp_set_weights(_linear_net(A, b).astype(np.float32))
         y = c.mlp_forward(x)
@@ -95,9 +100,8 @@ def test_training_gradient_matches_oracle():
         g, loss = c.mlp_grad(x, t)
     ref, rloss = O.mlp_grad(w.astype(np.float64), x, t, bf16=True)
     assert loss == pytest.approx(rloss, rel=1e-4)
-    blocks = {"W1": (0, 512), "b1": (512, 576), "W2": (576, 4672), "b2": (4672, 4736),
-              "W3": (4736, 4864), "b3": (4864, 4866)}
-    for name, (a, b) in blocks.items():
+    ends = list(O.MLP_OFF.values())[1:] + [MLP_PARAMS]
+    for (name, a), b in zip(O.MLP_OFF.items(), ends):
         err = np.linalg.norm(g[a:b] - ref[a:b]) / np.linalg.norm(ref[a:b])
         print(f"[gpu] grad {name} rel err {err:.2e}")
         assert err <= 2e-3, name
@@ -126,7 +130,7 @@ def test_training_pairs_match_oracle():
         n = nd[q]
         I = (n["ixx"], n["ixy"], n["iyy"])
         f, scale = O.mlp_features(d[q], np.sqrt(n["wsq"]), n["mass"], I, eps2, nodes[0]["mass"],
-                                  EPS)
+                                  EPS, T=n["t3"])
         np.testing.assert_allclose(x[q], f, rtol=2e-5, atol=2e-6)
         corr = O.cell_correction(sp, sm, n["bstart"], n["bend"], sp[body[q]], EPS * EPS)
         u = np.array([f[0], f[1]])
@@ -146,7 +150,7 @@ def test_learned_walk_matches_monopole_walk(theta):
     # the acceleration
     m, p, v = ics.exp_disk(16384, seed=6)
     kw = dict(theta=theta, softening=EPS, traversal=TRAV_GROUP, self_guard=int(theta > 0.71))
-    w = (np.random.default_rng(5).normal(size=4866) * 0.05).astype(np.float32)
+    w = (np.random.default_rng(5).normal(size=MLP_PARAMS) * 0.05).astype(np.float32)
     with _ctx(m, p, v, far_field=FAR_MONOPOLE, **kw) as a, \
             _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as b:
         b.mlp_set_weights(w)
@@ -169,14 +173,14 @@ def _train(m, p, v, theta, steps=4000):
         return c.mlp_weights(), k, l0, l2
 
 
-def test_trained_network_lowers_force_error():
-    theta = 0.8
+@pytest.mark.parametrize("theta", [0.8, 0.5])
+def test_trained_network_lowers_force_error(theta):
     w, k, l0, l2 = _train(*ics.exp_disk(32768, seed=21), theta)
     assert l2 < 0.1 * l0
     m, p, v = ics.exp_disk(32768, seed=22)  # another disk than the training one
     tg = ics.sample_targets(len(m), 2048, 1)
     ref = O.direct_sum(p, m, EPS * EPS, tg)
-    kw = dict(theta=theta, softening=EPS, traversal=TRAV_GROUP, self_guard=1)
+    kw = dict(theta=theta, softening=EPS, traversal=TRAV_GROUP, self_guard=int(theta > 0.71))
     with _ctx(m, p, v, far_field=FAR_MONOPOLE, **kw) as a, \
             _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as b, \
             _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as z:
@@ -189,13 +193,38 @@ def test_trained_network_lowers_force_error():
     ez = ics.force_errors(fz[tg], ref)
     print(f"[gpu] theta {theta}: monopole rms {ea[0]:.3e} max {ea[1]:.3e}; "
           f"quadrupole rms {ez[0]:.3e} max {ez[1]:.3e}; "
-          f"learned rms {eb[0]:.3e} max {eb[1]:.3e}")
+          f"learned rms {eb[0]:.3e} max {eb[1]:.3e}; network gain {eb[0] / ez[0]:.3f}")
     assert eb[0] < 0.5 * ea[0]
+    assert eb[0] <= 0.85 * ez[0]
+
+
+def test_third_moments_match_brute_force():
+    m, p, v = ics.exp_disk(8192, seed=14)
+    with _ctx(m, p, v, theta=0.8, softening=EPS, traversal=TRAV_GROUP, far_field=FAR_LEARNED) as c:
+        box, skeys, orig, cnt = c.debug_build()
+        nodes = c.debug_tree(cnt)
+    sp, sm = p[orig], m[orig]
+    cells = np.nonzero((nodes["bstart"] >= 0) & (nodes["wsq"] > 0))[0]
+    pick = np.random.default_rng(2).choice(cells, 1500, replace=False)
+    worst = 0.0
+    for q in np.concatenate([[0], pick]):
+        n = nodes[q]
+        bs, be = int(n["bstart"]), int(n["bend"])
+        w, x = sm[bs:be + 1], sp[bs:be + 1]
+        M = w.sum()
+        d = x - (w[:, None] * x).sum(0) / M
+        want = np.array([(w * d[:, 0] ** 3).sum(), (w * d[:, 0] ** 2 * d[:, 1]).sum(),
+                         (w * d[:, 0] * d[:, 1] ** 2).sum(), (w * d[:, 1] ** 3).sum()])
+        scale = M * n["wsq"] ** 1.5
+        err = np.abs(n["t3"].astype(np.float64) - want).max() / scale
+        worst = max(worst, err)
+        assert err <= 1e-5, (q, n["t3"], want)
+    print(f"[gpu] third moments vs brute force: max err / (M s^3) {worst:.2e}")
 
 
 def test_learned_graph_and_eager_steps_agree():
     m, p, v = ics.exp_disk(8192, seed=3)
-    w = np.random.default_rng(1).normal(size=4866).astype(np.float32) * 0.05
+    w = np.random.default_rng(1).normal(size=MLP_PARAMS).astype(np.float32) * 0.05
     kw = dict(theta=0.8, softening=EPS, traversal=TRAV_GROUP, self_guard=1,
               far_field=FAR_LEARNED)
     with _ctx(m, p, v, use_graph=1, **kw) as a, _ctx(m, p, v, use_graph=0, **kw) as b:
@@ -210,12 +239,11 @@ def test_learned_graph_and_eager_steps_agree():
 
 def test_far_field_correction_matches_oracle():
     """Per body, learned minus monopole equals the oracle's sum over the
-    subgroup's accepted cells of the correction at the subgroup centre (the
-    softened quadrupole term plus s/R times the network's output), minus the
-    summed softened-quadrupole Jacobians times the body's offset from that
+    subgroup's accepted cells of the softened quadrupole term at the body's
+    own position plus s/R times the network's output at the subgroup
     centre."""
     m, p, v = ics.exp_disk(4096, seed=12)
-    w = (np.random.default_rng(3).normal(size=4866) * 0.1).astype(np.float32)
+    w = (np.random.default_rng(3).normal(size=MLP_PARAMS) * 0.1).astype(np.float32)
     kw = dict(theta=0.8, softening=EPS, traversal=TRAV_GROUP, self_guard=1)
     with _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as b, \
             _ctx(m, p, v, far_field=FAR_MONOPOLE, **kw) as a:
@@ -226,6 +254,7 @@ def test_far_field_correction_matches_oracle():
         nodes = b.debug_tree(cnt)
         fa, _, _ = a.forces()
     got = (fb - fa)[orig]                       # sorted order, G = 1
+    sp = p[orig]
     eps2 = float(np.float32(EPS * EPS))
     m_root = nodes[0]["mass"]
     want = np.zeros_like(got)
@@ -236,28 +265,31 @@ def test_far_field_correction_matches_oracle():
             st, ct = rng_[g, k]
             nd = nodes[slots.ravel()[st:st + ct]]
             assert np.all(nd["c0"] >= 1)
-            X, S, U, Q = [], [], [], []
-            K = np.zeros((2, 2))
+            X, S, U = [], [], []
             for c in nd:
                 d = np.array([c["cx"], c["cy"]]) - centre[g, k]
                 I = (c["ixx"], c["ixy"], c["iyy"])
                 f, sc = O.mlp_features(d, np.sqrt(c["wsq"]), np.float32(c["mass"]), I, eps2,
-                                       m_root, EPS)
+                                       m_root, EPS, T=c["t3"])
                 X.append(f)
                 S.append(sc)
                 U.append(d / np.linalg.norm(d))
-                Q.append(O.mlp_quad_units(f, (d @ d) / (d @ d + eps2)))
-                K += O.quadrupole_jacobian(d, I, eps2)
             C = np.zeros(2)
             if ct:
                 y = O.mlp_forward(w.astype(np.float64), np.array(X, dtype=np.float32))
-                for yy, sc, u, qq, ff in zip(y.astype(np.float64), S, U, Q, X):
-                    yy = qq + ff[2] * yy
-                    C += sc * (yy[0] * u + yy[1] * np.array([-u[1], u[0]]))
+                for yy, sc, u, ff in zip(y.astype(np.float64), S, U, X):
+                    C += sc * ff[2] * (yy[0] * u + yy[1] * np.array([-u[1], u[0]]))
             members = np.nonzero((bsub == k) & (np.arange(len(bsub)) // 64 == g))[0]
             for i in members:
-                want[i] = C - K @ boff[i].astype(np.float64)
-                scale_ref[i] = np.abs(C).sum() + np.abs(K @ boff[i]).sum()
+                Q = np.zeros(2)
+                Qabs = 0.0
+                for c in nd:
+                    d = np.array([c["cx"], c["cy"]]) - sp[i]
+                    q = O.quadrupole(d, (c["ixx"], c["ixy"], c["iyy"]), eps2)
+                    Q += q
+                    Qabs += np.abs(q).sum()
+                want[i] = C + Q
+                scale_ref[i] = np.abs(C).sum() + Qabs
                 checked += 1
     assert checked == len(m)
     err = np.abs(got - want).max(1) / np.maximum(scale_ref, 1e-30)
@@ -269,7 +301,7 @@ def test_far_field_correction_matches_oracle():
 
 def test_region_growth_keeps_learned_forces():
     m, p, v = ics.exp_disk(8192, seed=13)
-    w = (np.random.default_rng(4).normal(size=4866) * 0.1).astype(np.float32)
+    w = (np.random.default_rng(4).normal(size=MLP_PARAMS) * 0.1).astype(np.float32)
     kw = dict(theta=0.8, softening=EPS, traversal=TRAV_GROUP, self_guard=1,
               far_field=FAR_LEARNED)
     with _ctx(m, p, v, **kw) as a, _ctx(m, p, v, cell_capacity=8, **kw) as b:

@@ -2,7 +2,6 @@
 
 The reference has no learned correction, no second moments and no network, so
 nothing here has a reference vector: each function is pinned by mathematics.
-  or_quadrupole_jacobian  central differences of or_quadrupole
   or_quadrupole       order of the residual exact - monopole - quadrupole
                       against brute-force sums (a wrong sign or coefficient
                       leaves a second-order residual), with and without eps
@@ -77,9 +76,50 @@ def test_mlp_features_hand_values():
     # I = (0.2, -0.1, 0.4): Iu = (0.04, 0.26), u.Iu = 0.232, w.Iu = 0.124,
     # w.Iw = tr I - u.Iu = 0.368
     f, scale = O.mlp_features((3.0, 4.0), 2.0, 0.5, (0.2, -0.1, 0.4), 11.0, 8.0, 0.25)
-    want = [0.6, 0.8, 2.0 / 6.0, -4.0 / 16.0, 0.116, 0.062, 0.184, np.log2(20.0) / 8.0]
-    np.testing.assert_allclose(f, want, rtol=1e-13)
+    want = [0.6, 0.8, 2.0 / 6.0, -4.0 / 16.0, 0.116, 0.062, 0.184, np.log2(20.0) / 8.0,
+            0.0, 0.0, 0.0, 0.0]
+    np.testing.assert_allclose(f, want, rtol=1e-13, atol=1e-15)
     assert scale == pytest.approx(0.5 * 4.0 * 5.0 / 6.0 ** 5, rel=1e-14)
+    # third moments: only T_xxx = 1.6 (M s^3 = 4); along u = (0.6, 0.8):
+    # T_uuu = 0.6^3 1.6, T_uuw = 0.6^2 (-0.8) 1.6, T_uww = 0.6 0.64 1.6,
+    # T_www = (-0.8)^3 1.6
+    f, _ = O.mlp_features((3.0, 4.0), 2.0, 0.5, (0.2, -0.1, 0.4), 11.0, 8.0, 0.25,
+                          T=(1.6, 0.0, 0.0, 0.0))
+    np.testing.assert_allclose(f[8:], np.array([0.216, -0.288, 0.384, -0.512]) * 0.4,
+                               rtol=1e-13)
+
+
+def test_mlp_third_moment_inputs_are_the_cell_moments_along_u_and_w():
+    """For bodies at offsets e_j from their COM, T = sum m e e e; the inputs
+    f8..f11 are sum m (e.u)^a (e.w)^b over M s^3 (a + b = 3): an independent
+    formulation of the contraction, and rotating the cell with the offset
+    leaves them unchanged."""
+    rng = np.random.default_rng(12)
+    for _ in range(10):
+        e = rng.normal(size=(7, 2)) * 0.3
+        m = rng.uniform(0.2, 2.0, 7)
+        e -= (m[:, None] * e).sum(0) / m.sum()
+        M, s = m.sum(), 1.3
+        T = [float((m * e[:, 0] ** 3).sum()), float((m * e[:, 0] ** 2 * e[:, 1]).sum()),
+             float((m * e[:, 0] * e[:, 1] ** 2).sum()), float((m * e[:, 1] ** 3).sum())]
+        I = [float((m * e[:, 0] ** 2).sum()), float((m * e[:, 0] * e[:, 1]).sum()),
+             float((m * e[:, 1] ** 2).sum())]
+        d = rng.normal(size=2) * 4
+        f, _ = O.mlp_features(d, s, M, I, 0.01, 10.0, 0.1, T=T)
+        u = d / np.linalg.norm(d)
+        w = np.array([-u[1], u[0]])
+        a, b = e @ u, e @ w
+        want = [(m * a ** 3).sum(), (m * a * a * b).sum(), (m * a * b * b).sum(), (m * b ** 3).sum()]
+        np.testing.assert_allclose(f[8:], np.array(want) / (M * s ** 3), rtol=1e-10, atol=1e-13)
+        c, sn = np.cos(0.7), np.sin(0.7)
+        R = np.array([[c, -sn], [sn, c]])
+        er = e @ R.T
+        Tr = [float((m * er[:, 0] ** 3).sum()), float((m * er[:, 0] ** 2 * er[:, 1]).sum()),
+              float((m * er[:, 0] * er[:, 1] ** 2).sum()), float((m * er[:, 1] ** 3).sum())]
+        Ir = [float((m * er[:, 0] ** 2).sum()), float((m * er[:, 0] * er[:, 1]).sum()),
+              float((m * er[:, 1] ** 2).sum())]
+        fr, _ = O.mlp_features(R @ d, s, M, Ir, 0.01, 10.0, 0.1, T=Tr)
+        np.testing.assert_allclose(fr[2:], f[2:], rtol=1e-9, atol=1e-12)
 
 
 @pytest.mark.parametrize("eps2", [0.0, 0.3])
@@ -107,20 +147,22 @@ def test_quadrupole_in_network_units_is_linear_in_moment_inputs(eps2):
 def _linear_net(A, b):
     """Parameters of a ReLU network computing y = A x + b exactly:
     h1 = (relu(x), relu(-x)), h2 = a permutation of h1, y = A (h2_+ - h2_-) + b."""
-    W1 = np.zeros((64, 8))
+    n = O.MLP_IN
+    W1 = np.zeros((64, n))
     b1 = np.zeros(64)
-    for j in range(8):
+    for j in range(n):
         W1[j, j] = 1.0
-        W1[8 + j, j] = -1.0
-    perm = np.array([(5 * k + 3) % 16 for k in range(16)])  # h2[perm[k]] = h1[k]
+        W1[n + j, j] = -1.0
+    perm = np.array([(5 * k + 3) % (2 * n) for k in range(2 * n)])  # h2[perm[k]] = h1[k]
+    assert len(set(perm.tolist())) == 2 * n
     W2 = np.zeros((64, 64))
-    for k in range(16):
+    for k in range(2 * n):
         W2[perm[k], k] = 1.0
     b2 = np.zeros(64)
     W3 = np.zeros((2, 64))
-    for j in range(8):
+    for j in range(n):
         W3[:, perm[j]] = A[:, j]
-        W3[:, perm[8 + j]] = -A[:, j]
+        W3[:, perm[n + j]] = -A[:, j]
     return np.concatenate([W1.ravel(), b1, W2.ravel(), b2, W3.ravel(), b])
 
 
@@ -128,10 +170,10 @@ def _linear_net(A, b):
 def test_mlp_forward_computes_a_known_linear_map(bf16):
     rng = np.random.default_rng(2)
     # entries representable in bf16 so the bf16 path is exact as well
-    A = rng.integers(-8, 9, size=(2, 8)) / 4.0
+    A = rng.integers(-8, 9, size=(2, O.MLP_IN)) / 4.0
     b = np.array([0.5, -1.25])
     p = _linear_net(A, b)
-    x = (rng.integers(-64, 65, size=(50, 8)) / 16.0).astype(np.float32)
+    x = (rng.integers(-64, 65, size=(50, O.MLP_IN)) / 16.0).astype(np.float32)
     y = O.mlp_forward(p, x, bf16=bf16)
     np.testing.assert_array_equal(y, (x.astype(np.float64) @ A.T + b).astype(np.float32))
 
@@ -145,11 +187,11 @@ def _bf16_round_np(v):
 def test_mlp_forward_bf16_rounding_matches_bitwise_rne():
     # one active path: y0 = relu(w x) with weight 1, so y0 = bf16(bf16(x))
     p = np.zeros(O.MLP_PARAMS)
-    p[0] = 1.0               # W1[0][0]
-    p[512 + 64] = 1.0        # W2[0][0]
-    p[512 + 64 + 4096 + 64] = 1.0   # W3[0][0]
+    p[O.MLP_OFF["W1"]] = 1.0   # W1[0][0]
+    p[O.MLP_OFF["W2"]] = 1.0   # W2[0][0]
+    p[O.MLP_OFF["W3"]] = 1.0   # W3[0][0]
     rng = np.random.default_rng(3)
-    x = np.zeros((2000, 8), dtype=np.float32)
+    x = np.zeros((2000, O.MLP_IN), dtype=np.float32)
     x[:, 0] = np.abs(rng.normal(size=2000)).astype(np.float32) * 3
     # exact ties of bf16 rounding (round half to even)
     x[:4, 0] = np.array([1 + 2 ** -8, 1 + 3 * 2 ** -8, 2 + 2 ** -7, 1 + 2 ** -9], np.float32)
@@ -161,16 +203,17 @@ def test_mlp_forward_bf16_rounding_matches_bitwise_rne():
 def test_mlp_grad_matches_finite_differences():
     rng = np.random.default_rng(4)
     p = rng.normal(size=O.MLP_PARAMS) * 0.3
-    x = rng.normal(size=(6, 8)).astype(np.float32)
+    x = rng.normal(size=(6, O.MLP_IN)).astype(np.float32)
     t = rng.normal(size=(6, 2)).astype(np.float32)
     g, loss = O.mlp_grad(p, x, t, bf16=False)
 
     def L(q):
         return O.mlp_grad(q, x, t, bf16=False)[1]  # the FP64 loss of the forward
 
-    idx = np.concatenate([rng.choice(512, 6, replace=False), 512 + np.arange(3),
-                          576 + rng.choice(4096, 8, replace=False), 4672 + np.arange(3),
-                          4736 + rng.choice(128, 6, replace=False), [4864, 4865]])
+    off = O.MLP_OFF
+    idx = np.concatenate([rng.choice(768, 8, replace=False), off["b1"] + np.arange(3),
+                          off["W2"] + rng.choice(4096, 8, replace=False), off["b2"] + np.arange(3),
+                          off["W3"] + rng.choice(128, 6, replace=False), [off["b3"], off["b3"] + 1]])
     h = 1e-6
     for i in idx:
         q1, q2 = p.copy(), p.copy()
@@ -180,18 +223,3 @@ def test_mlp_grad_matches_finite_differences():
         assert fd == pytest.approx(g[i], rel=1e-5, abs=1e-8), i
     y = O.mlp_forward(p, x, bf16=False).astype(np.float64)
     assert loss == pytest.approx(((y - t) ** 2).sum(1).mean(), rel=1e-6)
-
-
-@pytest.mark.parametrize("eps2", [0.0, 0.3])
-def test_quadrupole_jacobian_matches_central_differences(eps2):
-    rng = np.random.default_rng(9)
-    for _ in range(20):
-        d = rng.normal(size=2) * 2
-        I = np.array([rng.uniform(0.1, 1), rng.uniform(-0.3, 0.3), rng.uniform(0.1, 1)])
-        J = O.quadrupole_jacobian(d, I, eps2)
-        h = 1e-6 * np.linalg.norm(d)
-        for j in range(2):
-            e = np.zeros(2)
-            e[j] = h
-            fd = (O.quadrupole(d + e, I, eps2) - O.quadrupole(d - e, I, eps2)) / (2 * h)
-            np.testing.assert_allclose(J[:, j], fd, rtol=1e-6, atol=1e-9 * np.abs(J).max())
