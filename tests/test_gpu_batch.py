@@ -64,7 +64,7 @@ def test_batch_equals_single_systems(theta):
 
 def test_batch_learned_equals_single_systems():
     kw = dict(theta=0.6, softening=EPS, traversal=TRAV_GROUP, far_field=FAR_LEARNED)
-    w = (np.random.default_rng(8).normal(size=4866) * 0.05).astype(np.float32)
+    w = (np.random.default_rng(8).normal(size=nbody2d.abi.MLP_PARAMS) * 0.05).astype(np.float32)
     sizes = [2048, 300, 65, 5000]
     systems, m, p, v, off = _systems(sizes, seed=200)
     with nbody2d.Context(m, p, v, offsets=off, **kw) as b:

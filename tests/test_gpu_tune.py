@@ -21,7 +21,7 @@ def test_autotune_group_traversal(learned):
         kw["far_field"] = FAR_LEARNED
     with nbody2d.Context(m, p, v, **kw) as c:
         if learned:
-            c.mlp_set_weights((np.random.default_rng(2).normal(size=4866) * 0.05)
+            c.mlp_set_weights((np.random.default_rng(2).normal(size=nbody2d.abi.MLP_PARAMS) * 0.05)
                               .astype(np.float32))
         f0, i0, _ = c.forces()
         t = c.autotune(2)
