@@ -24,6 +24,12 @@ __device__ __forceinline__ uint64_t readfirst64(uint64_t v) {
 }
 
 // Inclusive scan of a u32 across the 64 lanes of a wave.
+// Set bits of a wave mask in the lanes below this one (v_mbcnt).
+__device__ __forceinline__ uint32_t mask_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     const int l = lane_id();
 #pragma unroll

@@ -525,13 +525,16 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                     }
                     ccnt += (uint32_t)__popcll(cmk);
                 }
-                // push the non-empty children of opened cells
+                // push the non-empty children of opened cells: a lane's slot
+                // is the children of the lanes below it, counted from one
+                // ballot per child position (no cross-lane scan)
                 const int32_t cm = open ? r.mask : 0;
-                const uint32_t nc = (uint32_t)__popc(cm);
-                const uint32_t incl = wave_incl_scan(nc);
-                const int total = (int)__shfl(incl, 63, 64);
+                const uint64_t q0 = ballot64(cm & 1), q1 = ballot64(cm & 2);
+                const uint64_t q2 = ballot64(cm & 4), q3 = ballot64(cm & 8);
+                const int total = __popcll(q0) + __popcll(q1) + __popcll(q2) + __popcll(q3);
                 if (open) {
-                    int at = sp + (int)(incl - nc);
+                    int at = sp + (int)(mask_below(q0) + mask_below(q1) + mask_below(q2) +
+                                        mask_below(q3));
 #pragma unroll
                     for (int q = 3; q >= 0; q--)
                         if ((cm >> q) & 1) s_stack[at++] = c0 + q;
