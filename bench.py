@@ -66,7 +66,7 @@ TRAIN_SAMPLE_BODIES = 65536
 TRAIN_BATCH = 16384
 TRAIN_SCHEDULE = ((1.0, 2e-3), (0.5, 5e-4), (0.5, 1e-4))
 # Plain legs timed to find the plain theta at a learned leg's error.
-EQUAL_ERROR_THETAS = (0.4, 0.35, 0.3, 0.25, 0.2, 0.15, 0.12, 0.1)
+EQUAL_ERROR_THETAS = (0.4, 0.35, 0.3, 0.25, 0.2, 0.15, 0.12, 0.1, 0.08, 0.07)
 # Ensemble network: pairs from this many systems spread over the size range.
 ENS_TRAIN_SYSTEMS = 16
 # A second learned point, at the error that the 1e-3 target asks for.
@@ -84,29 +84,99 @@ def traffic_from_profile(kernel: str):
     return None
 
 
-def cpu_baseline(mass, pos, vel, theta, target_s: float = 12.0):
-    """Reference threaded CPU Barnes-Hut (oracle restatement, reference flags)."""
-    sys.path.insert(0, str(REPO / "tests"))
-    import oracle_lib as O  # the cpu_baseline leg is the only bench use of oracle/
+def host_cpu() -> dict:
+    """The host's CPU model and logical CPU count, and the CPUs this process
+    may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count()
+    return {"cpu_model": model, "host_logical_cpus": os.cpu_count(), "allowed_cpus": allowed}
 
+
+def cpu_threads() -> int:
+    """Threads for the threaded CPU baseline: the CPU share this job is given
+    (OMP_NUM_THREADS on a shared host), at most the host's CPUs and the
+    reference's cap of 256 (Constants.hpp:22)."""
     share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    threads = max(1, min(share, os.cpu_count() or 1, 256))
-    rc, p1, v1, t1 = O.refcpu_bh_run(pos, vel, mass, 1.0, DT, theta, EPS * EPS, threads, 1,
+    return max(1, min(share, os.cpu_count() or 1, 256))
+
+
+def cpu_bh_rate(O, mass, pos, vel, theta, threads, steps, warmup=1):
+    """Body-steps/s of the reference's threaded CPU Barnes-Hut (the oracle's
+    restatement at the reference's flags) over `steps` steps after `warmup`."""
+    p, v = pos, vel
+    if warmup:
+        rc, p, v, _ = O.refcpu_bh_run(pos, vel, mass, 1.0, DT, theta, EPS * EPS, threads,
+                                      warmup, fast=True)
+        if rc != 0:
+            raise RuntimeError(f"cpu baseline failed: {rc}")
+    rc, _, _, secs = O.refcpu_bh_run(p, v, mass, 1.0, DT, theta, EPS * EPS, threads, steps,
                                      fast=True)
     if rc != 0:
         raise RuntimeError(f"cpu baseline failed: {rc}")
-    steps = max(1, min(20, int(target_s / max(t1, 1e-3))))
-    rc, _, _, secs = O.refcpu_bh_run(p1, v1, mass, 1.0, DT, theta, EPS * EPS, threads, steps,
-                                     fast=True)
-    return {
-        "value": len(mass) * steps / secs,
+    return len(mass) * steps / secs, secs
+
+
+def cpu_baseline(mass, pos, vel, theta, target_s: float = 12.0):
+    """Reference threaded CPU Barnes-Hut (oracle restatement, reference flags)
+    on the benchmark's own bodies, plus one thread on the same bodies and the
+    16,384-body disk of 100 steps that the reference runs itself."""
+    sys.path.insert(0, str(REPO / "tests"))
+    import oracle_lib as O  # the cpu_baseline leg is the only bench use of oracle/
+    from nbody2d import ics
+
+    threads = cpu_threads()
+    rate1, t1 = cpu_bh_rate(O, mass, pos, vel, theta, threads, 1)
+    steps = max(1, min(20, int(target_s * rate1 / len(mass))))
+    rate, secs = cpu_bh_rate(O, mass, pos, vel, theta, threads, steps)
+    # one thread, one step of the same bodies (the data are warm from above)
+    rate_1t, secs_1t = cpu_bh_rate(O, mass, pos, vel, theta, 1, 1, warmup=0)
+    # the 16,384-body exponential disk, 100 steps (all threads), 20 (one thread)
+    m0, p0, v0 = ics.exp_disk(1 << 14, seed=SEED)
+    c0, c0s = cpu_bh_rate(O, m0, p0, v0, theta, threads, 100)
+    c0_1t, c0s_1t = cpu_bh_rate(O, m0, p0, v0, theta, 1, 20)
+    out = {
+        "value": rate,
         "unit": "body-steps/s",
         "cores": threads,
         "kind": "port",
         "sample": f"exp_disk N={len(mass)} theta={theta} (diagonal MAC, reference CPU force "
                   f"kernel), {steps} steps after 1 warm-up step, {threads} threads, "
-                  f"{secs:.2f} s",
+                  f"{secs:.2f} s; the diagonal test at theta {theta} opens cells like the side "
+                  f"test at about theta {theta / np.sqrt(2):.2f}",
+        "one_thread": {"value": rate_1t, "steps": 1, "seconds": secs_1t},
+        "exp_disk_16k": {"n": 1 << 14, "theta": theta, "threads": threads, "steps": 100,
+                         "value": c0, "seconds": c0s,
+                         "one_thread": {"steps": 20, "value": c0_1t, "seconds": c0s_1t}},
     }
+    out.update(host_cpu())
+    return out
+
+
+def cpu_allpairs_rate(mass, pos, vel, n_sample: int = 16384):
+    """Interactions/s of the reference's single-threaded all-pairs step
+    (AllPairs.cpp:38-49: every pair once, symmetric update, drift then kick)
+    timed on the first n_sample bodies; a step delivers N(N-1) interactions in
+    N(N-1)/2 pair evaluations at any N, so the rate carries over to the full N."""
+    sys.path.insert(0, str(REPO / "tests"))
+    import oracle_lib as O  # the cpu_baseline leg is the only bench use of oracle/
+
+    k = min(n_sample, len(mass))
+    m, p, v = mass[:k], pos[:k], vel[:k]
+    t0 = time.perf_counter()
+    O.refcpu_allpairs_run(p, v, m, 1.0, DT, EPS * EPS, 1)
+    secs = time.perf_counter() - t0
+    return float(k) * (k - 1) / secs, k, secs
 
 
 def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
@@ -375,6 +445,14 @@ def run_allpairs(nbody2d, ics, args) -> dict:
     force_ms = phase_ms["force"] / steps_timed
     pairs = float(n) * (n - 1)
     achieved = pairs * FLOPS_PER_INTERACTION / (force_ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        rate, k, secs = cpu_allpairs_rate(mass, pos, vel)
+        cpu = {"value": rate, "unit": "interactions/s", "cores": 1, "kind": "port",
+               "sample": f"one single-threaded all-pairs step of the first {k} bodies "
+                         f"({secs:.2f} s); its rate, N(N-1) interactions per step, is that "
+                         f"of an N={n} step ({pairs / rate:.0f} s)"}
+        cpu.update(host_cpu())
     return {
         "metric": "all-pairs interactions/s (N=262,144 uniform disk)",
         "value": pairs * args.steps / wall,
@@ -394,7 +472,7 @@ def run_allpairs(nbody2d, ics, args) -> dict:
                      "traffic": traffic_from_profile("k_allpairs"),
                      "kernel": "k_allpairs (FP32 vector; 14 flops per interaction)",
                      "avg_launch_ms": force_ms},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
 
 
@@ -429,6 +507,17 @@ def run_two_disks(nbody2d, ics, args) -> dict:
     ctx.close()
     per = {k: v / steps_timed for k, v in phase_ms.items()}
     achieved = inter_timed / steps_timed * FLOPS_PER_INTERACTION / (per["force"] * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(REPO / "tests"))
+        import oracle_lib as O  # the cpu_baseline leg is the only bench use of oracle/
+
+        threads = cpu_threads()
+        rate, secs = cpu_bh_rate(O, mass, pos, vel, args.theta_plain, threads, 1, warmup=0)
+        cpu = {"value": rate, "unit": "body-steps/s", "cores": threads, "kind": "port",
+               "sample": f"one step of the initial state, {threads} threads, {secs:.2f} s "
+                         f"(diagonal MAC, reference CPU force kernel)"}
+        cpu.update(host_cpu())
     return {
         "metric": "N=4M two-disk body-steps/s (plain BH)",
         "value": n * args.steps / wall, "unit": "body-steps/s",
@@ -449,7 +538,7 @@ def run_two_disks(nbody2d, ics, args) -> dict:
                      "traffic": None,
                      "kernel": "k_traverse_group (FP32 vector; 14 flops per interaction)",
                      "avg_launch_ms": per["force"]},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
 
 
@@ -540,6 +629,22 @@ def run_ensemble(nbody2d, ics, args) -> dict:
     for c in ctxs:
         c.close()
     per = {k: v / steps_timed for k, v in phase_ms.items()}
+    cpu = None
+    if not args.no_cpu:
+        # plain Barnes-Hut (the reference has no learned far field), one step
+        # of every system in turn on all threads
+        sys.path.insert(0, str(REPO / "tests"))
+        import oracle_lib as O  # the cpu_baseline leg is the only bench use of oracle/
+
+        threads = cpu_threads()
+        secs = 0.0
+        for sm, sp, sv in systems:
+            secs += cpu_bh_rate(O, sm, sp, sv, theta, threads, 1, warmup=0)[1]
+        cpu = {"value": n / secs, "unit": "body-steps/s", "cores": threads, "kind": "port",
+               "sample": f"one plain Barnes-Hut step (theta {theta}, diagonal MAC, reference "
+                         f"CPU force kernel) of each of the {len(sizes)} systems in turn, "
+                         f"{threads} threads, {secs:.2f} s"}
+        cpu.update(host_cpu())
     return {
         "metric": "batched ensemble body-steps/s (512 systems, learned theta=0.6)",
         "value": n * args.steps / wall_b, "unit": "body-steps/s",
@@ -565,7 +670,7 @@ def run_ensemble(nbody2d, ics, args) -> dict:
         "training": {"pairs": pairs, "loss": loss, "seconds": train_s,
                      "systems": [int(sizes[s]) for s in picks]},
         "roofline": None,
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
 
 
@@ -713,6 +818,16 @@ def main() -> int:
         out["cpu_baseline"] = cb
         if "plain" in legs:
             out["plain_gpu_over_cpu"] = legs["plain"]["body_steps_per_s"] / cb["value"]
+        # the 16,384-body disk on the GPU (plain group Barnes-Hut, 100 steps)
+        m0, p0, v0 = ics.exp_disk(1 << 14, seed=SEED)
+        with nbody2d.Context(m0, p0, v0, G=1.0, dt=DT, theta=args.theta_plain, softening=EPS,
+                             traversal=abi.TRAV_GROUP) as c0:
+            c0.step(args.warmup)
+            t0 = time.perf_counter()
+            c0.step(100)
+            g0 = (1 << 14) * 100 / (time.perf_counter() - t0)
+        cb["exp_disk_16k"]["gpu_value"] = g0
+        cb["exp_disk_16k"]["gpu_over_cpu"] = g0 / cb["exp_disk_16k"]["value"]
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -744,28 +744,27 @@ int nb_get_state(nb_ctx* c, double* pos_xy, double* vel_xy) {
     if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
     const uint64_t n = c->n;
     if (n == 0) return NB_OK;
-    std::vector<double> p(pos_xy ? 2 * n : 0), v(vel_xy ? 2 * n : 0);
-    std::vector<uint32_t> o(n);
-    if (pos_xy)
-        NB_HIP(c, hipMemcpyAsync(p.data(), c->b.pos[c->cur], n * sizeof(double2),
-                                 hipMemcpyDeviceToHost, c->stream));
-    if (vel_xy)
-        NB_HIP(c, hipMemcpyAsync(v.data(), c->b.vel[c->cur], n * sizeof(double2),
-                                 hipMemcpyDeviceToHost, c->stream));
-    NB_HIP(c, hipMemcpyAsync(o.data(), c->b.orig[c->cur], n * sizeof(uint32_t),
-                             hipMemcpyDeviceToHost, c->stream));
-    NB_HIP(c, hipStreamSynchronize(c->stream));
-    for (uint64_t i = 0; i < n; i++) {
-        const uint64_t j = o[i];
-        if (pos_xy) {
-            pos_xy[2 * j] = p[2 * i];
-            pos_xy[2 * j + 1] = p[2 * i + 1];
-        }
-        if (vel_xy) {
-            vel_xy[2 * j] = v[2 * i];
-            vel_xy[2 * j + 1] = v[2 * i + 1];
-        }
+    const double2* ps = c->b.pos[c->cur];
+    const double2* vs = c->b.vel[c->cur];
+    if (is_bh(c)) {
+        // Morton order on the device: un-permute on the GPU into the idle copy
+        // of the body arrays (the next step's gather overwrites it anyway),
+        // then copy straight into the caller's arrays
+        double2* pt = c->b.pos[c->cur ^ 1];
+        double2* vt = c->b.vel[c->cur ^ 1];
+        launch_unpermute2(c, pt, vt);
+        NB_HIP(c, hipGetLastError());
+        ps = pt;
+        vs = vt;
     }
+    // (all-pairs contexts never reorder their bodies)
+    if (pos_xy)
+        NB_HIP(c, hipMemcpyAsync(pos_xy, ps, n * sizeof(double2), hipMemcpyDeviceToHost,
+                                 c->stream));
+    if (vel_xy)
+        NB_HIP(c, hipMemcpyAsync(vel_xy, vs, n * sizeof(double2), hipMemcpyDeviceToHost,
+                                 c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
     return NB_OK;
 }
 

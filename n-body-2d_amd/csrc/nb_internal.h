@@ -233,6 +233,7 @@ void launch_morton(nb_ctx* c, int src);
 void launch_radix_sort(nb_ctx* c);
 void launch_gather(nb_ctx* c, int src);
 void launch_unpermute(nb_ctx* c, double2* out);
+void launch_unpermute2(nb_ctx* c, double2* pos_out, double2* vel_out);
 void launch_scan_u32(nb_ctx* c, uint32_t* data, uint32_t count, uint32_t* total_out);
 
 // ---- launchers (nb_tree.hip) ----

@@ -937,6 +937,27 @@ void launch_unpermute(nb_ctx* c, double2* out) {
                        c->b.orig[c->cur], c->b.pos[c->cur], c->b.vel[c->cur], out, n);
 }
 
+// The same into two arrays (positions, velocities): the blocking readback
+// un-permutes into the idle copy of the body arrays.
+__global__ __launch_bounds__(256) void k_unpermute2(const uint32_t* __restrict__ orig,
+                                                    const double2* __restrict__ pos,
+                                                    const double2* __restrict__ vel,
+                                                    double2* __restrict__ pout,
+                                                    double2* __restrict__ vout, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = orig[i];
+    pout[j] = pos[i];
+    vout[j] = vel[i];
+}
+
+void launch_unpermute2(nb_ctx* c, double2* pos_out, double2* vel_out) {
+    const uint32_t n = (uint32_t)c->n;
+    hipLaunchKernelGGL(k_unpermute2, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                       c->b.orig[c->cur], c->b.pos[c->cur], c->b.vel[c->cur], pos_out, vel_out,
+                       n);
+}
+
 // ---------------------------------------------------------------------------
 // Exclusive in-place scan of u32 (reduce / top-level scan / downsweep).
 // ---------------------------------------------------------------------------

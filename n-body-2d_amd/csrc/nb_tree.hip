@@ -443,7 +443,7 @@ __global__ __launch_bounds__(256) void k_level_scatter(const uint32_t* __restric
 
 // The internal nodes of one level: segment [lvl_off[L * nblk], lvl_off[(L+1) * nblk])
 // of by_level (the last level ends at I). REF: the reference CPU order.
-template <bool REF>
+template <bool REF, bool M3>
 __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __restrict__ meta,
                                                    uint64_t icap, uint32_t nblk, uint32_t nsys,
                                                    const uint32_t* __restrict__ lvl_off,
@@ -457,20 +457,19 @@ __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __rest
     for (uint32_t k = b + blockIdx.x * 256 + threadIdx.x; k < e; k += gridDim.x * 256) {
         const uint4 en = by_level[k];  // node id, own slot, occupancy mask
         const uint32_t p = en.x, mask = en.z;
-        // an empty slot holds exact zeros: skip its load, combine() adds the same zeros
+        // an empty slot holds exact zeros: skip its load, combine() adds the same
+        // zeros; the third moments are loaded with the rest (one round trip)
         Com ch[4];
+        float4 t[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            ch[q] = (mask >> q) & 1u ? load_com(&nodes[nsys + 4ull * p + q]) : Com{0.0, 0.0, 0.0, 0.f, 0.f, 0.f};
+        for (int q = 0; q < 4; q++) {
+            const bool has = (mask >> q) & 1u;
+            ch[q] = has ? load_com(&nodes[nsys + 4ull * p + q]) : Com{0.0, 0.0, 0.0, 0.f, 0.f, 0.f};
+            if (M3) t[q] = has ? mom3[nsys + 4ull * p + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
         const Com o = REF ? combine_ref(ch) : combine(ch);
         store_com(&nodes[en.y], o, mask);
-        if (mom3) {
-            float4 t[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                t[q] = (mask >> q) & 1u ? mom3[nsys + 4ull * p + q] : make_float4(0.f, 0.f, 0.f, 0.f);
-            mom3[en.y] = combine3(ch, t, o);
-        }
+        if (M3) mom3[en.y] = combine3(ch, t, o);
     }
 }
 
@@ -506,16 +505,19 @@ void launch_tree_build(nb_ctx* c, int src) {
     // when its launch starts).
     const uint32_t cg = ng < 512 ? ng : 512;
     const bool ref = c->p.cells == NB_CELLS_REF_CPU;
+    const bool m3 = c->b.mom3 != nullptr;
+#define NB_COM(R, M)                                                                           \
+    hipLaunchKernelGGL((k_com_level<R, M>), dim3(cg), dim3(256), 0, c->stream, L, c->b.meta,   \
+                       (uint64_t)c->icap, ng, c->nsys, c->b.lvl_cnt, c->b.by_level,            \
+                       c->b.nodes, c->b.mom3)
     for (int L = kMaxLevel - 1; L >= 0; L--) {
-        if (ref)
-            hipLaunchKernelGGL(k_com_level<true>, dim3(cg), dim3(256), 0, c->stream, L,
-                               c->b.meta, (uint64_t)c->icap, ng, c->nsys, c->b.lvl_cnt,
-                               c->b.by_level, c->b.nodes, c->b.mom3);
-        else
-            hipLaunchKernelGGL(k_com_level<false>, dim3(cg), dim3(256), 0, c->stream, L,
-                               c->b.meta, (uint64_t)c->icap, ng, c->nsys, c->b.lvl_cnt,
-                               c->b.by_level, c->b.nodes, c->b.mom3);
+        if (ref) {
+            if (m3) NB_COM(true, true); else NB_COM(true, false);
+        } else {
+            if (m3) NB_COM(false, true); else NB_COM(false, false);
+        }
     }
+#undef NB_COM
 }
 
 }  // namespace nb

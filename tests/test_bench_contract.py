@@ -73,3 +73,17 @@ def test_equal_error_interpolation():
     # outside the measured range: no claim
     e = bench.equal_error(pts, {"force_error": {"rms": 1e-3}, "ms_per_step": 1.0})
     assert e["learned_over_plain"] is None
+
+
+def test_cpu_baseline_helpers():
+    """The CPU figures name the host and carry their units; the all-pairs rate
+    is that of the reference's single-threaded step on a bounded sample."""
+    h = bench.host_cpu()
+    assert h["host_logical_cpus"] >= 1 and h["allowed_cpus"] >= 1
+    assert 1 <= bench.cpu_threads() <= 256
+    sys.path.insert(0, str(REPO / "n-body-2d_amd"))
+    from nbody2d import ics
+
+    m, p, v = ics.uniform_disk(512, seed=2)
+    rate, k, secs = bench.cpu_allpairs_rate(m, p, v, n_sample=256)
+    assert k == 256 and secs > 0 and rate == pytest.approx(256 * 255 / secs)
