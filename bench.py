@@ -179,16 +179,54 @@ def cpu_allpairs_rate(mass, pos, vel, n_sample: int = 16384):
     return float(k) * (k - 1) / secs, k, secs
 
 
+_TUNED = {}
+
+
+def tune_walk(nbody2d, mass, pos, vel, theta, learned, args) -> dict:
+    """The group walk's settings for one configuration, picked by nb_autotune
+    on a scratch context a few steps evolved (the walk of the initial state is
+    not that of later steps); {} with --no-tune. The accepted cells, and so the
+    timing, do not depend on the network's weights, so the scratch context
+    keeps zero weights."""
+    if args.no_tune:
+        return {}
+    key = (theta, learned)
+    if key not in _TUNED:
+        abi = nbody2d.abi
+        with nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, theta=theta, softening=EPS,
+                             traversal=abi.TRAV_GROUP, self_guard=int(theta > 0.7071),
+                             far_field=abi.FAR_LEARNED if learned else abi.FAR_MONOPOLE) as c:
+            c.step(args.warmup + 2)
+            t = c.autotune(3)
+        _TUNED[key] = {"walk_bodies": t.walk_bodies, "walk_stack": t.walk_stack,
+                       "list_flush": t.list_flush, "group_budget": t.group_budget,
+                       "settings_timed": t.candidates, "force_ms": t.best_ms,
+                       "default_force_ms": t.default_ms}
+        print(f"[bench] tuned theta={theta} learned={learned}: bodies {t.walk_bodies} "
+              f"stack {t.walk_stack} flush {t.list_flush} budget {t.group_budget} "
+              f"({t.best_ms:.3f} vs {t.default_ms:.3f} ms per force evaluation)", flush=True)
+    return _TUNED[key]
+
+
+def walk_params(tuned: dict) -> dict:
+    return {k: tuned[k] for k in ("walk_bodies", "walk_stack", "list_flush", "group_budget")
+            if k in tuned}
+
+
 def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
     abi = nbody2d.abi
     n = len(mass)
     guard = 1 if theta > 0.7071 else 0
+    tuned = tune_walk(nbody2d, mass, pos, vel, theta, learned, args) \
+        if traversal == abi.TRAV_GROUP else {}
     ctx = nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, theta=theta, softening=EPS,
                           traversal=traversal, self_guard=guard,
-                          far_field=abi.FAR_LEARNED if learned else abi.FAR_MONOPOLE)
+                          far_field=abi.FAR_LEARNED if learned else abi.FAR_MONOPOLE,
+                          **walk_params(tuned))
     leg = {"theta": theta, "self_guard": guard,
            "far_field": "learned" if learned else "monopole",
-           "traversal": "group" if traversal == abi.TRAV_GROUP else "exact"}
+           "traversal": "group" if traversal == abi.TRAV_GROUP else "exact",
+           "autotune": tuned or None}
     if learned:
         t0 = time.perf_counter()
         ctx.mlp_init(1)
@@ -221,10 +259,6 @@ def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
         leg["far_field_cells"] = int(info.far_field_cells)
         leg["far_field_cells_per_group"] = info.far_field_cells / ((n + 63) // 64)
 
-    if args.tune_disk:
-        tn = ctx.autotune(2)
-        leg["autotune"] = {"group_budget": tn.group_budget, "settings": tn.candidates,
-                           "best_force_ms": tn.best_ms, "default_force_ms": tn.default_ms}
     ctx.step(args.warmup)
     t0 = time.perf_counter()
     ctx.step(args.steps)
@@ -274,8 +308,9 @@ def plain_point(nbody2d, ics, mass, pos, vel, theta, args) -> dict:
     and graph-replayed ms per step."""
     abi = nbody2d.abi
     n = len(mass)
+    tuned = tune_walk(nbody2d, mass, pos, vel, theta, False, args)
     with nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, theta=theta, softening=EPS,
-                         traversal=abi.TRAV_GROUP) as ctx:
+                         traversal=abi.TRAV_GROUP, **walk_params(tuned)) as ctx:
         t = ics.sample_targets(n, args.error_sample, seed=1)
         ref = nbody2d.direct_sum_f64(pos, mass, 1.0, EPS, t)
         acc, _, total = ctx.forces()
@@ -286,7 +321,7 @@ def plain_point(nbody2d, ics, mass, pos, vel, theta, args) -> dict:
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
     print(f"[bench] plain theta={theta}: {ms:.3f} ms/step, rms {rms:.3e}", flush=True)
     return {"theta": theta, "rms": rms, "max": mx, "ms_per_step": ms,
-            "interactions_per_body": total / n}
+            "interactions_per_body": total / n, "walk": walk_params(tuned) or None}
 
 
 def learned_point(nbody2d, ics, mass, pos, vel, theta, train, args) -> dict:
@@ -296,9 +331,10 @@ def learned_point(nbody2d, ics, mass, pos, vel, theta, train, args) -> dict:
     abi = nbody2d.abi
     n = len(mass)
     guard = 1 if theta > 0.7071 else 0
+    tuned = tune_walk(nbody2d, mass, pos, vel, theta, True, args)
     with nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, theta=theta, softening=EPS,
                          traversal=abi.TRAV_GROUP, self_guard=guard,
-                         far_field=abi.FAR_LEARNED) as ctx:
+                         far_field=abi.FAR_LEARNED, **walk_params(tuned)) as ctx:
         if train:
             ctx.mlp_init(1)
             ctx.mlp_generate(theta, TRAIN_SAMPLE_BODIES, TRAIN_PAIRS, seed=7)
@@ -316,7 +352,7 @@ def learned_point(nbody2d, ics, mass, pos, vel, theta, train, args) -> dict:
     print(f"[bench] learned theta={theta}, {what}: {ms:.3f} ms/step, rms {rms:.3e}", flush=True)
     return {"theta": theta, "network": "trained" if train else "zero", "rms": rms, "max": mx,
             "ms_per_step": ms, "force_error": {"rms": rms, "max": mx},
-            "interactions_per_body": total / n}
+            "interactions_per_body": total / n, "walk": walk_params(tuned) or None}
 
 
 def equal_error(points, learned) -> dict:
@@ -484,8 +520,9 @@ def run_two_disks(nbody2d, ics, args) -> dict:
     n = args.n if args.n != N_BODIES else 1 << 22
     mass, pos, vel = ics.two_disks(n, seed=SEED)
     abi = nbody2d.abi
+    tuned = tune_walk(nbody2d, mass, pos, vel, args.theta_plain, False, args)
     ctx = nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, theta=args.theta_plain, softening=EPS,
-                          traversal=abi.TRAV_GROUP)
+                          traversal=abi.TRAV_GROUP, **walk_params(tuned))
     t = ics.sample_targets(n, args.error_sample, seed=1)
     ref = nbody2d.direct_sum_f64(pos, mass, 1.0, EPS, t)
     acc, _, total = ctx.forces()
@@ -527,6 +564,7 @@ def run_two_disks(nbody2d, ics, args) -> dict:
         "data": "synthetic",
         "config": {"workload": f"two_disks N={n}, theta={args.theta_plain}, eps={EPS}, "
                                f"dt={DT}, plain group Barnes-Hut", "n_bodies": n},
+        "autotune": tuned or None,
         "force_error": {"rms": e0[0], "max": e0[1], "sample": int(len(t)), "state": "initial",
                         "reference": "GPU FP64 direct sum"},
         "force_error_after_steps": {"rms": e1[0], "max": e1[1],
@@ -582,14 +620,22 @@ def run_ensemble(nbody2d, ics, args) -> dict:
               far_field=abi.FAR_LEARNED)
     print(f"[bench] ensemble: {len(sizes)} systems, {n} bodies; network from {pairs} pairs "
           f"({train_s:.2f} s)", flush=True)
-    # batched
-    b = nbody2d.Context(mass, pos, vel, offsets=off, **kw)
-    b.mlp_set_weights(w)
+    # batched: the walk settings tuned on a scratch batch context a few steps
+    # evolved (tuned on the initial state, the list budget picked there once
+    # ran 6.6x slower per step after the systems evolved)
     tune = None
-    if args.tune_ensemble:
-        tn = b.autotune(2)
-        tune = {"group_budget": tn.group_budget, "settings": tn.candidates,
-                "best_force_ms": tn.best_ms, "default_force_ms": tn.default_ms}
+    if not args.no_tune:
+        with nbody2d.Context(mass, pos, vel, offsets=off, **kw) as sc:
+            sc.mlp_set_weights(w)
+            sc.step(args.warmup + 2)
+            tn = sc.autotune(3)
+        tune = {"walk_bodies": tn.walk_bodies, "walk_stack": tn.walk_stack,
+                "list_flush": tn.list_flush, "group_budget": tn.group_budget,
+                "settings_timed": tn.candidates, "force_ms": tn.best_ms,
+                "default_force_ms": tn.default_ms}
+        print(f"[bench] ensemble tuned: {tune}", flush=True)
+    b = nbody2d.Context(mass, pos, vel, offsets=off, **kw, **walk_params(tune or {}))
+    b.mlp_set_weights(w)
     # force error on 8 seeded systems against the FP64 direct sum, with the
     # trained network and with zero weights (the quadrupole term alone)
     fb, _, _ = b.forces()
@@ -698,14 +744,9 @@ def main() -> int:
                          "512 systems in one batch context vs looped single contexts")
     ap.add_argument("--systems", type=int, default=512, help="ensemble: number of systems")
     ap.add_argument("--no-tune", action="store_true",
-                    help="skip nb_autotune before timing the all-pairs workload")
-    ap.add_argument("--tune-ensemble", action="store_true",
-                    help="autotune the ensemble's group budget on its initial state (off by "
-                         "default: the budget it picks there, 4096, ran 6.6x slower per step "
-                         "once the systems had evolved)")
-    ap.add_argument("--tune-disk", action="store_true",
-                    help="also autotune the disk legs (tuned on the initial state only, "
-                         "which the exploding disk soon leaves)")
+                    help="skip nb_autotune (the disk legs and plain points are tuned on a "
+                         "scratch context a few steps evolved; the all-pairs workload on its "
+                         "state)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run a device copy of this many bytes first (counter calibration)")
     args = ap.parse_args()

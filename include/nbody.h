@@ -121,6 +121,16 @@ typedef struct nb_params {
     int32_t force_kernel; /* enum nb_force_kernel */
     int32_t mac;          /* enum nb_mac */
     int32_t cells;        /* enum nb_cells */
+    int32_t walk_stack;   /* GROUP: LDS stack entries of a traversal wavefront,
+                             1024 or 2048 (0 = 2048); fewer let more wavefronts
+                             share a CU, more let a wavefront take up to 64
+                             cells per step at any depth */
+    int32_t walk_bodies;  /* GROUP: bodies per traversal wavefront, 64 (one per
+                             lane) or 32 (two lanes per body, each summing half
+                             of the interaction list; smaller groups accept
+                             cells sooner) (0 = 64) */
+    int32_t list_flush;   /* GROUP: interaction-list entries held in LDS before
+                             they are summed, 64 .. 192 (0 = 128) */
     int32_t reserved;
 } nb_params;
 
@@ -283,21 +293,28 @@ int nb_direct_sum_f64_noalloc(uint64_t n, const double* pos_xy, const double* ma
                               double* acc_xy, void* stream);
 
 /* Autotuner: times one force evaluation of the current state (sort, tree and
- * traversal, or the all-pairs kernel; best of `repeats`) for each setting of
- * the context's launch tunables and keeps the fastest -- GROUP traversal:
- * the list budget after which a traversal group is split (1,024 .. 65,536);
- * all-pairs: bodies per thread (2, 4, 8) x the split of the j range (1 ..
- * 64). The state is not advanced; the captured step graphs are rebuilt. No
- * counterpart in the reference. The tree's stack depth (2,048 LDS entries,
- * above the deepest walk) and the 64 bodies per wavefront are fixed by the
- * kernels' design and not tuned. */
+ * traversal [+ far field], or the all-pairs kernel; mean of `repeats`) for
+ * each setting of the context's launch tunables and keeps the fastest -- a
+ * setting replaces the current one only when it is at least 3 % faster. GROUP
+ * traversal, one tunable at a time in this order: bodies per wavefront (64,
+ * 32), LDS stack depth (2048, 1024), list flush size (64, 128, 192); the list
+ * budget (nb_params.group_budget) is left as set, since the state it is timed
+ * on does not predict the steps that follow. All-pairs: bodies per thread
+ * (2, 4, 8) x the split of the j range (1 .. 64). The state is not advanced;
+ * the captured step graphs are rebuilt. Tune a state a few steps evolved:
+ * the walk of an initial state is not that of later steps. No counterpart in
+ * the reference. */
 typedef struct nb_tuning {
-    int32_t group_budget; /* chosen list budget (GROUP traversal) */
+    int32_t group_budget; /* list budget in use (GROUP traversal; not tuned) */
     int32_t ap_rows;      /* chosen all-pairs bodies per thread */
     int32_t ap_split;     /* chosen all-pairs j split */
     int32_t candidates;   /* settings timed */
-    double best_ms;       /* force evaluation with the chosen setting */
+    double best_ms;       /* force evaluation with the chosen setting (mean) */
     double default_ms;    /* force evaluation with the setting before tuning */
+    int32_t walk_bodies;  /* chosen bodies per traversal wavefront */
+    int32_t walk_stack;   /* chosen LDS stack entries */
+    int32_t list_flush;   /* chosen list flush size */
+    int32_t reserved;
 } nb_tuning;
 
 int nb_autotune(nb_ctx* ctx, uint32_t repeats, nb_tuning* out);

@@ -200,6 +200,12 @@ int check_params(const nb_params* p, std::string* why) {
         return *why = "unknown traversal", NB_ERR_INVALID_ARG;
     if (p->group_budget < 0) return *why = "group_budget must be >= 0", NB_ERR_INVALID_ARG;
     if (p->cell_capacity < 0) return *why = "cell_capacity must be >= 0", NB_ERR_INVALID_ARG;
+    if (p->walk_stack != 0 && p->walk_stack != 1024 && p->walk_stack != 2048)
+        return *why = "walk_stack must be 0, 1024 or 2048", NB_ERR_INVALID_ARG;
+    if (p->walk_bodies != 0 && p->walk_bodies != 32 && p->walk_bodies != 64)
+        return *why = "walk_bodies must be 0, 32 or 64", NB_ERR_INVALID_ARG;
+    if (p->list_flush != 0 && (p->list_flush < 64 || p->list_flush > 192))
+        return *why = "list_flush must be 0 or in 64 .. 192", NB_ERR_INVALID_ARG;
     if (p->far_field != NB_FAR_MONOPOLE && p->far_field != NB_FAR_LEARNED)
         return *why = "unknown far_field", NB_ERR_INVALID_ARG;
     if (p->force_kernel != NB_FORCE_PLUMMER && p->force_kernel != NB_FORCE_REF_CPU)
@@ -451,6 +457,43 @@ int nb_params_default(nb_params* p) {
 
 namespace {
 
+// Traversal groups of walk_bodies(c) consecutive bodies, never across two
+// systems, and the group walk's work queue (zero between kernels).
+hipError_t alloc_groups(nb_ctx* c) {
+    Buffers& b = c->b;
+    std::vector<uint32_t> grp_first, grp_info;
+    const uint32_t bpw = (uint32_t)walk_bodies(c);
+    const auto& off = c->sys_off_h;
+    for (uint32_t s = 0; s + 1 < (uint32_t)off.size(); s++) {
+        for (uint32_t g = off[s]; g < off[s + 1]; g += bpw) {
+            grp_first.push_back(g);
+            grp_info.push_back((s << 8) | std::min<uint32_t>(bpw, off[s + 1] - g));
+        }
+    }
+    if (b.grp_first) c->device_bytes -= 2ull * c->ngroups * sizeof(uint32_t);
+    if (b.queue) c->device_bytes -= (uint64_t)c->queue_cap * sizeof(uint64_t);
+    dfree(b.grp_first);
+    dfree(b.grp_info);
+    dfree(b.queue);
+    c->ngroups = (uint32_t)grp_first.size();
+    hipError_t e = dalloc(c, &b.grp_first, c->ngroups);
+    if (!e) e = dalloc(c, &b.grp_info, c->ngroups);
+    if (!e && c->ngroups) {
+        e = hipMemcpy(b.grp_first, grp_first.data(), c->ngroups * sizeof(uint32_t),
+                      hipMemcpyHostToDevice);
+        if (!e)
+            e = hipMemcpy(b.grp_info, grp_info.data(), c->ngroups * sizeof(uint32_t),
+                          hipMemcpyHostToDevice);
+    }
+    if (!e && is_bh(c)) {
+        c->queue_cap = (uint32_t)std::min<uint64_t>(4ull * c->ngroups + 64, 0xffffffffull);
+        e = dalloc(c, &b.queue, c->queue_cap);
+        if (!e) e = hipMemsetAsync(b.queue, 0, c->queue_cap * sizeof(uint64_t), c->stream);
+        if (!e) e = hipStreamSynchronize(c->stream);
+    }
+    return e;
+}
+
 // One context for `nsys` independent systems whose bodies are the contiguous
 // ranges [offsets[s], offsets[s+1]) (nsys = 1: the whole array).
 int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* offsets,
@@ -544,26 +587,15 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
         NB_TRY(hipMemsetAsync(b.sort_status, 0, sort_words * sizeof(uint64_t), c->stream));
     NB_TRY(dalloc(c, &b.box, (uint64_t)kBoxStride * nsys));
     // systems and traversal groups (<= 64 bodies, never across two systems)
-    std::vector<uint32_t> sys_of(n), grp_first, grp_info;
-    for (uint32_t s = 0; s < nsys; s++) {
+    std::vector<uint32_t> sys_of(n);
+    for (uint32_t s = 0; s < nsys; s++)
         for (uint32_t i = off[s]; i < off[s + 1]; i++) sys_of[i] = s;
-        for (uint32_t g = off[s]; g < off[s + 1]; g += 64) {
-            grp_first.push_back(g);
-            grp_info.push_back((s << 8) | std::min<uint32_t>(64, off[s + 1] - g));
-        }
-    }
-    c->ngroups = (uint32_t)grp_first.size();
+    c->sys_off_h = off;
     NB_TRY(dalloc(c, &b.sys_of, n));
     NB_TRY(dalloc(c, &b.sys_off, nsys + 1));
-    NB_TRY(dalloc(c, &b.grp_first, c->ngroups));
-    NB_TRY(dalloc(c, &b.grp_info, c->ngroups));
-    if (n) {
+    if (n)
         NB_TRY(hipMemcpy(b.sys_of, sys_of.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-        NB_TRY(hipMemcpy(b.grp_first, grp_first.data(), c->ngroups * sizeof(uint32_t),
-                         hipMemcpyHostToDevice));
-        NB_TRY(hipMemcpy(b.grp_info, grp_info.data(), c->ngroups * sizeof(uint32_t),
-                         hipMemcpyHostToDevice));
-    }
+    NB_TRY(alloc_groups(c));
     NB_TRY(hipMemcpy(b.sys_off, off.data(), (nsys + 1) * sizeof(uint32_t),
                      hipMemcpyHostToDevice));
     NB_TRY(dalloc(c, &b.bbox_part, 4ull * c->bbox_blocks));
@@ -576,12 +608,6 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
     NB_TRY(dalloc(c, &b.cnt, bh ? n : 0));
     NB_TRY(dalloc(c, &b.scan_tmp, scan_blocks));
     NB_TRY(dalloc(c, &b.multi_list, bh ? n / 2 + 1 : 0));
-    if (bh) {
-        // work queue of the group traversal, zero between kernels
-        c->queue_cap = (uint32_t)std::min<uint64_t>(4ull * c->ngroups + 64, 0xffffffffull);
-        NB_TRY(dalloc(c, &b.queue, c->queue_cap));
-        NB_TRY(hipMemsetAsync(b.queue, 0, c->queue_cap * sizeof(uint64_t), c->stream));
-    }
     NB_TRY(dalloc(c, &b.acc, n));
     NB_TRY(dalloc(c, &b.inter, n));
     NB_TRY(dalloc(c, &b.meta, META_COUNT));
@@ -902,9 +928,9 @@ int nb_autotune(nb_ctx* c, uint32_t repeats, nb_tuning* out) {
     if (c->n == 0) return NB_OK;
     if (repeats == 0) repeats = 1;
     const bool timing = c->timing;
-    // Force evaluation time (ms, best of `repeats`) with the current settings.
+    // Force evaluation time (ms, mean of `repeats`) with the current settings.
     auto measure = [&](double* ms_out) -> int {
-        double best = 1e300;
+        double sum = 0.0;
         for (uint32_t r = 0; r < repeats;) {
             int rc = clear_meta(c);
             if (rc) return rc;
@@ -932,12 +958,16 @@ int nb_autotune(nb_ctx* c, uint32_t repeats, nb_tuning* out) {
                 NB_HIP(c, hipEventElapsedTime(&t, c->ev[2 * ph], c->ev[2 * ph + 1]));
                 ms += t;
             }
-            best = std::min(best, ms);
+            sum += ms;
             r++;
         }
-        *ms_out = best;
+        *ms_out = sum / repeats;
         return NB_OK;
     };
+    // A setting replaces the current one only when it is clearly faster:
+    // walks vary from step to step, and a setting that wins a close race on
+    // one state (a small list budget, say) can lose on the next steps.
+    constexpr double kClearly = 0.97;
     nb_tuning t{};
     double base = 0.0;
     int rc = measure(&base);
@@ -945,21 +975,44 @@ int nb_autotune(nb_ctx* c, uint32_t repeats, nb_tuning* out) {
     double best = base;
     t.candidates = 1;
     if (is_bh(c) && c->p.traversal == NB_TRAV_GROUP) {
-        const int32_t orig = c->p.group_budget;
-        int32_t pick = orig;
-        for (int32_t budget : {1024, 2048, 4096, 8192, 16384, 65536}) {
-            if (budget == (orig > 0 ? orig : kGroupBudget)) continue;
-            c->p.group_budget = budget;
-            double ms = 0.0;
-            rc = measure(&ms);
-            if (rc) return rc;
-            t.candidates++;
-            if (ms < best) {
-                best = ms;
-                pick = budget;
+        // One tunable at a time, each from the best setting so far. Changing
+        // the bodies per wavefront rebuilds the traversal groups (and the
+        // learned far field's per-group buffers).
+        auto set_bodies = [&](int32_t v) -> int {
+            const uint64_t old_groups = c->ngroups;
+            c->p.walk_bodies = v;
+            destroy_graphs(c);
+            NB_HIP(c, alloc_groups(c));
+            return regroup_cells(c, old_groups);
+        };
+        auto tune = [&](int32_t* field, std::initializer_list<int32_t> values, int32_t deflt,
+                        bool regroup) -> int {
+            const int32_t start = *field > 0 ? *field : deflt;
+            int32_t pick = start;
+            for (int32_t v : values) {
+                if (v == start) continue;
+                int rc2 = regroup ? set_bodies(v) : (*field = v, NB_OK);
+                if (rc2) return rc2;
+                double ms = 0.0;
+                rc2 = measure(&ms);
+                if (rc2) return rc2;
+                t.candidates++;
+                if (ms < best * kClearly) {
+                    best = ms;
+                    pick = v;
+                }
             }
-        }
-        c->p.group_budget = pick;
+            if (regroup && c->p.walk_bodies != pick) return set_bodies(pick);
+            *field = pick;
+            return NB_OK;
+        };
+        if ((rc = tune(&c->p.walk_bodies, {64, 32}, 64, true))) return rc;
+        if ((rc = tune(&c->p.walk_stack, {2048, 1024}, 2048, false))) return rc;
+        if ((rc = tune(&c->p.list_flush, {64, 128, 192}, 128, false))) return rc;
+        // The list budget is not tuned: its effect changes from step to step.
+        // On a 1M disk at theta 0.15, a budget of 2048 timed 5 % faster on
+        // the tuned state and ran 20 % slower over the next steps (every
+        // split restarts walks, and which groups split moves with the bodies).
     } else if (!is_bh(c)) {
         const uint32_t r0 = c->ap_rows, s0 = c->ap_split;
         uint32_t pr = r0, ps = s0;
@@ -972,7 +1025,7 @@ int nb_autotune(nb_ctx* c, uint32_t repeats, nb_tuning* out) {
                 rc = measure(&ms);
                 if (rc) return rc;
                 t.candidates++;
-                if (ms < best) {
+                if (ms < best * kClearly) {
                     best = ms;
                     pr = rows;
                     ps = split;
@@ -985,6 +1038,9 @@ int nb_autotune(nb_ctx* c, uint32_t repeats, nb_tuning* out) {
     rc = build_graphs(c);
     if (rc) return rc;
     t.group_budget = c->p.group_budget > 0 ? c->p.group_budget : kGroupBudget;
+    t.walk_bodies = walk_bodies(c);
+    t.walk_stack = walk_stack(c);
+    t.list_flush = list_flush(c);
     t.ap_rows = (int32_t)c->ap_rows;
     t.ap_split = (int32_t)c->ap_split;
     t.best_ms = best;

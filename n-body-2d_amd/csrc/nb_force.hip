@@ -207,10 +207,8 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
 // entries, because a cell at level L has at most 30 - L levels below it, so
 // the stack cannot overflow.
 // ---------------------------------------------------------------------------
-constexpr int kGStack = 2048;
 constexpr int kGReserve = 3 * kMaxLevel + 4;
-constexpr int kGList = 192;
-constexpr int kGFlush = kGList - 64;
+constexpr int kGList = 256;   // the list is flushed past `flush` (<= kGList - 64) entries
 // Before its walk a group is split at its largest Morton jump when the two
 // parts are much more compact than the whole. As a safety net, a group whose
 // interaction list still passes `budget` entries (default kGroupBudget), or
@@ -235,14 +233,15 @@ constexpr int kGFlush = kGList - 64;
 // instead left more stragglers unsplit (theta 0.3: 1.91 vs 1.76 ms).
 constexpr uint32_t kGMinSteps = 96;
 constexpr uint32_t kGWalkSteps = 20;
-constexpr uint32_t kGWavesPerCU = 13;  // LDS-bound residency of k_traverse_group
 
-template <bool ZERO_EPS, bool REFF>
-__device__ __forceinline__ void eval_list(const float4* __restrict__ list, int cnt, float bx,
-                                          float by, float eps2f, Acc& a) {
+// ES = 2 when two lanes share a body (32 bodies per wavefront): each sums
+// every other entry, from e0.
+template <bool ZERO_EPS, bool REFF, int ES>
+__device__ __forceinline__ void eval_list(const float4* __restrict__ list, int e0, int cnt,
+                                          float bx, float by, float eps2f, Acc& a) {
     float px = 0.f, py = 0.f;
 #pragma unroll 8
-    for (int e = 0; e < cnt; e++) {
+    for (int e = e0; e < cnt; e += ES) {
         const float4 d = list[e];
         const float dx = d.x - bx, dy = d.y - by;
         float w;
@@ -269,13 +268,13 @@ __device__ __forceinline__ void eval_list(const float4* __restrict__ list, int c
 // position to the monopole,
 //   a = m d / R^3 + (3 / (2 R^5)) [(5 (d.I d) / R^2 - tr I) d - 2 I d],
 // d = COM - body, R^2 = |d|^2 + eps^2: 36 flops per entry.
-template <bool ZERO_EPS>
+template <bool ZERO_EPS, int ES>
 __device__ __forceinline__ void eval_list_quad(const float4* __restrict__ list,
-                                               const float2* __restrict__ quad, int cnt,
+                                               const float2* __restrict__ quad, int e0, int cnt,
                                                float bx, float by, float eps2f, Acc& a) {
     float px = 0.f, py = 0.f;
 #pragma unroll 4
-    for (int e = 0; e < cnt; e++) {
+    for (int e = e0; e < cnt; e += ES) {
         const float4 d = list[e];
         const float2 q = quad[e];
         const float dx = d.x - bx, dy = d.y - by;
@@ -356,7 +355,10 @@ __device__ __forceinline__ bool queue_pop(uint32_t* meta, uint64_t* queue, uint3
 // per-group counter, each member its subgroup and its offset from that
 // centre, and the raw acceleration (without G) goes to acc_out instead of a
 // kick.
-template <bool KICK, bool ZERO_EPS, bool GUARD, bool LEARN, bool REFF>
+// STACK: LDS stack entries (fewer: more resident wavefronts, smaller takes
+// near the top of the stack). BPW: bodies per wavefront, 64 (one per lane) or
+// 32 (lanes l and l + 32 hold the same body and sum alternate list entries).
+template <bool KICK, bool ZERO_EPS, bool GUARD, bool LEARN, bool REFF, int STACK, int BPW>
 __global__ __launch_bounds__(64) void k_traverse_group(
     const NodeRec* __restrict__ nodes, const double2* __restrict__ pos,
     const double* __restrict__ mass, double2* __restrict__ vel, double2* __restrict__ acc_out,
@@ -365,14 +367,19 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     const uint64_t* __restrict__ keys, uint32_t* __restrict__ cell_slot,
     uint32_t* __restrict__ cell_cnt, uint32_t cell_cap, SubRec* __restrict__ subs,
     float4* __restrict__ body_w, uint64_t* __restrict__ queue, uint32_t queue_cap,
-    const uint32_t* __restrict__ grp_first, const uint32_t* __restrict__ grp_info) {
+    const uint32_t* __restrict__ grp_first, const uint32_t* __restrict__ grp_info, int flush) {
+    static_assert(BPW == 64 || BPW == 32, "64 or 32 bodies per wavefront");
+    static_assert(STACK >= kGReserve + 3 * 64, "stack too small for a full take");
+    constexpr int ES = 64 / BPW;  // lanes per body
     if (meta[META_FLAGS] & kHaltFlags) return;
-    __shared__ __attribute__((aligned(16))) int32_t s_stack[kGStack];
+    __shared__ __attribute__((aligned(16))) int32_t s_stack[STACK];
     __shared__ float4 s_list[kGList];
     __shared__ float2 s_quad[LEARN ? kGList : 1];  // LEARN: ixy, iyy of each entry
     __shared__ int32_t s_todo[16];
-    __shared__ unsigned long long s_own;  // lanes whose own single-body leaf is listed
+    __shared__ unsigned long long s_own;  // bodies whose own single-body leaf is listed
     const int l = lane_id();
+    const int bl = l & (BPW - 1);  // the lane's body in the group
+    const int part = l / BPW;      // which of the body's lanes (BPW = 32)
     const uint32_t home = xcd_remap(blockIdx.x, gridDim.x);
     if (count_step && blockIdx.x == 0 && l == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
     const double theta_sq = k->theta_sq;
@@ -386,24 +393,25 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     // Pending subgroups of group g as (first lane << 8 | lane count); a
     // single body is never split (its group test is its own opening test).
     int nt = 1;
-    if (l == 0) s_todo[0] = 64;
+    if (l == 0) s_todo[0] = BPW;
     for (;;) {
         // group g: bodies first .. first + gn - 1 of system gsys
         const uint32_t first = grp_first[g], ginfo = grp_info[g];
         const int gn = (int)(ginfo & 255u);
         const int32_t root = (int32_t)(ginfo >> 8);
-        const uint32_t i = first + l;
-        const bool valid = l < gn;
+        const uint32_t i = first + bl;
+        const bool valid = bl < gn;
         const double2 p = valid ? pos[i] : pos[first];
         const int32_t me = (int32_t)i;
-        // Morton jump to the next body: digits shared with it (64 = none follows).
+        // Morton jump to the next body: digits shared with it (64 = none
+        // follows); the second lane of a body (BPW = 32) reports none.
         const uint64_t key = valid ? keys[i] : 0ull;
         const uint64_t key_next = __shfl_down(key, 1, 64);
-        const int jump = (valid && l + 1 < gn) ? __clzll(key ^ key_next) : 64;
+        const int jump = (valid && bl + 1 < gn && part == 0) ? __clzll(key ^ key_next) : 64;
         while (nt > 0 && !overflow) {
             const int32_t e = readfirst(s_todo[--nt]);
             const int f = e >> 8, cn = e & 255;
-            const bool member = valid && l >= f && l < f + cn;
+            const bool member = valid && bl >= f && bl < f + cn;
             if (ballot64(member) == 0) continue;
             double xmin = member ? p.x : INFINITY, xmax = member ? p.x : -INFINITY;
             double ymin = member ? p.y : INFINITY, ymax = member ? p.y : -INFINITY;
@@ -418,14 +426,15 @@ __global__ __launch_bounds__(64) void k_traverse_group(
             // parts are far more compact than the whole (the group straddles
             // a jump of the curve); no walk is wasted on the wide box.
             if (cn > 1) {
-                const int my = (member && l < f + cn - 1) ? jump : 65;
+                const int my = (member && bl < f + cn - 1) ? jump : 65;
                 int best = my;
 #pragma unroll
                 for (int d = 32; d >= 1; d >>= 1) best = min(best, __shfl_xor(best, d, 64));
+                // (jumps are reported by lanes 0..BPW-1 only: lane = body)
                 const uint64_t at = ballot64(member && my == best);
-                const int cut = __ffsll((unsigned long long)at);  // first lane after the jump
+                const int cut = __ffsll((unsigned long long)at);  // first body after the jump
                 if (best < 64 && cut > f && cut < f + cn) {
-                    const bool left = l < cut;
+                    const bool left = bl < cut;
                     double ax0 = member && left ? p.x : INFINITY, ax1 = member && left ? p.x : -INFINITY;
                     double ay0 = member && left ? p.y : INFINITY, ay1 = member && left ? p.y : -INFINITY;
                     double bx0 = member && !left ? p.x : INFINITY, bx1 = member && !left ? p.x : -INFINITY;
@@ -474,8 +483,8 @@ __global__ __launch_bounds__(64) void k_traverse_group(
             int32_t node = 0;
             NodeRec r;
             auto pop = [&]() -> bool {  // false: the stack would overflow
-                const int take = max(1, min(min(64, sp), (kGStack - kGReserve - sp) / 3));
-                if (sp + 3 > kGStack) return false;  // unreachable: see kGReserve
+                const int take = max(1, min(min(64, sp), (STACK - kGReserve - sp) / 3));
+                if (sp + 3 > STACK) return false;  // unreachable: see kGReserve
                 const int base = sp - take;
                 have = l < take;
                 node = have ? s_stack[base + l] : 0;
@@ -546,7 +555,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                     const int src = __ffsll((unsigned long long)mm) - 1;
                     mm &= mm - 1;
                     const int32_t js = __shfl(bs, src, 64), je = __shfl(be, src, 64);
-                    for (int32_t j = js; j <= je; j++) {
+                    for (int32_t j = js + part; j <= je; j += ES) {
                         if (j == me) continue;
                         const double2 q = pos[j];
                         const double ex = q.x - p.x, ey = q.y - p.y;
@@ -563,11 +572,11 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                     overflow = true;
                     break;
                 }
-                if (cnt > kGFlush) {
+                if (cnt > flush) {
                     if (LEARN)
-                        eval_list_quad<ZERO_EPS>(s_list, s_quad, cnt, bx, by, eps2f, t);
+                        eval_list_quad<ZERO_EPS, ES>(s_list, s_quad, part, cnt, bx, by, eps2f, t);
                     else
-                        eval_list<ZERO_EPS, REFF>(s_list, cnt, bx, by, eps2f, t);
+                        eval_list<ZERO_EPS, REFF, ES>(s_list, part, cnt, bx, by, eps2f, t);
                     listed += cnt;
                     cnt = 0;
                 }
@@ -595,9 +604,9 @@ __global__ __launch_bounds__(64) void k_traverse_group(
             }
             if (cnt > 0) {
                 if (LEARN)
-                    eval_list_quad<ZERO_EPS>(s_list, s_quad, cnt, bx, by, eps2f, t);
+                    eval_list_quad<ZERO_EPS, ES>(s_list, s_quad, part, cnt, bx, by, eps2f, t);
                 else
-                    eval_list<ZERO_EPS, REFF>(s_list, cnt, bx, by, eps2f, t);
+                    eval_list<ZERO_EPS, REFF, ES>(s_list, part, cnt, bx, by, eps2f, t);
                 listed += cnt;
             }
             // the subgroup is finished: its members write their results
@@ -605,7 +614,8 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                 uint32_t ks = 0;
                 if (l == 0) ks = atomicAdd(&cell_cnt[g], 1u);
                 ks = __shfl(ks, 0, 64) & 63u;
-                if (member) body_w[i] = make_float4(bx, by, __int_as_float((int)ks), 0.f);
+                if (member && part == 0)
+                    body_w[i] = make_float4(bx, by, __int_as_float((int)ks), 0.f);
                 if (l == 0) {
                     SubRec sr;
                     sr.ox = ox;
@@ -616,11 +626,17 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                 }
                 cbase += ccnt;
             }
-            if (member) {
-                const double ax = (double)t.ax - (double)t.cx;
-                const double ay = (double)t.ay - (double)t.cy;
+            double ax = (double)t.ax - (double)t.cx;
+            double ay = (double)t.ay - (double)t.cy;
+            uint32_t tn = t.n;
+            if (BPW == 32) {  // the body's two lanes summed alternate entries
+                ax += __shfl_xor(ax, 32, 64);
+                ay += __shfl_xor(ay, 32, 64);
+                tn += (uint32_t)__shfl_xor((int)tn, 32, 64);
+            }
+            if (member && part == 0) {
                 // the body's own single-body leaf, when listed, is a zero term
-                const uint32_t terms = listed - (uint32_t)((s_own >> l) & 1ull) + t.n;
+                const uint32_t terms = listed - (uint32_t)((s_own >> bl) & 1ull) + tn;
                 if (LEARN) {
                     acc_out[i] = make_double2(ax, ay);
                     if (!KICK) inter_out[i] = terms;
@@ -653,6 +669,27 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     }
 }
 
+// The group walk with the context's walk settings (stack depth, bodies per
+// wavefront; 0 = the defaults).
+template <bool K, bool Z, bool GD, bool LN, bool RF>
+void launch_walk(nb_ctx* c, int src, int count, int budget, uint32_t max_steps, int flush) {
+    const uint32_t n = (uint32_t)c->n;
+    const int stack = walk_stack(c), bpw = walk_bodies(c);
+#define NB_WALK(S, B)                                                                          \
+    hipLaunchKernelGGL((k_traverse_group<K, Z, GD, LN, RF, S, B>), dim3(c->ngroups), dim3(64), \
+                       0, c->stream, c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src], \
+                       c->b.acc, c->b.inter, n, c->b.consts, c->b.meta, count, budget,         \
+                       max_steps, c->b.keys[0], c->b.cell_slot, c->b.cell_cnt, c->cell_cap,    \
+                       c->b.subs, c->b.body_w, c->b.queue, c->queue_cap, c->b.grp_first,       \
+                       c->b.grp_info, flush)
+    if (stack == 1024) {
+        if (bpw == 32) NB_WALK(1024, 32); else NB_WALK(1024, 64);
+    } else {
+        if (bpw == 32) NB_WALK(2048, 32); else NB_WALK(2048, 64);
+    }
+#undef NB_WALK
+}
+
 void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
     const uint32_t n = (uint32_t)c->n;
     if (n == 0) return;
@@ -664,19 +701,18 @@ void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step) {
         const double grow = th >= 0.5 ? 1.0 : th > 1e-3 ? 0.5 / th : 1e6;
         const int budget = (int)std::min(
             (double)(c->p.group_budget > 0 ? c->p.group_budget : kGroupBudget) * grow, 1e9);
-        const uint64_t rounds = ((uint64_t)groups + (uint64_t)c->cus * kGWavesPerCU - 1) /
-                                ((uint64_t)c->cus * kGWavesPerCU);
+        // resident wavefronts per CU: the LDS stack and list bound them
+        const bool learn = c->p.far_field == NB_FAR_LEARNED;
+        const uint32_t lds = (uint32_t)walk_stack(c) * 4 + kGList * (learn ? 24 : 16) + 160;
+        const uint64_t per_cu = std::max<uint32_t>(1, (160u << 10) / lds);
+        const uint64_t rounds = ((uint64_t)groups + (uint64_t)c->cus * per_cu - 1) /
+                                ((uint64_t)c->cus * per_cu);
         const uint32_t max_steps = (uint32_t)std::min(
             std::max((double)kGMinSteps * grow, (double)(rounds * kGWalkSteps)), 1e9);
-        const bool learn = c->p.far_field == NB_FAR_LEARNED;
+        const int flush = list_flush(c);
         const bool reff = c->p.force_kernel == NB_FORCE_REF_CPU;
-#define NB_TRAVG(K, Z, GD, LN, RF)                                                             \
-    hipLaunchKernelGGL((k_traverse_group<K, Z, GD, LN, RF>), dim3(groups), dim3(64), 0,        \
-                       c->stream, c->b.nodes, c->b.pos[src], c->b.mass[src], c->b.vel[src],    \
-                       c->b.acc, c->b.inter, n, c->b.consts, c->b.meta,                        \
-                       (int)(count_step && !learn), budget, max_steps, c->b.keys[0],           \
-                       c->b.cell_slot, c->b.cell_cnt, c->cell_cap, c->b.subs, c->b.body_w,     \
-                       c->b.queue, c->queue_cap, c->b.grp_first, c->b.grp_info)
+        const int count = (int)(count_step && !learn);
+#define NB_TRAVG(K, Z, GD, LN, RF) launch_walk<K, Z, GD, LN, RF>(c, src, count, budget, max_steps, flush)
 #define NB_TRAVG_GD(K, Z, LN, RF)                                                              \
     {                                                                                          \
         if (gd)                                                                                \

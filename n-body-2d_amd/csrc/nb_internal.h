@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/nbody.h"
 #include "nb_mlp.h"
@@ -221,11 +222,17 @@ struct nb_ctx {
     bool dead = false;
     std::string dead_msg;
     bool keep_mom3 = false;         // the tree keeps third moments (b.mom3)
+    std::vector<uint32_t> sys_off_h;  // first body of each system, and n
     uint32_t fail_allocs = 0;       // test hook: allocations left to fail
     std::string err;
 };
 
 namespace nb {
+
+// Group walk settings of a context (nb_params; 0 = the default).
+inline int walk_stack(const nb_ctx* c) { return c->p.walk_stack > 0 ? c->p.walk_stack : 2048; }
+inline int walk_bodies(const nb_ctx* c) { return c->p.walk_bodies > 0 ? c->p.walk_bodies : 64; }
+inline int list_flush(const nb_ctx* c) { return c->p.list_flush > 0 ? c->p.list_flush : 128; }
 
 // ---- launchers (nb_sort.hip) ----
 void launch_bbox(nb_ctx* c, int src);
@@ -247,6 +254,7 @@ void launch_drift(nb_ctx* c, int src, bool check, bool count_step);
 void free_mlp(nb_ctx* c);
 int prepare_mlp(nb_ctx* c);
 int alloc_cells(nb_ctx* c, uint32_t cap);
+int regroup_cells(nb_ctx* c, uint64_t old_groups);
 void launch_mlp_apply(nb_ctx* c, int src, bool kick, bool count_step);
 // Builds the tree of the current state (nb_capi.hip).
 int build_tree(nb_ctx* c);

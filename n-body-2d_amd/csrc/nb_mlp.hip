@@ -678,6 +678,25 @@ int alloc_cells(nb_ctx* c, uint32_t cap) {
     return NB_OK;
 }
 
+// The traversal groups changed (nb_autotune): the per-group cell regions,
+// subgroup records and counters are allocated again for the new groups.
+int regroup_cells(nb_ctx* c, uint64_t old_groups) {
+    Buffers& b = c->b;
+    if (!b.cell_cnt) return NB_OK;  // no learned far field
+    c->device_bytes -= old_groups * (sizeof(uint32_t) + 64 * sizeof(SubRec) +
+                                     (uint64_t)c->cell_cap * sizeof(uint32_t)) +
+                       c->n * sizeof(float4);
+    for (void* q : {(void*)b.cell_slot, (void*)b.cell_cnt, (void*)b.subs, (void*)b.body_w})
+        (void)hipFree(q);
+    b.cell_slot = nullptr;
+    b.cell_cnt = nullptr;
+    b.subs = nullptr;
+    b.body_w = nullptr;
+    const uint32_t cap = c->cell_cap;
+    c->cell_cap = 0;
+    return alloc_cells(c, cap);
+}
+
 }  // namespace nb
 
 extern "C" {

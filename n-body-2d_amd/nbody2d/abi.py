@@ -60,6 +60,7 @@ class Params(C.Structure):
         ("far_field", C.c_int32), ("use_graph", C.c_int32), ("traversal", C.c_int32),
         ("group_budget", C.c_int32), ("cell_capacity", C.c_int32),
         ("force_kernel", C.c_int32), ("mac", C.c_int32), ("cells", C.c_int32),
+        ("walk_stack", C.c_int32), ("walk_bodies", C.c_int32), ("list_flush", C.c_int32),
         ("reserved", C.c_int32),
     ]
 
@@ -77,6 +78,8 @@ class Tuning(C.Structure):
     _fields_ = [
         ("group_budget", C.c_int32), ("ap_rows", C.c_int32), ("ap_split", C.c_int32),
         ("candidates", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double),
+        ("walk_bodies", C.c_int32), ("walk_stack", C.c_int32), ("list_flush", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

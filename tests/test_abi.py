@@ -29,8 +29,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layouts_match_header():
-    assert C.sizeof(abi.Params) == 4 * 8 + 12 * 4
+    assert C.sizeof(abi.Params) == 4 * 8 + 16 * 4  # 15 int32 fields + padding to 8
     assert C.sizeof(abi.Info) == 8 * 8
+    assert C.sizeof(abi.Tuning) == 4 * 4 + 2 * 8 + 4 * 4
     assert abi.NODE_DTYPE.itemsize == 4 * 4 + 4 * 8 + 4 * 4 + 4 * 4
 
 
@@ -44,7 +45,9 @@ def test_params_default():
 @pytest.mark.parametrize("field,value", [("dt", 0.0), ("dt", float("nan")), ("theta", -1.0),
                                          ("softening", -0.1), ("G", float("inf")),
                                          ("algorithm", 7), ("integrator", 3), ("far_field", 9),
-                                         ("force_kernel", 2), ("mac", -1), ("cells", 5)])
+                                         ("force_kernel", 2), ("mac", -1), ("cells", 5),
+                                         ("walk_stack", 512), ("walk_bodies", 16),
+                                         ("list_flush", 32), ("list_flush", 256)])
 def test_create_rejects_invalid_params(field, value):
     m = np.ones(4)
     x = np.zeros((4, 2))

@@ -338,14 +338,26 @@ def test_full_size_properties():
     assert rms <= 2.5e-2 and mx <= 0.3
 
 
+# Walk settings of the GROUP traversal (nb_params): the defaults (64 bodies per
+# wavefront, 2,048 stack entries, lists summed past 128 entries) and the other
+# values the autotuner tries.
+WALKS = {"default": {}, "bodies32_stack1024_flush64": dict(walk_bodies=32, walk_stack=1024,
+                                                           list_flush=64),
+         "flush192": dict(list_flush=192)}
+
+
+@pytest.mark.parametrize("walk", list(WALKS))
 @pytest.mark.parametrize("name", ["disk4k", "uniform3k", "dups", "ragged65"])
 @pytest.mark.parametrize("theta", [0.0, 0.5, 0.8])
-def test_group_traversal_refines_exact(name, theta):
+def test_group_traversal_refines_exact(name, theta, walk):
     """GROUP accepts a cell only if every member would: per body it applies at
     least the EXACT number of terms, its error against the FP64 direct sum is no
-    larger (RMS), and at theta = 0 it is the direct sum."""
+    larger (RMS), and at theta = 0 it is the direct sum -- with every walk
+    setting (32 bodies per wavefront: two lanes per body, each summing half
+    of the list)."""
     m, p, v = CASES[name]
-    with _ctx(m, p, v, theta=theta, softening=0.01, traversal=nbody2d.abi.TRAV_GROUP) as c:
+    with _ctx(m, p, v, theta=theta, softening=0.01, traversal=nbody2d.abi.TRAV_GROUP,
+              **WALKS[walk]) as c:
         acc, inter, total = c.forces()
     ref_exact, rinter = O.bh_forces(p, m, theta, 0.01)
     direct = O.direct_sum(p, m, 0.01 ** 2)
@@ -360,9 +372,11 @@ def test_group_traversal_refines_exact(name, theta):
         assert e_group[0] <= e_exact[0] * 1.0001 + RMS_TOL, (e_group, e_exact)
 
 
-def test_group_traversal_step_matches_forces():
+@pytest.mark.parametrize("walk", list(WALKS))
+def test_group_traversal_step_matches_forces(walk):
     m, p, v = ics.exp_disk(3000, seed=13)
-    kw = dict(theta=0.6, softening=0.01, traversal=nbody2d.abi.TRAV_GROUP, dt=1e-3)
+    kw = dict(theta=0.6, softening=0.01, traversal=nbody2d.abi.TRAV_GROUP, dt=1e-3,
+              **WALKS[walk])
     with _ctx(m, p, v, **kw) as c:
         acc, _, _ = c.forces()
     with _ctx(m, p, v, **kw) as c:
@@ -373,14 +387,16 @@ def test_group_traversal_step_matches_forces():
     assert np.allclose(gp, p + u * 1e-3, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("bodies", [64, 32])
 @pytest.mark.parametrize("guard", [0, 1])
-def test_group_split_to_single_bodies_equals_exact(guard):
+def test_group_split_to_single_bodies_equals_exact(guard, bodies):
     """With a budget of one entry every group splits down to single bodies,
     whose group test is the per-body opening test: the interaction sets equal
     EXACT's body by body and the forces agree to FP32 rounding."""
     m, p, v = CASES["disk4k"]
     kw = dict(theta=0.8, softening=0.01, self_guard=guard)
-    with _ctx(m, p, v, traversal=nbody2d.abi.TRAV_GROUP, group_budget=1, **kw) as c:
+    with _ctx(m, p, v, traversal=nbody2d.abi.TRAV_GROUP, group_budget=1, walk_bodies=bodies,
+              **kw) as c:
         ga, gi, gt = c.forces()
     with _ctx(m, p, v, traversal=nbody2d.abi.TRAV_EXACT, **kw) as c:
         ea, ei, et = c.forces()
