@@ -131,7 +131,13 @@ typedef struct nb_params {
                              cells sooner) (0 = 64) */
     int32_t list_flush;   /* GROUP: interaction-list entries held in LDS before
                              they are summed, 64 .. 192 (0 = 128) */
-    int32_t reserved;
+    int32_t quad_near;    /* LEARNED: an accepted cell whose centre of mass lies
+                             within quad_near half-diagonals of a traversal
+                             subgroup's box from the box centre adds its
+                             quadrupole term at every body's own position; a
+                             farther cell adds it once at the centre, moved to
+                             each body by its Jacobian (0 = 8; -1 = every cell
+                             at every body) */
 } nb_params;
 
 typedef struct nb_info {
@@ -147,6 +153,10 @@ typedef struct nb_info {
                                  nb_compute_forces, or the mean over the last
                                  nb_step call; the network runs once per cell
                                  and traversal subgroup */
+    uint64_t quad_terms;      /* of the same force evaluations (far_field =
+                                 LEARNED): body terms that also added the cell's
+                                 softened quadrupole term at the body (the
+                                 cells near each traversal subgroup) */
 } nb_info;
 
 /* Timing phases reported by nb_get_timing (milliseconds, summed over the
@@ -381,13 +391,15 @@ int nb_mlp_forward(nb_ctx* ctx, const float* features, uint64_t count, float* ou
  * tests. Groups are up to 64 consecutive bodies of the tree's sorted order,
  * never across two systems (*groups_out of them; ceil(n/64) with one
  * system). Per group: nsub[g] finished subgroups; per subgroup
- * slot (64 per group): its box centre sub_centre[2k..2k+1] and its range
+ * slot (64 per group): its box centre sub_centre[2k..2k+1], its range
  * {start, count} in cell_slots, the tree slots of all regions (groups x
- * *cell_cap_out entries, indexed from 0); per sorted body: its subgroup and
- * its offset from that subgroup's centre. Any output may be NULL. */
+ * *cell_cap_out entries, indexed from 0), and sub_thr[k], the squared
+ * distance from the centre beyond which a cell's quadrupole term is taken at
+ * the centre (nb_params.quad_near); per sorted body: its subgroup and its
+ * offset from that subgroup's centre. Any output may be NULL. */
 int nb_debug_far_field(nb_ctx* ctx, uint32_t* nsub, double* sub_centre, uint32_t* sub_range,
-                       uint32_t* cell_slots, int32_t* body_sub, float* body_off,
-                       uint32_t* cell_cap_out, uint32_t* groups_out);
+                       double* sub_thr, uint32_t* cell_slots, int32_t* body_sub,
+                       float* body_off, uint32_t* cell_cap_out, uint32_t* groups_out);
 
 #ifdef __cplusplus
 }

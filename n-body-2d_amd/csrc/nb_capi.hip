@@ -206,6 +206,8 @@ int check_params(const nb_params* p, std::string* why) {
         return *why = "walk_bodies must be 0, 32 or 64", NB_ERR_INVALID_ARG;
     if (p->list_flush != 0 && (p->list_flush < 64 || p->list_flush > 192))
         return *why = "list_flush must be 0 or in 64 .. 192", NB_ERR_INVALID_ARG;
+    if (p->quad_near < -1 || p->quad_near > 1024)
+        return *why = "quad_near must be -1, 0 or in 1 .. 1024", NB_ERR_INVALID_ARG;
     if (p->far_field != NB_FAR_MONOPOLE && p->far_field != NB_FAR_LEARNED)
         return *why = "unknown far_field", NB_ERR_INVALID_ARG;
     if (p->force_kernel != NB_FORCE_PLUMMER && p->force_kernel != NB_FORCE_REF_CPU)
@@ -339,6 +341,13 @@ uint64_t cells_total(const nb_ctx* c) {
            (uint64_t)c->host_meta[META_CELLS_TOTAL + 1] << 32;
 }
 
+// The learned walk's near-cell quadrupole terms (over all bodies), a 64-bit
+// counter in two meta words.
+uint64_t quad_total(const nb_ctx* c) {
+    return (uint64_t)c->host_meta[META_QUAD_TERMS] |
+           (uint64_t)c->host_meta[META_QUAD_TERMS + 1] << 32;
+}
+
 int read_meta(nb_ctx* c) {
     NB_HIP(c, hipMemcpyAsync(c->host_meta, c->b.meta, META_COUNT * sizeof(uint32_t),
                              hipMemcpyDeviceToHost, c->stream));
@@ -405,7 +414,10 @@ int build_and_force(nb_ctx* c, bool forces) {
         if (rc) return rc;
         const uint32_t flags = c->host_meta[META_FLAGS];
         c->last_internal = c->host_meta[META_INTERNAL];
-        if (forces) c->last_cells = cells_total(c);
+        if (forces) {
+            c->last_cells = cells_total(c);
+            c->last_quad = quad_total(c);
+        }
         if (flags & FLAG_NONFINITE) return fail(c, NB_ERR_NUMERIC, "non-finite body positions");
         if (flags & FLAG_STACK) return fail(c, NB_ERR_OVERFLOW, "traversal stack overflow");
         if (flags & FLAG_SORT) return mark_dead(c, fail(c, NB_ERR_HIP, "radix sort stalled"));
@@ -734,7 +746,10 @@ int nb_step(nb_ctx* c, uint64_t k) {
         const uint32_t flags = c->host_meta[META_FLAGS];
         const uint64_t done = std::min<uint64_t>(c->host_meta[META_STEPS_DONE], remaining);
         c->last_internal = c->host_meta[META_INTERNAL];
-        if (done) c->last_cells = cells_total(c) / done;
+        if (done) {
+            c->last_cells = cells_total(c) / done;
+            c->last_quad = quad_total(c) / done;
+        }
         c->iteration += done;
         c->sim_time += (double)done * c->dt;
         remaining -= done;
@@ -903,6 +918,7 @@ int nb_get_info(nb_ctx* c, nb_info* info) {
     info->node_capacity = c->icap;
     info->device_bytes = c->device_bytes;
     info->far_field_cells = c->last_cells;
+    info->quad_terms = c->last_quad;
     return NB_OK;
 }
 

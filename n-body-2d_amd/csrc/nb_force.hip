@@ -262,10 +262,10 @@ __device__ __forceinline__ void eval_list(const float4* __restrict__ list, int e
     kahan_add(a.ay, a.cy, py);
 }
 
-// The learned far field's list: every entry also carries its second moments
-// about the COM (ixx in the entry's w, ixy and iyy in `quad`; zero for single
-// bodies), and each body adds the cell's softened quadrupole term at its own
-// position to the monopole,
+// The learned far field's near cells: every entry also carries the cell's
+// second moments about the COM (ixx in the entry's w, ixy and iyy in `quad`),
+// and each body adds the cell's softened quadrupole term at its own position
+// to the monopole,
 //   a = m d / R^3 + (3 / (2 R^5)) [(5 (d.I d) / R^2 - tr I) d - 2 I d],
 // d = COM - body, R^2 = |d|^2 + eps^2: 36 flops per entry.
 template <bool ZERO_EPS, int ES>
@@ -346,9 +346,11 @@ __device__ __forceinline__ bool queue_pop(uint32_t* meta, uint64_t* queue, uint3
     return hi != 0;
 }
 
-// LEARN: each body adds the accepted cells' softened quadrupole terms at its
-// own position (eval_list_quad) and the walk leaves the far-field network to
-// k_mlp_apply. The tree slot
+// LEARN: each body adds the softened quadrupole terms of the accepted cells
+// near its subgroup (within near_sq squared box half-diagonals of the box
+// centre) at its own position (eval_list_quad); the farther cells' terms, and
+// the far-field network, are left to k_mlp_apply, at the subgroup centre. The
+// two kinds of entry fill the LDS list from its two ends. The tree slot
 // of every accepted internal cell is also appended to `cell_slot`, in the
 // region of the group the wavefront was launched for; each finished subgroup
 // k of group g gets subs[g][k] (its box centre and its cell range) from a
@@ -367,7 +369,8 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     const uint64_t* __restrict__ keys, uint32_t* __restrict__ cell_slot,
     uint32_t* __restrict__ cell_cnt, uint32_t cell_cap, SubRec* __restrict__ subs,
     float4* __restrict__ body_w, uint64_t* __restrict__ queue, uint32_t queue_cap,
-    const uint32_t* __restrict__ grp_first, const uint32_t* __restrict__ grp_info, int flush) {
+    const uint32_t* __restrict__ grp_first, const uint32_t* __restrict__ grp_info, int flush,
+    double near_sq) {
     static_assert(BPW == 64 || BPW == 32, "64 or 32 bodies per wavefront");
     static_assert(STACK >= kGReserve + 3 * 64, "stack too small for a full take");
     constexpr int ES = 64 / BPW;  // lanes per body
@@ -386,6 +389,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     const float eps2f = (float)k->eps2;
     bool overflow = false;
     uint32_t cbase = 0;  // cells written to the home region (LEARN)
+    uint64_t qterms = 0; // near-cell quadrupole terms applied, over all members (LEARN)
     const uint64_t region = (uint64_t)home * cell_cap;
     if (LEARN && l == 0)
         __hip_atomic_store(&cell_cnt[home], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -466,13 +470,19 @@ __global__ __launch_bounds__(64) void k_traverse_group(
             const double ox = 0.5 * (xmin + xmax), oy = 0.5 * (ymin + ymax);
             const double hx = 0.5 * (xmax - xmin), hy = 0.5 * (ymax - ymin);
             const float bx = (float)(p.x - ox), by = (float)(p.y - oy);
+            // LEARN: a cell whose COM is farther than this (squared) from the
+            // centre takes its quadrupole term at the centre in k_mlp_apply
+            // (every cell, for a single body: its offset from the centre is 0)
+            const double thr = near_sq < 0.0 ? 1e300 : near_sq * fma(hx, hx, hy * hy);
             const int32_t g_first = (int32_t)(first + f);
             const int32_t g_last = (int32_t)(first + min(f + cn, gn) - 1);
             const bool capped = cn > 1;
             Acc t;
             int sp = 1, cnt = 0;
+            int cnq = 0;        // LEARN: near cells, at the top end of the list
             uint32_t ccnt = 0;  // cells of this subgroup written after cbase
             uint32_t listed = 0, steps = 0;
+            uint32_t listedq = 0;  // LEARN: near-cell entries summed
             bool split = false;
             if (l == 0) s_own = 0ull;
             if (l == 0) s_stack[0] = root;  // the root of this group's system
@@ -512,19 +522,27 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                         open = !add;
                     }
                 }
-                // append accepted cells / single-body leaves to the list
-                const uint64_t am = ballot64(add);
+                // append accepted cells / single-body leaves to the list: the
+                // learned walk's near cells, whose quadrupole term every body
+                // adds at its own position, fill it from the top end
+                const bool nearc = LEARN && add && c0 >= 0 && !(fma(dxc, dxc, dyc * dyc) > thr);
+                const uint64_t am = ballot64(add && !nearc);
+                const uint64_t an = LEARN ? ballot64(nearc) : 0ull;
                 if (add) {
-                    const int at = cnt + (int)__popcll(am & lanemask_lt());
-                    const bool cell = c0 >= 0;
-                    s_list[at] = make_float4((float)dxc, (float)dyc, r.mf,
-                                             LEARN && cell ? r.ixx : 0.f);
-                    if (LEARN) s_quad[at] = cell ? make_float2(r.ixy, r.iyy) : make_float2(0.f, 0.f);
+                    if (LEARN && nearc) {
+                        const int at = kGList - 1 - cnq - (int)__popcll(an & lanemask_lt());
+                        s_list[at] = make_float4((float)dxc, (float)dyc, r.mf, r.ixx);
+                        s_quad[at] = make_float2(r.ixy, r.iyy);
+                    } else {
+                        const int at = cnt + (int)__popcll(am & lanemask_lt());
+                        s_list[at] = make_float4((float)dxc, (float)dyc, r.mf, 0.f);
+                    }
                     // a member's own single-body leaf: a zero term, not counted
                     if (c0 < 0 && bs >= g_first && bs <= g_last)
                         atomicOr(&s_own, 1ull << (bs - (int32_t)first));
                 }
                 cnt += (int)__popcll(am);
+                cnq += (int)__popcll(an);
                 if (LEARN) {
                     const bool cell = add && c0 >= 0;
                     const uint64_t cmk = ballot64(cell);
@@ -562,7 +580,8 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                         interact<REFF>(t, ex, ey, fma(ex, ex, ey * ey), (float)mass[j], eps2f);
                     }
                 }
-                if (capped && (listed + (uint32_t)cnt > (uint32_t)budget || steps > max_steps) &&
+                if (capped && (listed + (uint32_t)(cnt + cnq) > (uint32_t)budget ||
+                               steps > max_steps) &&
                     nt + 8 <= 16) {
                     split = true;
                     break;
@@ -572,13 +591,14 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                     overflow = true;
                     break;
                 }
-                if (cnt > flush) {
-                    if (LEARN)
-                        eval_list_quad<ZERO_EPS, ES>(s_list, s_quad, part, cnt, bx, by, eps2f, t);
-                    else
-                        eval_list<ZERO_EPS, REFF, ES>(s_list, part, cnt, bx, by, eps2f, t);
-                    listed += cnt;
-                    cnt = 0;
+                if (cnt + cnq > flush) {
+                    eval_list<ZERO_EPS, REFF, ES>(s_list, part, cnt, bx, by, eps2f, t);
+                    if (LEARN && cnq > 0)
+                        eval_list_quad<ZERO_EPS, ES>(s_list + (kGList - cnq), s_quad + (kGList - cnq),
+                                                     part, cnq, bx, by, eps2f, t);
+                    listed += cnt + cnq;
+                    listedq += cnq;
+                    cnt = cnq = 0;
                 }
                 if (!more) break;
             }
@@ -602,12 +622,13 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                 nt++;
                 continue;
             }
-            if (cnt > 0) {
-                if (LEARN)
-                    eval_list_quad<ZERO_EPS, ES>(s_list, s_quad, part, cnt, bx, by, eps2f, t);
-                else
-                    eval_list<ZERO_EPS, REFF, ES>(s_list, part, cnt, bx, by, eps2f, t);
-                listed += cnt;
+            if (cnt + cnq > 0) {
+                eval_list<ZERO_EPS, REFF, ES>(s_list, part, cnt, bx, by, eps2f, t);
+                if (LEARN && cnq > 0)
+                    eval_list_quad<ZERO_EPS, ES>(s_list + (kGList - cnq), s_quad + (kGList - cnq),
+                                                 part, cnq, bx, by, eps2f, t);
+                listed += cnt + cnq;
+                listedq += cnq;
             }
             // the subgroup is finished: its members write their results
             if (LEARN) {
@@ -622,9 +643,11 @@ __global__ __launch_bounds__(64) void k_traverse_group(
                     sr.oy = oy;
                     sr.start = (uint32_t)(region + cbase);
                     sr.count = ccnt;
+                    sr.thr = thr;
                     subs[(uint64_t)g * 64 + ks] = sr;
                 }
                 cbase += ccnt;
+                qterms += (uint64_t)listedq * (uint64_t)__popcll(ballot64(member && part == 0));
             }
             double ax = (double)t.ax - (double)t.cx;
             double ay = (double)t.ay - (double)t.cy;
@@ -664,6 +687,7 @@ __global__ __launch_bounds__(64) void k_traverse_group(
     if (overflow && l == 0) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_STACK);
     if (LEARN && l == 0) {
         atomicAdd((unsigned long long*)&meta[META_CELLS_TOTAL], (unsigned long long)cbase);
+        atomicAdd((unsigned long long*)&meta[META_QUAD_TERMS], (unsigned long long)qterms);
         atomicMax(&meta[META_CELLS_NEEDED], cbase);  // sizes the regions at creation
         if (cbase > cell_cap) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_CELL_CAPACITY);
     }
@@ -681,7 +705,7 @@ void launch_walk(nb_ctx* c, int src, int count, int budget, uint32_t max_steps, 
                        c->b.acc, c->b.inter, n, c->b.consts, c->b.meta, count, budget,         \
                        max_steps, c->b.keys[0], c->b.cell_slot, c->b.cell_cnt, c->cell_cap,    \
                        c->b.subs, c->b.body_w, c->b.queue, c->queue_cap, c->b.grp_first,       \
-                       c->b.grp_info, flush)
+                       c->b.grp_info, flush, quad_near_sq(c))
     if (stack == 1024) {
         if (bpw == 32) NB_WALK(1024, 32); else NB_WALK(1024, 64);
     } else {

@@ -57,6 +57,7 @@ enum Meta : int {
     META_MULTI_LEAVES = 3, // multi-body leaves queued by the current build
     META_CELLS_NEEDED = 4, // most accepted cells of one group (learned far field)
     META_CELLS_TOTAL = 32, // accepted cells of all groups (learned far field), u64 in 32..33
+    META_QUAD_TERMS = 34,  // near-cell quadrupole terms applied per body, u64 in 34..35
     META_QHEAD = 64,       // traversal work queue: entries pushed
     META_QTAIL = 96,       // traversal work queue: entries taken
     META_COUNT = 128
@@ -88,10 +89,13 @@ struct __attribute__((aligned(64))) NodeRec {
 static_assert(sizeof(NodeRec) == 64, "NodeRec must be 64 bytes");
 
 // A finished subgroup of a traversal group (learned far field): its box
-// centre and its range of accepted cells in cell_slot (absolute indices).
+// centre, its range of accepted cells in cell_slot (absolute indices) and
+// the squared distance from the centre beyond which a cell's quadrupole term
+// is taken at the centre (with its Jacobian) instead of at every body.
 struct SubRec {
     double ox, oy;
     uint32_t start, count;
+    double thr;
 };
 
 // Device-side constants of a step, read by the kernels so set_dt needs no
@@ -205,6 +209,7 @@ struct nb_ctx {
     uint32_t cell_cap = 0;          // accepted-cell region per group
     uint32_t queue_cap = 0;         // traversal work-queue entries
     uint64_t last_cells = 0;        // accepted cells of the last nb_compute_forces
+    uint64_t last_quad = 0;         // near-cell quadrupole terms of the same
     // asynchronous snapshots: two slots, used by tickets alternately
     struct Snapshot {
         double2* dev = nullptr;     // [2n] positions then velocities, original order
@@ -233,6 +238,12 @@ namespace nb {
 inline int walk_stack(const nb_ctx* c) { return c->p.walk_stack > 0 ? c->p.walk_stack : 2048; }
 inline int walk_bodies(const nb_ctx* c) { return c->p.walk_bodies > 0 ? c->p.walk_bodies : 64; }
 inline int list_flush(const nb_ctx* c) { return c->p.list_flush > 0 ? c->p.list_flush : 128; }
+// Squared near radius of the learned walk's per-body quadrupole terms, in
+// squared subgroup half-diagonals (< 0: every cell is near).
+inline double quad_near_sq(const nb_ctx* c) {
+    const int k = c->p.quad_near == 0 ? 8 : c->p.quad_near;
+    return k < 0 ? -1.0 : (double)k * k;
+}
 
 // ---- launchers (nb_sort.hip) ----
 void launch_bbox(nb_ctx* c, int src);

@@ -35,9 +35,10 @@
 // over f2 = s/R. The rest is led by the octupole, of order s/R in these
 // units and linear in the third moments f8..f11 (with coefficients that
 // depend on rho), so every target is of order one and a far cell's share of
-// the network's error is small. In a step the walk adds each cell's softened
-// quadrupole term at every body's own position, and f2 (y0, y1) is added at
-// the centre of the body's subgroup.
+// the network's error is small. In a step the walk adds the softened
+// quadrupole term of each cell near the body's subgroup at the body's own
+// position; the farther cells' terms (moved to the body by their Jacobian,
+// quad_jacobian) and f2 (y0, y1) are added at the centre of the subgroup.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -183,6 +184,31 @@ __device__ inline float mlp_features(float dx, float dy, float s, float M, float
     q[0] = 1.5f * fmaf(5.f * rho, f[4], -fmaf(3.f, f[4], f[6]));
     q[1] = -3.f * f[5];
     return ms2 * r * (iR2 * iR2 * inv_R);
+}
+
+// Jacobian dQ/dd of the softened quadrupole term Q(d) = (3/(2R^5))
+// [-tr(I) d - 2 I d + 5 (d.Id) d / R^2] (d = COM - point, R^2 = |d|^2 + eps^2),
+// FP32: J = {dQx/ddx, dQx/ddy, dQy/ddx, dQy/ddy}. Moving the point by b
+// changes d by -b, so Q(point + b) ~ Q - J b.
+__device__ inline void quad_jacobian(float dx, float dy, float ixx, float ixy, float iyy,
+                                     float eps2, float J[4]) {
+    const float R2 = fmaf(dx, dx, fmaf(dy, dy, eps2));
+    const float iR = __builtin_amdgcn_rsqf(R2);
+    const float iR2 = iR * iR;
+    const float iR5 = iR2 * iR2 * iR;
+    const float t = ixx + iyy;
+    const float idx = fmaf(ixx, dx, ixy * dy), idy = fmaf(ixy, dx, iyy * dy);
+    const float S = fmaf(dx, idx, dy * idy);
+    const float px = fmaf(5.f * S * iR2, dx, -fmaf(t, dx, 2.f * idx));
+    const float py = fmaf(5.f * S * iR2, dy, -fmaf(t, dy, 2.f * idy));
+    const float k = 1.5f * iR5;
+    const float diag = 5.f * S * iR2 - t;
+    const float a = 5.f * iR2, b = 10.f * iR2, c = 10.f * S * iR2 * iR2;
+    // J_ij = k [ -a d_j P_i - t d_ij - 2 I_ij + b (Id)_j d_i + 5 S R^-2 d_ij - c d_i d_j ]
+    J[0] = k * (-a * dx * px + diag - 2.f * ixx + b * idx * dx - c * dx * dx);
+    J[1] = k * (-a * dy * px - 2.f * ixy + b * idy * dx - c * dx * dy);
+    J[2] = k * (-a * dx * py - 2.f * ixy + b * idx * dy - c * dy * dx);
+    J[3] = k * (-a * dy * py + diag - 2.f * iyy + b * idy * dy - c * dy * dy);
 }
 
 // 1 / ell for the distance input: eps, or the root cell side when eps = 0.

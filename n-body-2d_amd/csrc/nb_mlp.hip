@@ -350,10 +350,13 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X
 // go through the network 64 at a time at the subgroup's box centre (lane l
 // prepares cell l; tile q of the MFMA pass is cells 16q..16q+15). A subgroup
 // ends with C = the sum of the network's terms (s/R times its output, in
-// acceleration units) at the centre; every member adds C to the walk's
-// acceleration, which already holds each cell's softened quadrupole term at
-// the member's own position, and kicks (or stores G a). The sums run in a
-// fixed order.
+// acceleration units) and of the far cells' softened quadrupole terms at the
+// centre, and J = the sum of those terms' Jacobians; every member at offset b
+// from the centre adds C - J b to the walk's acceleration (which already
+// holds the near cells' quadrupole terms at the member's own position), and
+// kicks (or stores G a). A cell is far when its COM lies beyond the
+// subgroup's threshold from the centre, decided in FP64 exactly as the walk
+// decided it. The sums run in a fixed order.
 // ---------------------------------------------------------------------------
 constexpr int kApplyWaves = 4;
 
@@ -370,9 +373,10 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     const uint32_t* __restrict__ grp_info) {
     if (meta[META_FLAGS] & kHaltFlags) return;
     __shared__ __attribute__((aligned(16))) float sb[kMlpBiasLen];
-    __shared__ float su[kApplyWaves][2 * 64];        // [C_x, C_y][subgroup]
+    __shared__ float su[kApplyWaves][6 * 64];        // [C_x, C_y, J00, J01, J10, J11][subgroup]
     __shared__ __attribute__((aligned(16))) float sf[kApplyWaves][64 * kMlpIn];  // cell inputs
-    __shared__ float2 sr[kApplyWaves][64];           // cell scale * (s/R) * u
+    __shared__ float2 sr[kApplyWaves][64];           // cell scale * u
+    __shared__ float4 sq[kApplyWaves][64];           // far cell's quadrupole term (units), s/R
     for (int i = threadIdx.x; i < kMlpBiasLen; i += 64 * kApplyWaves) sb[i] = bias[i];
     if (count_step && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
     __syncthreads();
@@ -420,7 +424,7 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     for (int ks = 0; ks < nsub; ks++) {
         const SubRec sub = subs[(uint64_t)g * 64 + ks];
         const uint64_t end = min((uint64_t)sub.start + sub.count, slots);
-        float cx = 0.f, cy = 0.f;
+        float cx = 0.f, cy = 0.f, j00 = 0.f, j01 = 0.f, j10 = 0.f, j11 = 0.f;
         CellIn nx = load_cells(sub.start, end);
         for (uint64_t b0 = sub.start; b0 < end; b0 += 64) {
             const CellIn cur = nx;
@@ -441,7 +445,24 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
             float f[kMlpIn], ux, uy, qd[2];
             float scale = mlp_features(dx, dy, s, M, ixx, ixy, iyy, t3, eps2f, inv_mroot,
                                        inv_ell, f, ux, uy, qd);
-            scale = ok ? scale * f[2] : 0.f;
+            scale = ok ? scale : 0.f;
+            // A far cell (the walk's rule, in FP64 on the same values) adds its
+            // quadrupole term here, at the centre, and its Jacobian moves the
+            // term to each member; a near cell's term every member added itself.
+            bool far = false;
+            if (ok) {
+                const double ex = cur.cx - sub.ox, ey = cur.cy - sub.oy;
+                far = fma(ex, ex, ey * ey) > sub.thr;
+            }
+            if (far) {
+                float J[4];
+                quad_jacobian(dx, dy, ixx, ixy, iyy, eps2f, J);
+                j00 += J[0];
+                j01 += J[1];
+                j10 += J[2];
+                j11 += J[3];
+            }
+            sq[wv][l] = make_float4(far ? qd[0] : 0.f, far ? qd[1] : 0.f, f[2], 0.f);
 #pragma unroll
             for (int h = 0; h < kMlpIn / 4; h++)
                 *reinterpret_cast<float4*>(f_all + kMlpIn * l + 4 * h) =
@@ -464,9 +485,10 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const float2 su_ = sr[wv][(16 * q + l) & 63];
+                const float4 qq = sq[wv][(16 * q + l) & 63];
                 if (l < 16) {
-                    // the network's term: (s/R) * output along (u, w)
-                    const float yu = y[q].x, yw = y[q].y;
+                    // (s/R) * output along (u, w), plus a far cell's quadrupole term
+                    const float yu = fmaf(y[q].x, qq.z, qq.x), yw = fmaf(y[q].y, qq.z, qq.y);
                     cx += yu * su_.x - yw * su_.y;
                     cy += yu * su_.y + yw * su_.x;
                 }
@@ -477,10 +499,18 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
         for (int d = 1; d < 64; d <<= 1) {
             cx += __shfl_xor(cx, d, 64);
             cy += __shfl_xor(cy, d, 64);
+            j00 += __shfl_xor(j00, d, 64);
+            j01 += __shfl_xor(j01, d, 64);
+            j10 += __shfl_xor(j10, d, 64);
+            j11 += __shfl_xor(j11, d, 64);
         }
         if (l == 0) {
             u[0 * 64 + ks] = cx;
             u[1 * 64 + ks] = cy;
+            u[2 * 64 + ks] = j00;
+            u[3 * 64 + ks] = j01;
+            u[4 * 64 + ks] = j10;
+            u[5 * 64 + ks] = j11;
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -488,7 +518,10 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     const uint32_t i = first + l;
     const float4 bw = body_w[i];
     const int ks = __float_as_int(bw.z) & 63;
-    const float ax = u[0 * 64 + ks], ay = u[1 * 64 + ks];
+    // the body sits at b = (bw.x, bw.y) from the centre: d = COM - point moves
+    // by -b, so a far cell's term there is Q - J b
+    const float ax = u[0 * 64 + ks] - fmaf(u[2 * 64 + ks], bw.x, u[3 * 64 + ks] * bw.y);
+    const float ay = u[1 * 64 + ks] - fmaf(u[4 * 64 + ks], bw.x, u[5 * 64 + ks] * bw.y);
     const double2 a0 = acc[i];
     const double2 a = make_double2(a0.x + (double)ax, a0.y + (double)ay);
     if (KICK) {
@@ -892,8 +925,8 @@ int nb_mlp_grad(nb_ctx* c, const float* features, const float* targets, uint64_t
 }
 
 int nb_debug_far_field(nb_ctx* c, uint32_t* nsub, double* sub_centre, uint32_t* sub_range,
-                       uint32_t* cell_slots, int32_t* body_sub, float* body_off,
-                       uint32_t* cell_cap_out, uint32_t* groups_out) {
+                       double* sub_thr, uint32_t* cell_slots, int32_t* body_sub,
+                       float* body_off, uint32_t* cell_cap_out, uint32_t* groups_out) {
     if (!c) return NB_ERR_INVALID_ARG;
     if (!c->b.cell_slot)
         return mlp_fail(c, NB_ERR_NOT_SUPPORTED, "no learned far field in this context");
@@ -901,7 +934,7 @@ int nb_debug_far_field(nb_ctx* c, uint32_t* nsub, double* sub_centre, uint32_t* 
     if (cell_cap_out) *cell_cap_out = c->cell_cap;
     if (groups_out) *groups_out = (uint32_t)groups;
     if (nsub) MLP_HIP(c, scopy(c, nsub, c->b.cell_cnt, groups * 4, hipMemcpyDeviceToHost));
-    if (sub_centre || sub_range) {
+    if (sub_centre || sub_range || sub_thr) {
         std::vector<SubRec> h(groups * 64);
         MLP_HIP(c, scopy(c, h.data(), c->b.subs, h.size() * sizeof(SubRec),
                          hipMemcpyDeviceToHost));
@@ -914,6 +947,7 @@ int nb_debug_far_field(nb_ctx* c, uint32_t* nsub, double* sub_centre, uint32_t* 
                 sub_range[2 * k] = h[k].start;
                 sub_range[2 * k + 1] = h[k].count;
             }
+            if (sub_thr) sub_thr[k] = h[k].thr;
         }
     }
     if (cell_slots)

@@ -61,7 +61,7 @@ class Params(C.Structure):
         ("group_budget", C.c_int32), ("cell_capacity", C.c_int32),
         ("force_kernel", C.c_int32), ("mac", C.c_int32), ("cells", C.c_int32),
         ("walk_stack", C.c_int32), ("walk_bodies", C.c_int32), ("list_flush", C.c_int32),
-        ("reserved", C.c_int32),
+        ("quad_near", C.c_int32),
     ]
 
 
@@ -70,7 +70,7 @@ class Info(C.Structure):
         ("n", C.c_uint64), ("iteration", C.c_uint64), ("simulated_time", C.c_double),
         ("tree_nodes", C.c_uint64), ("internal_nodes", C.c_uint64),
         ("node_capacity", C.c_uint64), ("device_bytes", C.c_uint64),
-        ("far_field_cells", C.c_uint64),
+        ("far_field_cells", C.c_uint64), ("quad_terms", C.c_uint64),
     ]
 
 
@@ -133,7 +133,7 @@ def lib() -> C.CDLL:
                                    C.POINTER(C.c_float)]
         L.nb_mlp_grad.argtypes = [P, P, P, C.c_uint64, P, C.POINTER(C.c_float)]
         L.nb_mlp_forward.argtypes = [P, P, C.c_uint64, P]
-        L.nb_debug_far_field.argtypes = [P, P, P, P, P, P, P, C.POINTER(C.c_uint32),
+        L.nb_debug_far_field.argtypes = [P, P, P, P, P, P, P, P, C.POINTER(C.c_uint32),
                                          C.POINTER(C.c_uint32)]
         L.nb_autotune.argtypes = [P, C.c_uint32, C.POINTER(Tuning)]
         L.nb_debug_fail_alloc.argtypes = [P, C.c_uint32]
@@ -355,20 +355,23 @@ class Context:
         return y
 
 
-    def debug_far_field(self):
+    def debug_far_field(self, with_thr: bool = False):
         cap, ng = C.c_uint32(), C.c_uint32()
-        self._check(lib().nb_debug_far_field(self._h, None, None, None, None, None, None,
+        self._check(lib().nb_debug_far_field(self._h, None, None, None, None, None, None, None,
                                              C.byref(cap), C.byref(ng)))
         groups = ng.value
         nsub = np.empty(groups, dtype=np.uint32)
         centre = np.empty((groups, 64, 2))
         rng = np.empty((groups, 64, 2), dtype=np.uint32)
+        thr = np.empty((groups, 64))
         slots = np.empty((groups, cap.value), dtype=np.uint32)
         bsub = np.empty(self.n, dtype=np.int32)
         boff = np.empty((self.n, 2), dtype=np.float32)
         self._check(lib().nb_debug_far_field(self._h, _ptr(nsub), _ptr(centre), _ptr(rng),
-                                             _ptr(slots), _ptr(bsub), _ptr(boff), C.byref(cap),
-                                             C.byref(ng)))
+                                             _ptr(thr), _ptr(slots), _ptr(bsub), _ptr(boff),
+                                             C.byref(cap), C.byref(ng)))
+        if with_thr:
+            return nsub, centre, rng, thr, slots, bsub, boff
         return nsub, centre, rng, slots, bsub, boff
 
 

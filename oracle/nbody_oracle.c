@@ -27,6 +27,7 @@
  *   Part D (the reference has no counterpart; pinned by mathematics):
  *   or_quadrupole             order of the residual exact - monopole -
  *                             quadrupole against brute-force sums
+ *   or_quadrupole_jacobian    central differences of or_quadrupole
  *   or_cell_correction        brute-force pair sums; zero for one body
  *   or_mlp_features           hand-computed values
  *   or_mlp_quad_units         or_quadrupole taken along (u, w) over the
@@ -1068,6 +1069,30 @@ void or_quadrupole(double dx, double dy, double ixx, double ixy, double iyy, dou
     const double k = 1.5 / R5;
     *ax = k * (-tr * dx - 2.0 * idx + 5.0 * dId * dx / R2);
     *ay = k * (-tr * dy - 2.0 * idy + 5.0 * dId * dy / R2);
+}
+
+/* Jacobian J[i][j] = dQ_i/dd_j of or_quadrupole, written out from
+ * Q_i = (3/2) R^-5 P_i, P = -tr(I) d - 2 I d + 5 S d R^-2, S = d.Id:
+ *   J_ij = (3/2) R^-5 [ -5 R^-2 d_j P_i - tr(I) d_ij - 2 I_ij
+ *                       + 10 R^-2 (Id)_j d_i + 5 S R^-2 d_ij - 10 S R^-4 d_i d_j ]
+ * (pinned by central differences of or_quadrupole in the tests). */
+void or_quadrupole_jacobian(double dx, double dy, double ixx, double ixy, double iyy,
+                            double eps2, double* J) {
+    const double R2 = dx * dx + dy * dy + eps2, iR2 = 1.0 / R2, iR5 = iR2 * iR2 / sqrt(R2);
+    const double t = ixx + iyy;
+    const double d[2] = {dx, dy};
+    const double Id[2] = {ixx * dx + ixy * dy, ixy * dx + iyy * dy};
+    const double I[2][2] = {{ixx, ixy}, {ixy, iyy}};
+    const double S = dx * Id[0] + dy * Id[1];
+    double P[2];
+    for (int i = 0; i < 2; i++) P[i] = -t * d[i] - 2.0 * Id[i] + 5.0 * S * d[i] * iR2;
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+            const double dij = i == j ? 1.0 : 0.0;
+            J[2 * i + j] = 1.5 * iR5 * (-5.0 * iR2 * d[j] * P[i] - t * dij - 2.0 * I[i][j] +
+                                        10.0 * iR2 * Id[j] * d[i] + 5.0 * S * iR2 * dij -
+                                        10.0 * S * iR2 * iR2 * d[i] * d[j]);
+        }
 }
 
 /* T(a, b, c) = sum_ijk a_i b_j c_k T_ijk for the symmetric 2-D 3-tensor

@@ -138,6 +138,9 @@ double or_mlp_grad(const double* params, uint64_t count, const float* x, const f
  * R^2 = |d|^2 + eps2. */
 void or_quadrupole(double dx, double dy, double ixx, double ixy, double iyy, double eps2,
                    double* ax, double* ay);
+/* Jacobian J = {dQx/ddx, dQx/ddy, dQy/ddx, dQy/ddy} of or_quadrupole. */
+void or_quadrupole_jacobian(double dx, double dy, double ixx, double ixy, double iyy,
+                            double eps2, double* J);
 /* Network inputs f[12] of a (cell, point) pair (t3 = third moments {T_xxx,
  * T_xxy, T_xyy, T_yyy} about the COM); returns the output scale. */
 double or_mlp_features(double dx, double dy, double s, double M, double ixx, double ixy,

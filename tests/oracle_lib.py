@@ -58,6 +58,7 @@ def load(fast: bool = False) -> C.CDLL:
     L.or_mlp_grad.argtypes = [_P, u64, _P, _P, _P, i]
     L.or_mlp_grad.restype = d
     L.or_quadrupole.argtypes = [d, d, d, d, d, d, C.POINTER(d), C.POINTER(d)]
+    L.or_quadrupole_jacobian.argtypes = [d, d, d, d, d, d, _P]
     L.or_mlp_features.argtypes = [d, d, d, d, d, d, d, _P, d, d, d, _P]
     L.or_mlp_features.restype = d
     L.or_cell_correction.argtypes = [_P, _P, i64, i64, d, d, d, _P]
@@ -263,6 +264,14 @@ def quadrupole(d, I, eps2=0.0):
                          float(eps2), C.byref(ax), C.byref(ay))
     return np.array([ax.value, ay.value])
 
+
+
+def quadrupole_jacobian(d, I, eps2=0.0):
+    """J[i, j] = dQ_i / dd_j of the softened quadrupole term (d = COM - point)."""
+    J = np.empty(4)
+    load().or_quadrupole_jacobian(float(d[0]), float(d[1]), float(I[0]), float(I[1]),
+                                  float(I[2]), float(eps2), ptr(J))
+    return J.reshape(2, 2)
 
 def mlp_features(d, s, M, I, eps2, m_root, ell, T=(0.0, 0.0, 0.0, 0.0)):
     """Network inputs f[12] and output scale; T = third moments {xxx, xxy,

@@ -31,12 +31,24 @@ def test_roofline_uses_the_timed_launches():
     assert r["interactions_per_launch"] == 7.5e8 and r["avg_launch_ms"] == 0.6
 
 
+def test_learned_roofline_counts_near_cell_quadrupole_terms():
+    """The learned walk's flops: 14 per interaction plus 22 for each term that
+    also adds a near cell's quadrupole term at the body."""
+    r = bench.roofline_walk(_leg(far_field="learned", quad_terms_per_step_timed=2e8), "group")
+    assert r["achieved"] == pytest.approx((7.5e8 * 14 + 2e8 * 22) / 0.6e-3 / 1e12)
+    assert r["quad_terms_per_launch"] == 2e8
+
+
 def test_roofline_traffic_comes_from_the_profiles_per_leg():
+    """Each leg's traffic is the committed counter summary's line for its own
+    walk kernel: the plain walk has one; the learned walk and the far-field
+    apply have none for the current code (their counter passes were not
+    taken), so their traffic is null rather than another kernel's."""
     plain = bench.roofline_walk(_leg(), "group")["traffic"]
     learned = bench.roofline_walk(_leg(far_field="learned"), "group")["traffic"]
-    assert plain is not None and learned is not None
-    assert 1e8 < plain < 1e9 and 1e8 < learned < 1e9
-    assert bench.traffic_from_profile("k_mlp_apply") is not None
+    assert plain is not None and 1e8 < plain < 1e9
+    assert learned is None
+    assert bench.traffic_from_profile("k_mlp_apply") is None
     assert bench.traffic_from_profile("no_such_kernel") is None
 
 

@@ -237,19 +237,31 @@ def test_learned_graph_and_eager_steps_agree():
     assert np.array_equal(pa, pb) and np.array_equal(va, vb)
 
 
-def test_far_field_correction_matches_oracle():
+def _fma_sq(ex, ey):
+    """fma(ex, ex, ey * ey) in FP64, as the kernels compute a cell's squared
+    distance from the subgroup centre (the product ey * ey rounded first)."""
+    e = np.longdouble(ex) * np.longdouble(ex) + np.longdouble(np.float64(ey) * np.float64(ey))
+    return np.float64(e)
+
+
+@pytest.mark.parametrize("quad_near", [-1, 0, 2])
+def test_far_field_correction_matches_oracle(quad_near):
     """Per body, learned minus monopole equals the oracle's sum over the
-    subgroup's accepted cells of the softened quadrupole term at the body's
-    own position plus s/R times the network's output at the subgroup
-    centre."""
+    subgroup's accepted cells of s/R times the network's output at the
+    subgroup centre plus the softened quadrupole term: at the body's own
+    position for a cell within the subgroup's near radius (quad_near box
+    half-diagonals; -1: every cell), and for a farther cell at the centre,
+    moved to the body by the term's Jacobian (Q - J b, b = the body's offset
+    from the centre)."""
     m, p, v = ics.exp_disk(4096, seed=12)
     w = (np.random.default_rng(3).normal(size=MLP_PARAMS) * 0.1).astype(np.float32)
     kw = dict(theta=0.8, softening=EPS, traversal=TRAV_GROUP, self_guard=1)
-    with _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as b, \
+    with _ctx(m, p, v, far_field=FAR_LEARNED, quad_near=quad_near, **kw) as b, \
             _ctx(m, p, v, far_field=FAR_MONOPOLE, **kw) as a:
         b.mlp_set_weights(w)
         fb, _, _ = b.forces()
-        nsub, centre, rng_, slots, bsub, boff = b.debug_far_field()
+        quad_terms = int(b.info().quad_terms)
+        nsub, centre, rng_, thr, slots, bsub, boff = b.debug_far_field(with_thr=True)
         box, keys, orig, cnt = b.debug_build()
         nodes = b.debug_tree(cnt)
         fa, _, _ = a.forces()
@@ -259,7 +271,7 @@ def test_far_field_correction_matches_oracle():
     m_root = nodes[0]["mass"]
     want = np.zeros_like(got)
     scale_ref = np.zeros(len(got))
-    checked = 0
+    checked = nfar = ncells = near_terms = 0
     for g in range(len(nsub)):
         for k in range(int(nsub[g])):
             st, ct = rng_[g, k]
@@ -280,21 +292,36 @@ def test_far_field_correction_matches_oracle():
                 for yy, sc, u, ff in zip(y.astype(np.float64), S, U, X):
                     C += sc * ff[2] * (yy[0] * u + yy[1] * np.array([-u[1], u[0]]))
             members = np.nonzero((bsub == k) & (np.arange(len(bsub)) // 64 == g))[0]
+            far = [_fma_sq(c["cx"] - centre[g, k][0], c["cy"] - centre[g, k][1]) > thr[g, k]
+                   for c in nd]
+            nfar += sum(far)
+            ncells += len(nd)
+            near_terms += (len(nd) - sum(far)) * len(members)
             for i in members:
                 Q = np.zeros(2)
                 Qabs = 0.0
-                for c in nd:
-                    d = np.array([c["cx"], c["cy"]]) - sp[i]
-                    q = O.quadrupole(d, (c["ixx"], c["ixy"], c["iyy"]), eps2)
+                b_i = sp[i] - centre[g, k]
+                for c, cf in zip(nd, far):
+                    I = (c["ixx"], c["ixy"], c["iyy"])
+                    if cf:  # at the centre, moved to the body by the Jacobian
+                        dc = np.array([c["cx"], c["cy"]]) - centre[g, k]
+                        q = O.quadrupole(dc, I, eps2) - O.quadrupole_jacobian(dc, I, eps2) @ b_i
+                    else:
+                        q = O.quadrupole(np.array([c["cx"], c["cy"]]) - sp[i], I, eps2)
                     Q += q
                     Qabs += np.abs(q).sum()
                 want[i] = C + Q
                 scale_ref[i] = np.abs(C).sum() + Qabs
                 checked += 1
     assert checked == len(m)
+    if quad_near == -1:
+        assert nfar == 0
+    else:
+        assert 0 < nfar < ncells
+    assert quad_terms == near_terms  # what the walk reports for the roofline
     err = np.abs(got - want).max(1) / np.maximum(scale_ref, 1e-30)
-    print(f"[gpu] far-field correction vs oracle: rel err median {np.median(err):.2e} "
-          f"max {err.max():.2e}")
+    print(f"[gpu] far-field correction vs oracle (quad_near {quad_near}, {nfar}/{ncells} far): "
+          f"rel err median {np.median(err):.2e} max {err.max():.2e}")
     assert np.median(err) < 1e-4
     assert err.max() < 2e-2
 

@@ -2,6 +2,7 @@
 
 The reference has no learned correction, no second moments and no network, so
 nothing here has a reference vector: each function is pinned by mathematics.
+  or_quadrupole_jacobian  central differences of or_quadrupole
   or_quadrupole       order of the residual exact - monopole - quadrupole
                       against brute-force sums (a wrong sign or coefficient
                       leaves a second-order residual), with and without eps
@@ -223,3 +224,18 @@ def test_mlp_grad_matches_finite_differences():
         assert fd == pytest.approx(g[i], rel=1e-5, abs=1e-8), i
     y = O.mlp_forward(p, x, bf16=False).astype(np.float64)
     assert loss == pytest.approx(((y - t) ** 2).sum(1).mean(), rel=1e-6)
+
+
+@pytest.mark.parametrize("eps2", [0.0, 0.3])
+def test_quadrupole_jacobian_matches_central_differences(eps2):
+    rng = np.random.default_rng(9)
+    for _ in range(20):
+        d = rng.normal(size=2) * 2
+        I = np.array([rng.uniform(0.1, 1), rng.uniform(-0.3, 0.3), rng.uniform(0.1, 1)])
+        J = O.quadrupole_jacobian(d, I, eps2)
+        h = 1e-6 * np.linalg.norm(d)
+        for j in range(2):
+            e = np.zeros(2)
+            e[j] = h
+            fd = (O.quadrupole(d + e, I, eps2) - O.quadrupole(d - e, I, eps2)) / (2 * h)
+            np.testing.assert_allclose(J[:, j], fd, rtol=1e-6, atol=1e-9 * np.abs(J).max())
