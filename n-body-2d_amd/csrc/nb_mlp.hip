@@ -395,19 +395,56 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
     const int nsub = (int)min(cell_cnt[g], 64u);
     const uint64_t slots = (uint64_t)groups * cell_cap;
     const int col = l & 15;
-    // The records of the next batch of cells are loaded while the current
-    // batch runs through the network (raw fields, converted when used).
+    // Lane k holds subgroup k's record; the members' own data, needed only at
+    // the end, load now. Every subgroup's sums start at zero (one without
+    // cells keeps C = J = 0).
+    SubRec mys{0.0, 0.0, 0u, 0u, 0.0};
+    if (l < nsub) mys = subs[(uint64_t)g * 64 + l];
+    const bool mem = (uint32_t)l < gn;
+    const uint32_t i = first + (mem ? (uint32_t)l : 0u);
+    const float4 bw = mem ? body_w[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const double2 a0 = mem ? acc[i] : make_double2(0.0, 0.0);
+    for (int e = l; e < 6 * 64; e += 64) u[e] = 0.f;
+    auto sub_begin = [&](int kk) -> uint64_t {
+        return (uint64_t)(uint32_t)__shfl((int)mys.start, kk, 64);
+    };
+    auto sub_end = [&](int kk) -> uint64_t {
+        return min(sub_begin(kk) + (uint32_t)__shfl((int)mys.count, kk, 64), slots);
+    };
+    // The batches of 64 cells of all subgroups, in order, as one sequence:
+    // the cells' slot indices are loaded two batches ahead and their records
+    // one batch ahead, across subgroup boundaries, so the subgroup records,
+    // slot indices and tree records are never waited for in turn.
+    struct It {
+        int ks;
+        uint64_t b0, end;
+    };
+    auto seek = [&](int kk) -> It {  // the first batch of the first non-empty subgroup >= kk
+        for (; kk < nsub; kk++) {
+            const uint64_t b = sub_begin(kk), e = sub_end(kk);
+            if (b < e) return It{kk, b, e};
+        }
+        return It{nsub, 0, 0};
+    };
+    auto advance = [&](It t) -> It {
+        if (t.ks >= nsub) return t;
+        t.b0 += 64;
+        return t.b0 < t.end ? t : seek(t.ks + 1);
+    };
+    auto load_slot = [&](const It& t) -> uint32_t {
+        const uint64_t cell = t.b0 + l;
+        return (t.ks < nsub && cell < t.end) ? cell_slot[cell] : 0xffffffffu;
+    };
+    // raw fields of a cell's records, converted when used
     struct CellIn {
         double cx, cy, wsq;
         float mf, ixx, ixy, iyy;
         float4 t3;
         bool ok;
     };
-    auto load_cells = [&](uint64_t b0, uint64_t end) {
+    auto load_cell = [&](uint32_t slot) {
         CellIn c{0.0, 0.0, 1.0, 1.f, 0.f, 0.f, 0.f, make_float4(0.f, 0.f, 0.f, 0.f), false};
-        const uint64_t cell = b0 + l;
-        if (cell < end) {
-            const uint32_t slot = cell_slot[cell];
+        if (slot != 0xffffffffu) {
             const NodeRec* r = nodes + slot;
             c.t3 = mom3[slot];
             c.cx = r->cx;
@@ -421,108 +458,114 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_mlp_apply(
         }
         return c;
     };
-    for (int ks = 0; ks < nsub; ks++) {
-        const SubRec sub = subs[(uint64_t)g * 64 + ks];
-        const uint64_t end = min((uint64_t)sub.start + sub.count, slots);
-        float cx = 0.f, cy = 0.f, j00 = 0.f, j01 = 0.f, j10 = 0.f, j11 = 0.f;
-        CellIn nx = load_cells(sub.start, end);
-        for (uint64_t b0 = sub.start; b0 < end; b0 += 64) {
-            const CellIn cur = nx;
-            if (b0 + 64 < end) nx = load_cells(b0 + 64, end);
-            const bool ok = cur.ok;
-            float dx = 1.f, dy = 0.f, s = 1.f, M = 1.f, ixx = 0.f, ixy = 0.f, iyy = 0.f;
-            float4 t3 = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ok) {
-                t3 = cur.t3;
-                dx = (float)(cur.cx - sub.ox);
-                dy = (float)(cur.cy - sub.oy);
-                s = sqrtf((float)cur.wsq);
-                M = cur.mf;
-                ixx = cur.ixx;
-                ixy = cur.ixy;
-                iyy = cur.iyy;
-            }
-            float f[kMlpIn], ux, uy, qd[2];
-            float scale = mlp_features(dx, dy, s, M, ixx, ixy, iyy, t3, eps2f, inv_mroot,
-                                       inv_ell, f, ux, uy, qd);
-            scale = ok ? scale : 0.f;
-            // A far cell (the walk's rule, in FP64 on the same values) adds its
-            // quadrupole term here, at the centre, and its Jacobian moves the
-            // term to each member; a near cell's term every member added itself.
-            bool far = false;
-            if (ok) {
-                const double ex = cur.cx - sub.ox, ey = cur.cy - sub.oy;
-                far = fma(ex, ex, ey * ey) > sub.thr;
-            }
-            if (far) {
-                float J[4];
-                quad_jacobian(dx, dy, ixx, ixy, iyy, eps2f, J);
-                j00 += J[0];
-                j01 += J[1];
-                j10 += J[2];
-                j11 += J[3];
-            }
-            sq[wv][l] = make_float4(far ? qd[0] : 0.f, far ? qd[1] : 0.f, f[2], 0.f);
-#pragma unroll
-            for (int h = 0; h < kMlpIn / 4; h++)
-                *reinterpret_cast<float4*>(f_all + kMlpIn * l + 4 * h) =
-                    make_float4(f[4 * h], f[4 * h + 1], f[4 * h + 2], f[4 * h + 3]);
-            sr[wv][l] = make_float2(scale * ux, scale * uy);
-            __builtin_amdgcn_wave_barrier();
-            f32x4 x[4], y[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                x[q] = f32x4{0.f, 0.f, 0.f, 0.f};
-                if (l < 16 * (kMlpIn / 4)) {
-                    const float4 v = *reinterpret_cast<const float4*>(
-                        f_all + kMlpIn * (16 * q + col) + 4 * (l >> 4));
-                    x[q] = f32x4{v.x, v.y, v.z, v.w};
-                }
-            }
-            mlp_forward_tiles<2>(x, w, sb, l, y);
-            mlp_forward_tiles<2>(x + 2, w, sb, l, y + 2);
-            // lanes 0-15: cells 16q + l, rotated back to (x, y)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const float2 su_ = sr[wv][(16 * q + l) & 63];
-                const float4 qq = sq[wv][(16 * q + l) & 63];
-                if (l < 16) {
-                    // (s/R) * output along (u, w), plus a far cell's quadrupole term
-                    const float yu = fmaf(y[q].x, qq.z, qq.x), yw = fmaf(y[q].y, qq.z, qq.y);
-                    cx += yu * su_.x - yw * su_.y;
-                    cy += yu * su_.y + yw * su_.x;
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
+    It cur = seek(0);
+    It nxt = advance(cur);
+    CellIn rec = load_cell(load_slot(cur));
+    uint32_t slot_n = load_slot(nxt);
+    float cx = 0.f, cy = 0.f, j00 = 0.f, j01 = 0.f, j10 = 0.f, j11 = 0.f;
+    while (cur.ks < nsub) {
+        const CellIn c = rec;
+        const It nn = advance(nxt);
+        rec = load_cell(slot_n);      // the next batch's records
+        slot_n = load_slot(nn);       // and the one after's slot indices
+        const int ks = cur.ks;
+        const double ox = __shfl(mys.ox, ks, 64), oy = __shfl(mys.oy, ks, 64);
+        const double thr = __shfl(mys.thr, ks, 64);
+        const bool ok = c.ok;
+        float dx = 1.f, dy = 0.f, s = 1.f, M = 1.f, ixx = 0.f, ixy = 0.f, iyy = 0.f;
+        float4 t3 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ok) {
+            t3 = c.t3;
+            dx = (float)(c.cx - ox);
+            dy = (float)(c.cy - oy);
+            s = sqrtf((float)c.wsq);
+            M = c.mf;
+            ixx = c.ixx;
+            ixy = c.ixy;
+            iyy = c.iyy;
         }
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            cx += __shfl_xor(cx, d, 64);
-            cy += __shfl_xor(cy, d, 64);
-            j00 += __shfl_xor(j00, d, 64);
-            j01 += __shfl_xor(j01, d, 64);
-            j10 += __shfl_xor(j10, d, 64);
-            j11 += __shfl_xor(j11, d, 64);
+        float f[kMlpIn], ux, uy, qd[2];
+        float scale = mlp_features(dx, dy, s, M, ixx, ixy, iyy, t3, eps2f, inv_mroot, inv_ell, f,
+                                   ux, uy, qd);
+        scale = ok ? scale : 0.f;
+        // A far cell (the walk's rule, in FP64 on the same values) adds its
+        // quadrupole term here, at the centre, and its Jacobian moves the
+        // term to each member; a near cell's term every member added itself.
+        bool far = false;
+        if (ok) {
+            const double ex = c.cx - ox, ey = c.cy - oy;
+            far = fma(ex, ex, ey * ey) > thr;
         }
-        if (l == 0) {
-            u[0 * 64 + ks] = cx;
-            u[1 * 64 + ks] = cy;
-            u[2 * 64 + ks] = j00;
-            u[3 * 64 + ks] = j01;
-            u[4 * 64 + ks] = j10;
-            u[5 * 64 + ks] = j11;
+        if (far) {
+            float J[4];
+            quad_jacobian(dx, dy, ixx, ixy, iyy, eps2f, J);
+            j00 += J[0];
+            j01 += J[1];
+            j10 += J[2];
+            j11 += J[3];
+        }
+        sq[wv][l] = make_float4(far ? qd[0] : 0.f, far ? qd[1] : 0.f, f[2], 0.f);
+#pragma unroll
+        for (int h = 0; h < kMlpIn / 4; h++)
+            *reinterpret_cast<float4*>(f_all + kMlpIn * l + 4 * h) =
+                make_float4(f[4 * h], f[4 * h + 1], f[4 * h + 2], f[4 * h + 3]);
+        sr[wv][l] = make_float2(scale * ux, scale * uy);
+        __builtin_amdgcn_wave_barrier();
+        f32x4 x[4], y[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            x[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+            if (l < 16 * (kMlpIn / 4)) {
+                const float4 v = *reinterpret_cast<const float4*>(
+                    f_all + kMlpIn * (16 * q + col) + 4 * (l >> 4));
+                x[q] = f32x4{v.x, v.y, v.z, v.w};
+            }
+        }
+        mlp_forward_tiles<2>(x, w, sb, l, y);
+        mlp_forward_tiles<2>(x + 2, w, sb, l, y + 2);
+        // lanes 0-15: cells 16q + l, rotated back to (x, y)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float2 su_ = sr[wv][(16 * q + l) & 63];
+            const float4 qq = sq[wv][(16 * q + l) & 63];
+            if (l < 16) {
+                // (s/R) * output along (u, w), plus a far cell's quadrupole term
+                const float yu = fmaf(y[q].x, qq.z, qq.x), yw = fmaf(y[q].y, qq.z, qq.y);
+                cx += yu * su_.x - yw * su_.y;
+                cy += yu * su_.y + yw * su_.x;
+            }
         }
         __builtin_amdgcn_wave_barrier();
+        if (nxt.ks != ks) {  // the subgroup's last batch: its sums, in a fixed order
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                cx += __shfl_xor(cx, d, 64);
+                cy += __shfl_xor(cy, d, 64);
+                j00 += __shfl_xor(j00, d, 64);
+                j01 += __shfl_xor(j01, d, 64);
+                j10 += __shfl_xor(j10, d, 64);
+                j11 += __shfl_xor(j11, d, 64);
+            }
+            if (l == 0) {
+                u[0 * 64 + ks] = cx;
+                u[1 * 64 + ks] = cy;
+                u[2 * 64 + ks] = j00;
+                u[3 * 64 + ks] = j01;
+                u[4 * 64 + ks] = j10;
+                u[5 * 64 + ks] = j11;
+            }
+            cx = cy = j00 = j01 = j10 = j11 = 0.f;
+        }
+        cur = nxt;
+        nxt = nn;
     }
-    if ((uint32_t)l >= gn) return;
-    const uint32_t i = first + l;
-    const float4 bw = body_w[i];
+    __builtin_amdgcn_wave_barrier();
+    if (!mem) return;
     const int ks = __float_as_int(bw.z) & 63;
     // the body sits at b = (bw.x, bw.y) from the centre: d = COM - point moves
     // by -b, so a far cell's term there is Q - J b
     const float ax = u[0 * 64 + ks] - fmaf(u[2 * 64 + ks], bw.x, u[3 * 64 + ks] * bw.y);
     const float ay = u[1 * 64 + ks] - fmaf(u[4 * 64 + ks], bw.x, u[5 * 64 + ks] * bw.y);
-    const double2 a0 = acc[i];
     const double2 a = make_double2(a0.x + (double)ax, a0.y + (double)ay);
     if (KICK) {
         const double gdt = k->G_dt;
