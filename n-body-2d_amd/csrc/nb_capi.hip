@@ -69,7 +69,7 @@ bool is_dk(const nb_ctx* c) { return c->p.integrator == NB_INTEG_DK; }
 uint64_t tree_bytes(uint64_t icap, uint64_t nsys, bool mom3) {
     const uint64_t ng = (icap + 255) / 256;
     return (nsys + 4 * icap) * (sizeof(NodeRec) + (mom3 ? sizeof(float4) : 0)) +
-           icap * (2 * sizeof(uint32_t) + 2 + sizeof(uint4)) + kMaxLevel * ng * sizeof(uint32_t);
+           icap * (2 * sizeof(uint32_t) + 2 + 2 * sizeof(uint4)) + kMaxLevel * ng * sizeof(uint32_t);
 }
 
 // Third moments are kept for the learned far field's inputs only (contexts
@@ -84,6 +84,7 @@ void free_tree(nb_ctx* c) {
     dfree(c->b.inode_level);
     dfree(c->b.inode_mask);
     dfree(c->b.node_a);
+    dfree(c->b.inode_info);
     dfree(c->b.by_level);
     dfree(c->b.lvl_cnt);
     c->icap = 0;
@@ -98,6 +99,7 @@ int alloc_tree(nb_ctx* c, uint64_t icap) {
     NB_HIP(c, dalloc(c, &c->b.inode_level, icap));
     NB_HIP(c, dalloc(c, &c->b.inode_mask, icap));
     NB_HIP(c, dalloc(c, &c->b.node_a, icap));
+    NB_HIP(c, dalloc(c, &c->b.inode_info, icap));
     NB_HIP(c, dalloc(c, &c->b.by_level, icap));
     NB_HIP(c, dalloc(c, &c->b.lvl_cnt, kMaxLevel * ((icap + 255) / 256)));
     c->device_bytes = before + tree_bytes(icap, c->nsys, wants_mom3(c));

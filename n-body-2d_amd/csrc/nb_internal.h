@@ -141,6 +141,7 @@ struct Buffers {
     float4* mom3;          // [1 + 4*icap] third moments about the COM {T_xxx, T_xxy,
                            //     T_xyy, T_yyy} of each slot (learned far field only)
     uint32_t* node_a;      // [icap] first body of internal node p
+    uint4* inode_info;     // [icap] internal node p: {first body, last body, its wsq as two words}
     uint32_t* multi_list;  // [N/2 + 1] slots of multi-body leaves
     uint32_t* lvl_cnt;     // [kMaxLevel * ceil(icap/256)] level counts, then offsets
     uint4* by_level;       // [icap] internal nodes sorted by level: {id, slot, mask, 0}

@@ -47,10 +47,29 @@ __global__ __launch_bounds__(256) void k_tree_delta(const uint64_t* __restrict__
     cnt[i] = hi > prev ? (uint32_t)(hi - prev) : 0u;
 }
 
+// Writes a whole 64-byte record with four 16-byte stores, so every record
+// reaches memory as one full line (a record written in parts costs HBM
+// read-modify-writes of partial lines).
+__device__ __forceinline__ void store_rec(NodeRec* __restrict__ r, double cx, double cy, double m,
+                                          double wsq, int32_t c0, uint32_t mask, uint32_t bs,
+                                          uint32_t be, float ixx, float ixy, float iyy) {
+    uint4* w = reinterpret_cast<uint4*>(r);
+    w[0] = make_uint4(__double2loint(cx), __double2hiint(cx), __double2loint(cy),
+                      __double2hiint(cy));
+    w[1] = make_uint4(__double2loint(m), __double2hiint(m), __double2loint(wsq),
+                      __double2hiint(wsq));
+    w[2] = make_uint4((uint32_t)c0, mask, bs, be);
+    w[3] = make_uint4(__float_as_uint(ixx), __float_as_uint(ixy), __float_as_uint(iyy),
+                      __float_as_uint((float)m));
+}
+
 // Builds internal node p = (first body a, level L): its body range, the four
 // child ranges (binary searches on the digit, lower_bound_quadrant of
-// BarnesHutCuda.cu:153-163) and the child slots. Leaves get their COM here;
-// multi-body leaves (only possible at level 30) are queued for k_leaf_multi.
+// BarnesHutCuda.cu:153-163) and the child slots. Leaves get their whole
+// record here; multi-body leaves (only possible at level 30) are queued for
+// k_leaf_multi. An internal child only learns its slot (inode_slot): its
+// record is written once, whole, by the COM reduction, from the range and
+// wsq that its own thread keeps in inode_info.
 __device__ void build_node(uint32_t p, const uint64_t* __restrict__ keys, uint32_t nsys,
                            const uint32_t* __restrict__ sys_of,
                            const uint32_t* __restrict__ sys_off,
@@ -59,8 +78,9 @@ __device__ void build_node(uint32_t p, const uint64_t* __restrict__ keys, uint32
                            const double* __restrict__ box, const uint32_t* __restrict__ node_a,
                            NodeRec* __restrict__ nodes, uint32_t* __restrict__ inode_slot,
                            const uint8_t* __restrict__ inode_level,
-                           uint8_t* __restrict__ inode_mask, uint32_t* __restrict__ multi_list,
-                           uint32_t* __restrict__ meta, float4* __restrict__ mom3) {
+                           uint8_t* __restrict__ inode_mask, uint4* __restrict__ inode_info,
+                           uint32_t* __restrict__ multi_list, uint32_t* __restrict__ meta,
+                           float4* __restrict__ mom3) {
     const uint32_t a = node_a[p];
     const int L = inode_level[p];
     const uint32_t sys = sys_of[a];
@@ -114,46 +134,24 @@ __device__ void build_node(uint32_t p, const uint64_t* __restrict__ keys, uint32
         if (cs >= ce1) continue;  // empty: bit q of the mask stays clear
         mask |= 1u << q;
         const uint32_t ce = ce1 - 1;
-        if (cs == ce || L + 1 == kMaxLevel) {
-            r->wsq = cwsq;
-            r->c0 = -1;
-            r->mask = 0;
-            r->bs = (int32_t)cs;
-            r->be = (int32_t)ce;
-            if (cs == ce) {
-                const double2 x = pos[cs];
-                const double m = mass[cs];
-                r->cx = x.x;
-                r->cy = x.y;
-                r->m = m;
-                r->mf = (float)m;
-                r->ixx = 0.f;
-                r->ixy = 0.f;
-                r->iyy = 0.f;
-                if (mom3) mom3[nsys + 4ull * p + q] = make_float4(0.f, 0.f, 0.f, 0.f);
-            } else {
-                const uint32_t k = atomicAdd(&meta[META_MULTI_LEAVES], 1u);
-                multi_list[k] = (uint32_t)(nsys + 4ull * p + q);
-            }
+        if (cs == ce) {
+            const double2 x = pos[cs];
+            store_rec(r, x.x, x.y, mass[cs], cwsq, -1, 0u, cs, ce, 0.f, 0.f, 0.f);
+            if (mom3) mom3[nsys + 4ull * p + q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else if (L + 1 == kMaxLevel) {
+            store_rec(r, 0.0, 0.0, 0.0, cwsq, -1, 0u, cs, ce, 0.f, 0.f, 0.f);
+            const uint32_t k = atomicAdd(&meta[META_MULTI_LEAVES], 1u);
+            multi_list[k] = (uint32_t)(nsys + 4ull * p + q);
         } else {
             const int prev = cs ? (int)delta[cs - 1] : -1;
             const uint32_t cid = off[cs] + (uint32_t)(L + 1 - prev - 1);
-            r->wsq = cwsq;
-            r->c0 = (int32_t)(nsys + 4ull * cid);
-            r->bs = (int32_t)cs;
-            r->be = (int32_t)ce;
             inode_slot[cid] = (uint32_t)(nsys + 4ull * p + q);
         }
     }
     inode_mask[p] = (uint8_t)mask;
-    if (L == 0) {  // the root of system `sys` lives in slot `sys`
-        NodeRec* r = &nodes[sys];
-        r->wsq = root_wsq;
-        r->c0 = (int32_t)(nsys + 4ull * p);
-        r->bs = (int32_t)a;
-        r->be = (int32_t)b;
-        inode_slot[p] = sys;
-    }
+    const double wsq = ldexp(root_wsq, -2 * L);
+    inode_info[p] = make_uint4(a, b, __double2loint(wsq), __double2hiint(wsq));
+    if (L == 0) inode_slot[p] = sys;  // the root of system `sys` lives in slot `sys`
 }
 
 // Lists every internal node: pair i owns levels (delta_{i-1}, min(delta_i, 29)],
@@ -217,14 +215,14 @@ __global__ __launch_bounds__(256) void k_tree_nodes(
     const uint32_t* __restrict__ off, const double* __restrict__ box, uint32_t* __restrict__ meta,
     uint64_t icap, const uint32_t* __restrict__ node_a, NodeRec* __restrict__ nodes,
     uint32_t* __restrict__ inode_slot, const uint8_t* __restrict__ inode_level,
-    uint8_t* __restrict__ inode_mask, uint32_t* __restrict__ multi_list,
-    float4* __restrict__ mom3) {
+    uint8_t* __restrict__ inode_mask, uint4* __restrict__ inode_info,
+    uint32_t* __restrict__ multi_list, float4* __restrict__ mom3) {
     const uint32_t I = meta[META_INTERNAL];
     if (I > icap || (meta[META_FLAGS] & kHaltFlags)) return;
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     if (p >= I) return;
     build_node(p, keys, nsys, sys_of, sys_off, pos, mass, delta, off, box, node_a, nodes,
-               inode_slot, inode_level, inode_mask, multi_list, meta, mom3);
+               inode_slot, inode_level, inode_mask, inode_info, multi_list, meta, mom3);
 }
 
 // Multi-body leaf COM, one wave per leaf, in the order of compute_leaf_com
@@ -385,17 +383,6 @@ __device__ __forceinline__ float4 combine3(const Com ch[4], const float4 t[4], c
     return make_float4((float)txxx, (float)txxy, (float)txyy, (float)tyyy);
 }
 
-__device__ __forceinline__ void store_com(NodeRec* __restrict__ r, const Com& o, uint32_t mask) {
-    r->cx = o.cx;
-    r->cy = o.cy;
-    r->m = o.m;
-    r->mf = (float)o.m;
-    r->mask = (int32_t)mask;
-    r->ixx = o.ixx;
-    r->ixy = o.ixy;
-    r->iyy = o.iyy;
-}
-
 __device__ __forceinline__ bool tree_halted(const uint32_t* meta, uint64_t icap) {
     return meta[META_INTERNAL] > icap || (meta[META_FLAGS] & kHaltFlags);
 }
@@ -448,6 +435,7 @@ __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __rest
                                                    uint64_t icap, uint32_t nblk, uint32_t nsys,
                                                    const uint32_t* __restrict__ lvl_off,
                                                    const uint4* __restrict__ by_level,
+                                                   const uint4* __restrict__ inode_info,
                                                    NodeRec* __restrict__ nodes,
                                                    float4* __restrict__ mom3) {
     // the segment bounds are loaded alongside the halt check (in bounds either way)
@@ -457,6 +445,7 @@ __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __rest
     for (uint32_t k = b + blockIdx.x * 256 + threadIdx.x; k < e; k += gridDim.x * 256) {
         const uint4 en = by_level[k];  // node id, own slot, occupancy mask
         const uint32_t p = en.x, mask = en.z;
+        const uint4 info = inode_info[p];  // body range and wsq
         // an empty slot holds exact zeros: skip its load, combine() adds the same
         // zeros; the third moments are loaded with the rest (one round trip)
         Com ch[4];
@@ -468,7 +457,8 @@ __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __rest
             if (M3) t[q] = has ? mom3[nsys + 4ull * p + q] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
         const Com o = REF ? combine_ref(ch) : combine(ch);
-        store_com(&nodes[en.y], o, mask);
+        store_rec(&nodes[en.y], o.cx, o.cy, o.m, __hiloint2double((int)info.w, (int)info.z),
+                  (int32_t)(nsys + 4ull * p), mask, info.x, info.y, o.ixx, o.ixy, o.iyy);
         if (M3) mom3[en.y] = combine3(ch, t, o);
     }
 }
@@ -492,7 +482,7 @@ void launch_tree_build(nb_ctx* c, int src) {
                        c->b.sys_of, c->b.sys_off, c->b.pos[src], c->b.mass[src], c->b.delta,
                        c->b.cnt, c->b.box, c->b.meta, (uint64_t)c->icap, c->b.node_a,
                        c->b.nodes, c->b.inode_slot, c->b.inode_level, c->b.inode_mask,
-                       c->b.multi_list, c->b.mom3);
+                       c->b.inode_info, c->b.multi_list, c->b.mom3);
     hipLaunchKernelGGL(k_leaf_multi, dim3(64), dim3(256), 0, c->stream, c->b.pos[src],
                        c->b.mass[src], c->b.multi_list, c->b.meta, c->b.nodes, c->b.mom3);
     hipLaunchKernelGGL(k_level_hist, dim3(ng), dim3(256), 0, c->stream, c->b.meta,
@@ -509,7 +499,7 @@ void launch_tree_build(nb_ctx* c, int src) {
 #define NB_COM(R, M)                                                                           \
     hipLaunchKernelGGL((k_com_level<R, M>), dim3(cg), dim3(256), 0, c->stream, L, c->b.meta,   \
                        (uint64_t)c->icap, ng, c->nsys, c->b.lvl_cnt, c->b.by_level,            \
-                       c->b.nodes, c->b.mom3)
+                       c->b.inode_info, c->b.nodes, c->b.mom3)
     for (int L = kMaxLevel - 1; L >= 0; L--) {
         if (ref) {
             if (m3) NB_COM(true, true); else NB_COM(true, false);
