@@ -163,7 +163,7 @@ void free_all(nb_ctx* c) {
     dfree(b.ref_part);
     dfree(b.delta);
     dfree(b.cnt);
-    dfree(b.scan_tmp);
+    dfree(b.scan_status);
     dfree(b.multi_list);
     dfree(b.queue);
     dfree(b.sys_of);
@@ -585,10 +585,11 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
     }
     c->sort_tiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
     c->bbox_blocks = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((n + 2047) / 2048, 1), 1024);
-    // block sums of the widest scan: the level counts of the largest node pool
-    // (internal nodes are fewer than 30 per body pair)
-    const uint64_t scan_blocks = std::max<uint64_t>((n + 1023) / 1024,
-                                                    (uint64_t)kMaxLevel * (30 * n / 256 + 2) / 1024) + 2;
+    // published words of the widest scan (one per 1024 values, more than its
+    // tiles): the level counts of the largest node pool (internal nodes are
+    // fewer than 30 per body pair); zeroed once, as the sort's
+    const uint64_t scan_words = std::max<uint64_t>((n + 1023) / 1024,
+                                                   (uint64_t)kMaxLevel * (30 * n / 256 + 2) / 1024) + 2;
     // radix sort state: zeroed once, so no published word can match a sort's tag
     // (tiles + tile groups of 16) published words, and one histogram row per
     // Morton workgroup (kSortTile keys each)
@@ -620,7 +621,8 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
     }
     NB_TRY(dalloc(c, &b.delta, bh ? n : 0));
     NB_TRY(dalloc(c, &b.cnt, bh ? n : 0));
-    NB_TRY(dalloc(c, &b.scan_tmp, scan_blocks));
+    NB_TRY(dalloc(c, &b.scan_status, scan_words));
+    NB_TRY(hipMemsetAsync(b.scan_status, 0, scan_words * sizeof(uint64_t), c->stream));
     NB_TRY(dalloc(c, &b.multi_list, bh ? n / 2 + 1 : 0));
     NB_TRY(dalloc(c, &b.acc, n));
     NB_TRY(dalloc(c, &b.inter, n));

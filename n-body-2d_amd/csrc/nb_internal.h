@@ -37,6 +37,7 @@ constexpr int kSortMaxPasses = 10;     // 8 key passes + 2 system-number passes
 // sort_glob words: per-pass digit histograms, per-pass tile counters, epoch
 constexpr int kSortGlobTile = kSortMaxPasses * kSortBins;
 constexpr int kSortGlobEpoch = kSortGlobTile + 16;
+constexpr int kSortGlobScanEpoch = kSortGlobEpoch + 1;  // tag of launch_scan_u32's look-back
 constexpr int kSortGlobWords = kSortGlobEpoch + 16;
 constexpr int kBoxStride = 8;          // doubles of box data per system
 constexpr int kRefBoxTile = 2048;      // bodies per block of the REF_CPU box scan
@@ -133,7 +134,7 @@ struct Buffers {
     double4* ref_part;     // [ref blocks] {min x, min y, max x, max y} per block of it
     int8_t* delta;         // [N] common-prefix digit count of pair (i, i+1)
     uint32_t* cnt;         // [N] internal nodes owned by pair i, then its offset
-    uint32_t* scan_tmp;    // block sums of the scans
+    uint64_t* scan_status; // published tile totals / prefixes of launch_scan_u32
     uint32_t* inode_slot;  // [icap] slot of internal node p
     uint8_t* inode_level;  // [icap]
     uint8_t* inode_mask;   // [icap]
@@ -187,7 +188,6 @@ struct nb_ctx {
     uint32_t bbox_blocks = 0;
     uint32_t ref_blocks = 0;     // blocks of the REF_CPU box scan (kRefBoxTile bodies each)
     uint32_t sort_tiles = 0;
-    uint32_t scan_blocks = 0;
     uint32_t ap_split = 1;
     uint32_t ap_rows = 4;        // all-pairs bodies per thread (2, 4 or 8)
     uint32_t ap_split_max = 1;   // j splits the partial buffer holds

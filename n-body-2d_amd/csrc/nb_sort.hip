@@ -959,79 +959,126 @@ void launch_unpermute2(nb_ctx* c, double2* pos_out, double2* vel_out) {
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive in-place scan of u32 (reduce / top-level scan / downsweep).
+// Exclusive in-place scan of u32, one launch: each workgroup scans a tile of
+// kScanTile values, publishes its total, and reads the prefix of the earlier
+// tiles back from what they published (a decoupled look-back over one
+// column, 64 tiles per round trip, as the sort's). Published words (u64):
+// flag (1 = tile total, 2 = inclusive prefix) << 62 | tag << 32 | value, with
+// a tag that every scan advances (the last tile bumps it when it is done:
+// by then every tile of the scan has read it), so words of earlier scans
+// never match. Workgroups start in index order, so the lowest unfinished tile
+// always runs and every wait ends; the bounded spins flag FLAG_SORT instead
+// of hanging if that ever failed.
+// HBM traffic: 4 B read + 4 B written per value.
 // ---------------------------------------------------------------------------
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 4;
+constexpr int kScanItems = 16;  // consecutive values per thread
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ data,
-                                                              uint32_t count,
-                                                              uint32_t* __restrict__ sums) {
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-    uint32_t v = 0;
-#pragma unroll
-    for (int r = 0; r < kScanItems; r++) {
-        const uint64_t idx = base + (uint64_t)r * kScanThreads + threadIdx.x;
-        if (idx < count) v += data[idx];
-    }
-    v = wave_sum(v);
-    __shared__ uint32_t s[kScanThreads / 64];
-    if (lane_id() == 0) s[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = 0;
-        for (int k = 0; k < kScanThreads / 64; k++) t += s[k];
-        sums[blockIdx.x] = t;
-    }
-}
-
-__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* __restrict__ sums, uint32_t nb,
-                                                   uint32_t* __restrict__ total) {
-    __shared__ uint32_t s[1024 / 64 + 1];
-    uint32_t carry = 0;
-    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
-        const uint32_t k = b0 + threadIdx.x;
-        const uint32_t v = k < nb ? sums[k] : 0u;
-        uint32_t sum;
-        const uint32_t ex = block_excl_scan<1024>(v, s, &sum);
-        if (k < nb) sums[k] = carry + ex;
-        carry += sum;
-    }
-    if (threadIdx.x == 0) *total = carry;
-}
-
-__global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict__ data,
-                                                            uint32_t count,
-                                                            const uint32_t* __restrict__ sums) {
+__global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ data,
+                                                       uint32_t count,
+                                                       uint64_t* __restrict__ status,
+                                                       uint32_t* __restrict__ epoch,
+                                                       uint32_t* __restrict__ total_out,
+                                                       uint32_t* __restrict__ meta) {
     __shared__ uint32_t s[kScanThreads / 64 + 1];
-    // Blocked layout: thread t owns items t*kScanItems .. +kScanItems-1.
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    __shared__ uint32_t s_prefix;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tag = *epoch & 0x3fffffffu;
+    const uint64_t base = (uint64_t)tile * kScanTile + (uint64_t)threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
+    if (base + kScanItems <= count) {
+        const uint4* src = reinterpret_cast<const uint4*>(data + base);
+#pragma unroll
+        for (int r = 0; r < kScanItems / 4; r++) {
+            const uint4 x = src[r];
+            v[4 * r] = x.x;
+            v[4 * r + 1] = x.y;
+            v[4 * r + 2] = x.z;
+            v[4 * r + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < kScanItems; r++) v[r] = base + r < count ? data[base + r] : 0u;
+    }
     uint32_t tsum = 0;
 #pragma unroll
-    for (int r = 0; r < kScanItems; r++) {
-        v[r] = (base + r) < count ? data[base + r] : 0u;
-        tsum += v[r];
+    for (int r = 0; r < kScanItems; r++) tsum += v[r];
+    uint32_t agg;
+    const uint32_t ex = block_excl_scan<kScanThreads>(tsum, s, &agg);
+    if (threadIdx.x < 64) {
+        const int l = lane_id();
+        if (l == 0)
+            __hip_atomic_store(status + tile, os_word(tile == 0 ? 2u : 1u, tag, agg),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t excl = 0;
+        bool ok = true;
+        int64_t k = (int64_t)tile - 1;  // lane l reads tile k - l
+        uint32_t spins = 0;
+        while (k >= 0) {
+            const uint64_t w = k - l >= 0 ? __hip_atomic_load(status + (k - l), __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT)
+                                          : os_word(2u, tag, 0u);  // nothing before tile 0
+            const uint32_t f = (uint32_t)(w >> 62), wt = (uint32_t)(w >> 32) & 0x3fffffffu;
+            const bool ready = wt == tag && f != 0u;
+            const uint64_t notready = __ballot(!ready), incl = __ballot(ready && f == 2u);
+            const int first_nr = notready ? __ffsll((unsigned long long)notready) - 1 : 64;
+            const int first_in = incl ? __ffsll((unsigned long long)incl) - 1 : 64;
+            const int take = first_in < first_nr ? first_in + 1 : first_nr;  // lanes 0..take-1
+            excl += wave_sum(l < take ? (uint32_t)w : 0u);
+            if (first_in < first_nr) break;
+            k -= take;
+            if (take == 0) {
+                if (++spins > kOsSpinLimit) {
+                    ok = false;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        if (l == 0) {
+            if (tile > 0)
+                __hip_atomic_store(status + tile, os_word(2u, tag, excl + agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            if (!ok) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_SORT);  // never expected
+            s_prefix = excl;
+        }
     }
-    uint32_t total;
-    uint32_t run = sums[blockIdx.x] + block_excl_scan<kScanThreads>(tsum, s, &total);
+    __syncthreads();
+    uint32_t run = s_prefix + ex;
+    if (base + kScanItems <= count) {
+        uint4* dst = reinterpret_cast<uint4*>(data + base);
 #pragma unroll
-    for (int r = 0; r < kScanItems; r++) {
-        if ((base + r) < count) data[base + r] = run;
-        run += v[r];
+        for (int r = 0; r < kScanItems / 4; r++) {
+            uint4 x;
+            x.x = run;
+            run += v[4 * r];
+            x.y = run;
+            run += v[4 * r + 1];
+            x.z = run;
+            run += v[4 * r + 2];
+            x.w = run;
+            run += v[4 * r + 3];
+            dst[r] = x;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < kScanItems; r++) {
+            if (base + r < count) data[base + r] = run;
+            run += v[r];
+        }
+    }
+    if (tile == gridDim.x - 1 && threadIdx.x == 0) {
+        if (total_out) *total_out = s_prefix + agg;
+        *epoch += 1u;
     }
 }
 
 void launch_scan_u32(nb_ctx* c, uint32_t* data, uint32_t count, uint32_t* total_out) {
-    const uint32_t nb = (count + kScanTile - 1) / kScanTile;
-    const uint32_t g = nb ? nb : 1;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(g), dim3(kScanThreads), 0, c->stream, data, count,
-                       c->b.scan_tmp);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, c->stream, c->b.scan_tmp, g,
-                       total_out ? total_out : c->b.scan_tmp + g);
-    hipLaunchKernelGGL(k_scan_down, dim3(g), dim3(kScanThreads), 0, c->stream, data, count,
-                       c->b.scan_tmp);
+    const uint32_t tiles = (count + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan, dim3(tiles ? tiles : 1), dim3(kScanThreads), 0, c->stream, data,
+                       count, c->b.scan_status, c->b.sort_glob + kSortGlobScanEpoch, total_out,
+                       c->b.meta);
 }
 
 }  // namespace nb
