@@ -888,16 +888,21 @@ void launch_radix_sort(nb_ctx* c) {
 }
 
 // ---------------------------------------------------------------------------
-// Gather every body array through the permutation (BarnesHutCuda.cu:135-147).
-// HBM traffic: 4 B perm + 44 B read + 44 B written per body.
+// Gather every body array through the permutation (BarnesHutCuda.cu:135-147),
+// and, in the same launch, the first stage of the tree build on the sorted
+// keys: each pair's delta and internal-node count (tree_pair).
+// HBM traffic: 4 B perm + 44 B read + 44 B written per body, plus the pair's.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_gather(
     const uint32_t* __restrict__ perm, const double2* __restrict__ pin,
     const double2* __restrict__ vin, const double* __restrict__ min_,
     const uint32_t* __restrict__ oin, double2* __restrict__ pout, double2* __restrict__ vout,
-    double* __restrict__ mout, uint32_t* __restrict__ oout, uint32_t n) {
+    double* __restrict__ mout, uint32_t* __restrict__ oout, uint32_t n,
+    const uint64_t* __restrict__ keys, const uint32_t* __restrict__ sys_of,
+    int8_t* __restrict__ delta, uint32_t* __restrict__ cnt) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    if (i + 1 < n) tree_pair(i, keys, sys_of, delta, cnt);
     uint32_t s = perm[i];
     if (s >= n) s = i;  // only after a stalled sort (FLAG_SORT), whose step is void
     pout[i] = pin[s];
@@ -911,7 +916,8 @@ void launch_gather(nb_ctx* c, int src) {
     const int dst = src ^ 1;
     hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->b.vals[0],
                        c->b.pos[src], c->b.vel[src], c->b.mass[src], c->b.orig[src],
-                       c->b.pos[dst], c->b.vel[dst], c->b.mass[dst], c->b.orig[dst], n);
+                       c->b.pos[dst], c->b.vel[dst], c->b.mass[dst], c->b.orig[dst], n,
+                       c->b.keys[0], c->b.sys_of, c->b.delta, c->b.cnt);
 }
 
 // ---------------------------------------------------------------------------

@@ -23,29 +23,6 @@
 
 namespace nb {
 
-__device__ __forceinline__ int pair_delta(uint64_t a, uint64_t b) {
-    const uint64_t x = a ^ b;
-    if (x == 0) return kMaxLevel;
-    const int d = __clzll(x) >> 1;
-    return d < kMaxLevel ? d : kMaxLevel;
-}
-
-// Per pair: delta and the number of internal nodes it owns.
-// HBM traffic: 8 B keys read (x3 from cache), 5 B written per pair.
-__global__ __launch_bounds__(256) void k_tree_delta(const uint64_t* __restrict__ keys, uint32_t n,
-                                                    const uint32_t* __restrict__ sys_of,
-                                                    int8_t* __restrict__ delta,
-                                                    uint32_t* __restrict__ cnt) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i + 1 >= n) return;
-    const uint64_t k0 = keys[i], k1 = keys[i + 1];
-    const uint32_t s0 = sys_of[i];
-    const int d = s0 == sys_of[i + 1] ? pair_delta(k0, k1) : -1;
-    const int prev = (i && sys_of[i - 1] == s0) ? pair_delta(keys[i - 1], k0) : -1;
-    const int hi = d < kMaxLevel - 1 ? d : kMaxLevel - 1;
-    delta[i] = (int8_t)d;
-    cnt[i] = hi > prev ? (uint32_t)(hi - prev) : 0u;
-}
 
 // Writes a whole 64-byte record with four 16-byte stores, so every record
 // reaches memory as one full line (a record written in parts costs HBM
@@ -67,9 +44,9 @@ __device__ __forceinline__ void store_rec(NodeRec* __restrict__ r, double cx, do
 // child ranges (binary searches on the digit, lower_bound_quadrant of
 // BarnesHutCuda.cu:153-163) and the child slots. Leaves get their whole
 // record here; multi-body leaves (only possible at level 30) are queued for
-// k_leaf_multi. An internal child only learns its slot (inode_slot): its
-// record is written once, whole, by the COM reduction, from the range and
-// wsq that its own thread keeps in inode_info.
+// leaf_multi (run by the k_level_hist launch). An internal child only learns
+// its slot (inode_slot): its record is written once, whole, by the COM
+// reduction, from the range and wsq that its own thread keeps in inode_info.
 __device__ void build_node(uint32_t p, const uint64_t* __restrict__ keys, uint32_t nsys,
                            const uint32_t* __restrict__ sys_of,
                            const uint32_t* __restrict__ sys_off,
@@ -228,16 +205,18 @@ __global__ __launch_bounds__(256) void k_tree_nodes(
 // Multi-body leaf COM, one wave per leaf, in the order of compute_leaf_com
 // (BarnesHutCuda.cu:277-309): 128 strided partial sums (lane l holds slots l
 // and l+64), then the halving tree reduction. Second moments about the COM.
-__global__ __launch_bounds__(256) void k_leaf_multi(const double2* __restrict__ pos,
-                                                    const double* __restrict__ mass,
-                                                    const uint32_t* __restrict__ multi_list,
-                                                    const uint32_t* __restrict__ meta,
-                                                    NodeRec* __restrict__ nodes,
-                                                    float4* __restrict__ mom3) {
+// Workgroup `blk` of `nblk` (256 threads each).
+__device__ __forceinline__ void leaf_multi(uint32_t blk, uint32_t nblk,
+                                           const double2* __restrict__ pos,
+                                           const double* __restrict__ mass,
+                                           const uint32_t* __restrict__ multi_list,
+                                           const uint32_t* __restrict__ meta,
+                                           NodeRec* __restrict__ nodes,
+                                           float4* __restrict__ mom3) {
     const uint32_t count = meta[META_MULTI_LEAVES];
-    const uint32_t waves = gridDim.x * 4;
+    const uint32_t waves = nblk * 4;
     const int l = lane_id();
-    for (uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6); k < count; k += waves) {
+    for (uint32_t k = blk * 4 + (threadIdx.x >> 6); k < count; k += waves) {
         NodeRec* r = &nodes[multi_list[k]];
         const int32_t bs = r->bs, be = r->be;
         double M[2] = {0.0, 0.0}, X[2] = {0.0, 0.0}, Y[2] = {0.0, 0.0};
@@ -390,11 +369,22 @@ __device__ __forceinline__ bool tree_halted(const uint32_t* meta, uint64_t icap)
 // Counting sort of the internal nodes by level, so each level of the
 // bottom-up reduction touches only its own nodes: per-block level histograms
 // (LDS atomics), one exclusive scan over the level-major counts, then a
-// scatter. lvl_cnt has kMaxLevel rows of `nblk` counts.
+// scatter. lvl_cnt has kMaxLevel rows of `nblk` counts. The same launch runs
+// the multi-body leaves (independent work) in kMultiBlocks more workgroups.
+constexpr uint32_t kMultiBlocks = 64;
 __global__ __launch_bounds__(256) void k_level_hist(const uint32_t* __restrict__ meta,
-                                                    uint64_t icap,
+                                                    uint64_t icap, uint32_t nblk,
                                                     const uint8_t* __restrict__ inode_level,
-                                                    uint32_t* __restrict__ lvl_cnt) {
+                                                    uint32_t* __restrict__ lvl_cnt,
+                                                    const double2* __restrict__ pos,
+                                                    const double* __restrict__ mass,
+                                                    const uint32_t* __restrict__ multi_list,
+                                                    NodeRec* __restrict__ nodes,
+                                                    float4* __restrict__ mom3) {
+    if (blockIdx.x >= nblk) {
+        leaf_multi(blockIdx.x - nblk, kMultiBlocks, pos, mass, multi_list, meta, nodes, mom3);
+        return;
+    }
     __shared__ uint32_t h[kMaxLevel];
     if (threadIdx.x < kMaxLevel) h[threadIdx.x] = 0;
     __syncthreads();
@@ -403,7 +393,7 @@ __global__ __launch_bounds__(256) void k_level_hist(const uint32_t* __restrict__
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     if (ok && p < I) atomicAdd(&h[inode_level[p]], 1u);
     __syncthreads();
-    if (threadIdx.x < kMaxLevel) lvl_cnt[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+    if (threadIdx.x < kMaxLevel) lvl_cnt[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
 // Each entry carries what the level's reduction needs besides the child
@@ -466,12 +456,8 @@ __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __rest
 void launch_tree_build(nb_ctx* c, int src) {
     const uint32_t n = (uint32_t)c->n;
     const uint32_t pairs = n > 1 ? n - 1 : 0;
-    const uint32_t pg = pairs ? (pairs + 255) / 256 : 1;
     const uint32_t bg = (n + 255) / 256;
-    if (pairs) {
-        hipLaunchKernelGGL(k_tree_delta, dim3(pg), dim3(256), 0, c->stream, c->b.keys[0], n,
-                           c->b.sys_of, c->b.delta, c->b.cnt);
-    }
+    // the pair deltas and counts were made by the gather (k_gather)
     launch_scan_u32(c, c->b.cnt, pairs, &c->b.meta[META_INTERNAL]);
     hipLaunchKernelGGL(k_tree_enum, dim3(bg), dim3(256), 0, c->stream, c->b.delta, c->b.cnt, n,
                        c->b.sys_of, c->b.sys_off, c->b.meta, (uint64_t)c->icap, c->b.node_a,
@@ -483,10 +469,9 @@ void launch_tree_build(nb_ctx* c, int src) {
                        c->b.cnt, c->b.box, c->b.meta, (uint64_t)c->icap, c->b.node_a,
                        c->b.nodes, c->b.inode_slot, c->b.inode_level, c->b.inode_mask,
                        c->b.inode_info, c->b.multi_list, c->b.mom3);
-    hipLaunchKernelGGL(k_leaf_multi, dim3(64), dim3(256), 0, c->stream, c->b.pos[src],
-                       c->b.mass[src], c->b.multi_list, c->b.meta, c->b.nodes, c->b.mom3);
-    hipLaunchKernelGGL(k_level_hist, dim3(ng), dim3(256), 0, c->stream, c->b.meta,
-                       (uint64_t)c->icap, c->b.inode_level, c->b.lvl_cnt);
+    hipLaunchKernelGGL(k_level_hist, dim3(ng + kMultiBlocks), dim3(256), 0, c->stream, c->b.meta,
+                       (uint64_t)c->icap, ng, c->b.inode_level, c->b.lvl_cnt, c->b.pos[src],
+                       c->b.mass[src], c->b.multi_list, c->b.nodes, c->b.mom3);
     launch_scan_u32(c, c->b.lvl_cnt, (uint32_t)(kMaxLevel * ng), nullptr);
     hipLaunchKernelGGL(k_level_scatter, dim3(ng), dim3(256), 0, c->stream, c->b.meta,
                        (uint64_t)c->icap, c->b.inode_level, c->b.inode_mask, c->b.inode_slot,
