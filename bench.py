@@ -254,6 +254,14 @@ def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
     info = ctx.info()
     leg["interactions"] = total
     leg["interactions_per_body"] = total / n
+    if traversal == abi.TRAV_GROUP:
+        # the per-body opening test's interactions on the same state (EXACT
+        # traversal with monopoles, the reference's sets at this theta): what
+        # the group test adds
+        with nbody2d.Context(mass, pos, vel, G=1.0, dt=DT, theta=theta, softening=EPS,
+                             traversal=abi.TRAV_EXACT, self_guard=guard) as ex:
+            _, _, total_exact = ex.forces()
+        leg["interactions_per_body_exact"] = total_exact / n
     leg["tree_nodes"] = int(info.tree_nodes)
     leg["internal_nodes"] = int(info.internal_nodes)
     if learned:
@@ -448,6 +456,15 @@ def roofline_walk(leg, traversal_name):
     }
     if learned:
         out["quad_terms_per_launch"] = quad
+    if leg.get("interactions_per_body_exact"):
+        # the reference's per-body sets hold fewer interactions than the
+        # group's shared lists (ratio on the initial state): the rate of the
+        # interactions the per-body test would have listed
+        r = leg["interactions_per_body_exact"] / leg["interactions_per_body"]
+        out["exact_over_group_interactions"] = r
+        out["achieved_exact_equivalent"] = (
+            leg["interactions_per_step_timed"] * r * FLOPS_PER_INTERACTION
+            / (force_ms * 1e-3) / 1e12)
     return out
 
 

@@ -31,6 +31,17 @@ def test_roofline_uses_the_timed_launches():
     assert r["interactions_per_launch"] == 7.5e8 and r["avg_launch_ms"] == 0.6
 
 
+def test_roofline_reports_the_per_body_test_equivalent():
+    """With the EXACT traversal's interaction count on the same state, the
+    roofline also gives the rate of the interactions the per-body test keeps."""
+    leg = _leg(interactions_per_body=720.0, interactions_per_body_exact=380.0)
+    r = bench.roofline_walk(leg, "group")
+    assert r["exact_over_group_interactions"] == pytest.approx(380 / 720)
+    assert r["achieved_exact_equivalent"] == pytest.approx(
+        7.5e8 * 380 / 720 * 14 / 0.6e-3 / 1e12)
+    assert "exact_over_group_interactions" not in bench.roofline_walk(_leg(), "group")
+
+
 def test_learned_roofline_counts_near_cell_quadrupole_terms():
     """The learned walk's flops: 14 per interaction plus 22 for each term that
     also adds a near cell's quadrupole term at the body."""
