@@ -418,8 +418,9 @@ void launch_bbox(nb_ctx* c, int src) {
 // Morton kernels count the digits of every pass while they produce the keys
 // (per workgroup in LDS, one row per workgroup), k_os_reduce adds the rows.
 // ---------------------------------------------------------------------------
-constexpr int kKeyItems = 16;  // keys per thread of the Morton kernels
-constexpr int kKeyTile = 256 * kKeyItems;
+constexpr int kKeyThreads = 1024;  // threads of a Morton workgroup (4 wavefronts per SIMD)
+constexpr int kKeyItems = 4;       // keys per thread of the Morton kernels
+constexpr int kKeyTile = kKeyThreads * kKeyItems;
 
 // Digit of pass p: key passes take bits [8p, 8p + 8) of the key; the system
 // passes (p >= 8) take bits [8(p - 8), ...) of the system of the key's
@@ -522,7 +523,7 @@ __device__ __forceinline__ uint32_t quantise(double v, double lo, double inv) {
     return (uint32_t)floor(s);
 }
 
-__global__ __launch_bounds__(256) void k_morton(const double2* __restrict__ pos, uint32_t n,
+__global__ __launch_bounds__(kKeyThreads) void k_morton(const double2* __restrict__ pos, uint32_t n,
                                                 const double* __restrict__ box,
                                                 const uint32_t* __restrict__ sys_of,
                                                 uint64_t* __restrict__ keys,
@@ -535,14 +536,14 @@ __global__ __launch_bounds__(256) void k_morton(const double2* __restrict__ pos,
     uint32_t ss[kKeyItems];
 #pragma unroll
     for (int r = 0; r < kKeyItems; r++) {
-        const uint32_t i = blockIdx.x * kKeyTile + r * 256 + threadIdx.x;
+        const uint32_t i = blockIdx.x * kKeyTile + r * kKeyThreads + threadIdx.x;
         pp[r] = i < n ? pos[i] : make_double2(0.0, 0.0);
         ss[r] = i < n ? sys_of[i] : 0u;
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kKeyItems; r++) {
-        const uint32_t i = blockIdx.x * kKeyTile + r * 256 + threadIdx.x;
+        const uint32_t i = blockIdx.x * kKeyTile + r * kKeyThreads + threadIdx.x;
         const bool valid = i < n;
         uint64_t key = 0;
         if (valid) {
@@ -587,7 +588,7 @@ __device__ __forceinline__ uint64_t ref_key(double2 p, const double* __restrict_
     return key;
 }
 
-__global__ __launch_bounds__(256) void k_morton_ref(const double2* __restrict__ pos, uint32_t n,
+__global__ __launch_bounds__(kKeyThreads) void k_morton_ref(const double2* __restrict__ pos, uint32_t n,
                                                     const double* __restrict__ box,
                                                     uint64_t* __restrict__ keys,
                                                     uint32_t* __restrict__ vals,
@@ -597,13 +598,13 @@ __global__ __launch_bounds__(256) void k_morton_ref(const double2* __restrict__ 
     double2 pp[kKeyItems];
 #pragma unroll
     for (int r = 0; r < kKeyItems; r++) {
-        const uint32_t i = blockIdx.x * kKeyTile + r * 256 + threadIdx.x;
+        const uint32_t i = blockIdx.x * kKeyTile + r * kKeyThreads + threadIdx.x;
         pp[r] = i < n ? pos[i] : make_double2(0.0, 0.0);
     }
     __syncthreads();
 #pragma unroll 2
     for (int r = 0; r < kKeyItems; r++) {
-        const uint32_t i = blockIdx.x * kKeyTile + r * 256 + threadIdx.x;
+        const uint32_t i = blockIdx.x * kKeyTile + r * kKeyThreads + threadIdx.x;
         const bool valid = i < n;
         uint64_t key = 0;
         if (valid) {
@@ -621,10 +622,10 @@ void launch_morton(nb_ctx* c, int src) {
     const uint32_t grid = (n + kKeyTile - 1) / kKeyTile;
     const int npass = 64 / kSortBits + (c->nsys > 1 ? 2 : 0);
     if (c->p.cells == NB_CELLS_REF_CPU)
-        hipLaunchKernelGGL(k_morton_ref, dim3(grid), dim3(256), 0, c->stream, c->b.pos[src], n,
+        hipLaunchKernelGGL(k_morton_ref, dim3(grid), dim3(kKeyThreads), 0, c->stream, c->b.pos[src], n,
                            c->b.box, c->b.keys[0], c->b.vals[0], c->b.sort_part);
     else
-        hipLaunchKernelGGL(k_morton, dim3(grid), dim3(256), 0, c->stream, c->b.pos[src], n,
+        hipLaunchKernelGGL(k_morton, dim3(grid), dim3(kKeyThreads), 0, c->stream, c->b.pos[src], n,
                            c->b.box, c->b.sys_of, c->b.keys[0], c->b.vals[0], npass,
                            c->b.sort_part);
     hipLaunchKernelGGL(k_os_reduce, dim3(npass, kOsSlices), dim3(kSortBins), 0, c->stream,
@@ -736,7 +737,7 @@ __device__ __forceinline__ bool os_sum_counts(const uint64_t* __restrict__ statu
     return true;
 }
 
-// One pass: 512 threads (8 waves) per tile of kSortTile keys, tile =
+// One pass: 1024 threads (16 waves) per tile of kSortTile keys, tile =
 // workgroup index. Workgroups are dispatched in index order (per XCD), so
 // the lowest unfinished tile is always running and every wait below ends;
 // the bounded spins only guard against that assumption failing, and then
@@ -746,7 +747,7 @@ __device__ __forceinline__ bool os_sum_counts(const uint64_t* __restrict__ statu
 // the groups before it are looked back over (counts / inclusive prefixes
 // published by each group's last tile): about two round trips after the
 // counts appear, however many tiles there are.
-constexpr int kOsThreads = 512;
+constexpr int kOsThreads = 1024;
 constexpr int kOsItems = kSortTile / kOsThreads;
 static_assert(kOsThreads * kOsItems == kSortTile, "tile size");
 
