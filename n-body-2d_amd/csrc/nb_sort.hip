@@ -812,13 +812,31 @@ __global__ __launch_bounds__(kOsThreads) void k_os_pass(
     uint32_t total;
     const uint32_t toff = block_excl_scan<kOsThreads>(cnt, s_scan, &total);
     if (owner) s_off[t] = toff;
-    // the tile's exclusive prefix over the earlier tiles, digit by digit
+    // the tile's exclusive prefix over the earlier tiles, digit by digit: the
+    // counts are published first, and the scan of the global counts and the
+    // reordering of the tile in LDS (neither needs the prefix) run while
+    // they travel to the later tiles
     const uint32_t tag = (glob[kSortGlobEpoch] * 16u + (uint32_t)pass) & 0x3fffffffu;
     const uint32_t g = tile / kOsGroup, j = tile % kOsGroup;
-    uint32_t excl = 0;
-    if (owner) {
+    if (owner)
         __hip_atomic_store(status + (uint64_t)tile * kSortBins + t, os_word(1u, tag, cnt),
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t gtotal;
+    const uint32_t gex =
+        block_excl_scan<kOsThreads>(owner ? glob[pass * kSortBins + t] : 0u, s_scan, &gtotal);
+    // the tile in digit order in LDS (s_off and the wave offsets in wc are
+    // visible after the scan's barriers)
+#pragma unroll
+    for (int r = 0; r < kOsItems; r++) {
+        const uint32_t k = (uint32_t)(w * kWaveKeys + r * 64 + l);
+        if (k < nvalid) {
+            const uint32_t slot = s_off[dig[r]] + wc[w][dig[r]] + rank[r];
+            s_key[slot] = key[r];
+            s_val[slot] = val[r];
+            s_dig[slot] = dig[r];
+        }
+    }
+    if (owner) {
         uint32_t within = 0, groups = 0;
         bool ok = os_sum_counts(status, g * kOsGroup, j, (uint32_t)t, tag, &within);
         uint64_t* gmine = gstatus + (uint64_t)g * kSortBins + t;
@@ -831,25 +849,10 @@ __global__ __launch_bounds__(kOsThreads) void k_os_pass(
             __hip_atomic_store(gmine, os_word(2u, tag, groups + within + cnt), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         if (!ok) atomicOr(&meta[META_FLAGS], (uint32_t)FLAG_SORT);  // never expected
-        excl = groups + within;
-    }
-    uint32_t gtotal;
-    const uint32_t gex =
-        block_excl_scan<kOsThreads>(owner ? glob[pass * kSortBins + t] : 0u, s_scan, &gtotal);
-    if (owner) s_dst[t] = gex + excl;
-    __syncthreads();
-    // the tile in digit order in LDS, then runs of equal digits written out
-#pragma unroll
-    for (int r = 0; r < kOsItems; r++) {
-        const uint32_t k = (uint32_t)(w * kWaveKeys + r * 64 + l);
-        if (k < nvalid) {
-            const uint32_t slot = s_off[dig[r]] + wc[w][dig[r]] + rank[r];
-            s_key[slot] = key[r];
-            s_val[slot] = val[r];
-            s_dig[slot] = dig[r];
-        }
+        s_dst[t] = gex + groups + within;
     }
     __syncthreads();
+    // runs of equal digits written out
 #pragma unroll
     for (int r = 0; r < kOsItems; r++) {
         const uint32_t slot = (uint32_t)r * kOsThreads + t;
