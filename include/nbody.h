@@ -277,6 +277,15 @@ int nb_debug_build(nb_ctx* ctx, double* box4, uint64_t* sorted_keys, uint32_t* o
  * NB_ERR_OUT_OF_MEMORY. */
 int nb_debug_fail_alloc(nb_ctx* ctx, uint32_t count);
 
+/* Test hook: the tree's mass/COM reduction takes the levels from `com_deep`
+ * to 29 in one single-workgroup launch and the levels above one launch each.
+ * A context picks com_deep from its first tree (3 + the deepest level holding
+ * internal nodes) and picks it again, capturing its steps again, when the
+ * single-workgroup launch met a level of more than 2,048 internal nodes.
+ * `set` in 0..30 replaces it (-1 leaves it); any value gives the same tree.
+ * *current (if not NULL) receives the value in use afterwards. */
+int nb_debug_com_deep(nb_ctx* ctx, int32_t set, int32_t* current);
+
 /* Copies the tree built by the last step or nb_debug_build. */
 int nb_debug_tree(nb_ctx* ctx, nb_node* nodes, uint64_t cap);
 

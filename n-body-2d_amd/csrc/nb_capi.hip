@@ -357,6 +357,20 @@ int read_meta(nb_ctx* c) {
     return NB_OK;
 }
 
+// The first level of the COM reduction's one-workgroup launch: kComMargin
+// levels past the deepest non-empty level of the trees read back (those and
+// the levels above stay one launch each). Picked from the first tree, and
+// again (the steps captured again, ~ms) only when that launch met a level of
+// more than kComRefit nodes, i.e. the trees grew deeper than the cut.
+int fit_com_deep(nb_ctx* c, bool first) {
+    if (!is_bh(c) || (c->host_meta[META_FLAGS] & kHaltFlags)) return NB_OK;
+    if (!first && c->host_meta[META_COM_DEEP_MAX] <= (uint32_t)kComRefit) return NB_OK;
+    const int deep = (int)std::min<uint32_t>(c->host_meta[META_COM_HI] + kComMargin, kMaxLevel);
+    if (deep == c->com_deep) return NB_OK;
+    c->com_deep = deep;
+    return c->graph[0] ? build_graphs(c) : NB_OK;
+}
+
 // A context whose regrowth failed: the step graphs are gone and buffers
 // they used are freed or missing, so nothing may launch on it any more.
 int mark_dead(nb_ctx* c, int rc) {
@@ -413,6 +427,8 @@ int build_and_force(nb_ctx* c, bool forces) {
         }
         NB_HIP(c, hipGetLastError());
         rc = read_meta(c);
+        if (rc) return rc;
+        rc = fit_com_deep(c, false);
         if (rc) return rc;
         const uint32_t flags = c->host_meta[META_FLAGS];
         c->last_internal = c->host_meta[META_INTERNAL];
@@ -678,6 +694,8 @@ int create_impl(const nb_params* p, uint64_t n, uint32_t nsys, const uint64_t* o
         const bool learned = c->p.far_field == NB_FAR_LEARNED;
         rc = build_and_force(c, learned);
         if (rc) return bail(rc);
+        rc = fit_com_deep(c, true);  // from the first tree (no steps captured yet)
+        if (rc) return bail(rc);
         if (c->last_internal + c->last_internal / 4 > c->icap) {
             rc = alloc_tree(c, c->last_internal + c->last_internal / 4 + 1024);
             if (rc) return bail(rc);
@@ -746,6 +764,8 @@ int nb_step(nb_ctx* c, uint64_t k) {
             }
         }
         rc = read_meta(c);
+        if (rc) return rc;
+        rc = fit_com_deep(c, false);
         if (rc) return rc;
         const uint32_t flags = c->host_meta[META_FLAGS];
         const uint64_t done = std::min<uint64_t>(c->host_meta[META_STEPS_DONE], remaining);
@@ -902,6 +922,20 @@ int nb_compute_forces(nb_ctx* c, double* acc_xy, uint32_t* interactions, uint64_
 int nb_debug_fail_alloc(nb_ctx* c, uint32_t count) {
     if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
     c->fail_allocs = count;
+    return NB_OK;
+}
+
+int nb_debug_com_deep(nb_ctx* c, int32_t set, int32_t* current) {
+    if (!c) return fail(nullptr, NB_ERR_INVALID_ARG, "ctx is NULL");
+    if (set < -1 || set > kMaxLevel) return fail(c, NB_ERR_INVALID_ARG, "level out of range");
+    if (set >= 0 && set != c->com_deep) {
+        if (int rc = check_alive(c)) return rc;
+        c->com_deep = set;
+        if (c->graph[0]) {
+            if (int rc = build_graphs(c)) return rc;
+        }
+    }
+    if (current) *current = c->com_deep;
     return NB_OK;
 }
 

@@ -27,6 +27,8 @@ namespace nb {
 
 constexpr int kWave = 64;
 constexpr int kMaxLevel = 30;          // level-30 nodes are leaves (BarnesHutCuda.cu:182)
+constexpr int kComMargin = 2;          // levels past the deepest non-empty one kept as launches
+constexpr int kComRefit = 2048;        // a deeper level of more nodes has the cut picked again
 constexpr int kStackDepth = 128;       // >= 3*30+1, the deepest masked DFS stack
 constexpr int kSortBits = 8;           // radix digit width
 constexpr int kSortBins = 1 << kSortBits;
@@ -59,6 +61,8 @@ enum Meta : int {
     META_CELLS_NEEDED = 4, // most accepted cells of one group (learned far field)
     META_CELLS_TOTAL = 32, // accepted cells of all groups (learned far field), u64 in 32..33
     META_QUAD_TERMS = 34,  // near-cell quadrupole terms applied per body, u64 in 34..35
+    META_COM_HI = 40,        // 1 + deepest non-empty tree level, the most of the steps since cleared
+    META_COM_DEEP_MAX = 41,  // most nodes of one level the one-workgroup COM launch took
     META_QHEAD = 64,       // traversal work queue: entries pushed
     META_QTAIL = 96,       // traversal work queue: entries taken
     META_COUNT = 128
@@ -193,6 +197,9 @@ struct nb_ctx {
     uint32_t ap_split_max = 1;   // j splits the partial buffer holds
     uint64_t device_bytes = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // COM reduction: levels >= com_deep in one single-workgroup launch, the
+    // others one launch each (fixed in the captured steps)
+    int com_deep = nb::kMaxLevel;
     bool timing = false;
     hipEvent_t ev[2 * NB_PHASE_COUNT] = {};  // start/stop per phase
     uint32_t marked = 0;                     // phases recorded in the current step

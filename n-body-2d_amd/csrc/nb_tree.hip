@@ -418,8 +418,38 @@ __global__ __launch_bounds__(256) void k_level_scatter(const uint32_t* __restric
         make_uint4(p, inode_slot[p], (uint32_t)inode_mask[p], 0u);
 }
 
-// The internal nodes of one level: segment [lvl_off[L * nblk], lvl_off[(L+1) * nblk])
-// of by_level (the last level ends at I). REF: the reference CPU order.
+// One internal node of the reduction (by_level entry: node id, own slot,
+// occupancy mask). An empty slot holds exact zeros: its load is skipped and
+// combine() adds the same zeros; the third moments are loaded with the rest
+// (one round trip). REF: the reference CPU order.
+template <bool REF, bool M3>
+__device__ __forceinline__ void com_node(const uint4 en, uint32_t nsys,
+                                         const uint4* __restrict__ inode_info,
+                                         NodeRec* __restrict__ nodes, float4* __restrict__ mom3) {
+    const uint32_t p = en.x, mask = en.z;
+    const uint4 info = inode_info[p];  // body range and wsq
+    Com ch[4];
+    float4 t[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const bool has = (mask >> q) & 1u;
+        ch[q] = has ? load_com(&nodes[nsys + 4ull * p + q]) : Com{0.0, 0.0, 0.0, 0.f, 0.f, 0.f};
+        if (M3) t[q] = has ? mom3[nsys + 4ull * p + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    const Com o = REF ? combine_ref(ch) : combine(ch);
+    store_rec(&nodes[en.y], o.cx, o.cy, o.m, __hiloint2double((int)info.w, (int)info.z),
+              (int32_t)(nsys + 4ull * p), mask, info.x, info.y, o.ixx, o.ixy, o.iyy);
+    if (M3) mom3[en.y] = combine3(ch, t, o);
+}
+
+// The internal nodes of level L: segment [lvl_off[L * nblk], lvl_off[(L+1) * nblk])
+// of by_level (the last level ends at I).
+__device__ __forceinline__ uint32_t level_start(const uint32_t* __restrict__ lvl_off,
+                                                const uint32_t* __restrict__ meta, uint32_t nblk,
+                                                int L) {
+    return L < kMaxLevel ? lvl_off[(uint64_t)L * nblk] : meta[META_INTERNAL];
+}
+
 template <bool REF, bool M3>
 __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __restrict__ meta,
                                                    uint64_t icap, uint32_t nblk, uint32_t nsys,
@@ -429,27 +459,53 @@ __global__ __launch_bounds__(256) void k_com_level(int L, const uint32_t* __rest
                                                    NodeRec* __restrict__ nodes,
                                                    float4* __restrict__ mom3) {
     // the segment bounds are loaded alongside the halt check (in bounds either way)
-    const uint32_t b = lvl_off[(uint64_t)L * nblk];
-    const uint32_t e = L + 1 < kMaxLevel ? lvl_off[(uint64_t)(L + 1) * nblk] : meta[META_INTERNAL];
+    const uint32_t b = level_start(lvl_off, meta, nblk, L);
+    const uint32_t e = level_start(lvl_off, meta, nblk, L + 1);
     if (tree_halted(meta, icap)) return;
-    for (uint32_t k = b + blockIdx.x * 256 + threadIdx.x; k < e; k += gridDim.x * 256) {
-        const uint4 en = by_level[k];  // node id, own slot, occupancy mask
-        const uint32_t p = en.x, mask = en.z;
-        const uint4 info = inode_info[p];  // body range and wsq
-        // an empty slot holds exact zeros: skip its load, combine() adds the same
-        // zeros; the third moments are loaded with the rest (one round trip)
-        Com ch[4];
-        float4 t[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const bool has = (mask >> q) & 1u;
-            ch[q] = has ? load_com(&nodes[nsys + 4ull * p + q]) : Com{0.0, 0.0, 0.0, 0.f, 0.f, 0.f};
-            if (M3) t[q] = has ? mom3[nsys + 4ull * p + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (uint32_t k = b + blockIdx.x * 256 + threadIdx.x; k < e; k += gridDim.x * 256)
+        com_node<REF, M3>(by_level[k], nsys, inode_info, nodes, mom3);
+}
+
+// The deepest levels, from com_deep to 29, in one launch: one workgroup
+// reduces them level after level with a barrier in between (a level's
+// children are the nodes of the level below, written by this workgroup).
+// com_deep is set a few levels below the deepest non-empty level of the
+// context's trees, so these levels are empty or hold a few nodes (pairs of
+// bodies closer than the deeper cells), and one launch replaces a launch per
+// level. The launch records 1 + the deepest non-empty level and the most
+// nodes of a level it took: a tree that grew so deep that this workgroup took
+// a level of more than kComRefit nodes (still reduced, in more rounds) has
+// the host pick com_deep again.
+template <bool REF, bool M3>
+__global__ __launch_bounds__(1024) void k_com_deep(int com_deep, uint32_t* __restrict__ meta,
+                                                   uint64_t icap, uint32_t nblk, uint32_t nsys,
+                                                   const uint32_t* __restrict__ lvl_off,
+                                                   const uint4* __restrict__ by_level,
+                                                   const uint4* __restrict__ inode_info,
+                                                   NodeRec* __restrict__ nodes,
+                                                   float4* __restrict__ mom3) {
+    __shared__ uint32_t s_start[kMaxLevel + 1];
+    if (tree_halted(meta, icap)) return;
+    const int t = threadIdx.x;
+    if (t <= kMaxLevel) s_start[t] = level_start(lvl_off, meta, nblk, t);
+    __syncthreads();
+    const int last = max(com_deep, 0);
+    if (t == 0) {
+        int deepest = 0;
+        uint32_t most = 0;
+        for (int L = 0; L < kMaxLevel; L++) {
+            const uint32_t cnt = s_start[L + 1] - s_start[L];
+            if (cnt) deepest = L + 1;
+            if (L >= last) most = max(most, cnt);
         }
-        const Com o = REF ? combine_ref(ch) : combine(ch);
-        store_rec(&nodes[en.y], o.cx, o.cy, o.m, __hiloint2double((int)info.w, (int)info.z),
-                  (int32_t)(nsys + 4ull * p), mask, info.x, info.y, o.ixx, o.ixy, o.iyy);
-        if (M3) mom3[en.y] = combine3(ch, t, o);
+        atomicMax(&meta[META_COM_HI], (uint32_t)deepest);
+        atomicMax(&meta[META_COM_DEEP_MAX], most);
+    }
+    for (int L = kMaxLevel - 1; L >= last; L--) {
+        const uint32_t b = s_start[L], e = s_start[L + 1];
+        for (uint32_t k = b + t; k < e; k += 1024)
+            com_node<REF, M3>(by_level[k], nsys, inode_info, nodes, mom3);
+        __syncthreads();
     }
 }
 
@@ -476,16 +532,25 @@ void launch_tree_build(nb_ctx* c, int src) {
     hipLaunchKernelGGL(k_level_scatter, dim3(ng), dim3(256), 0, c->stream, c->b.meta,
                        (uint64_t)c->icap, c->b.inode_level, c->b.inode_mask, c->b.inode_slot,
                        c->b.lvl_cnt, c->b.by_level);
-    // One launch per level, deepest first (a level's children are all done
-    // when its launch starts).
+    // Deepest first (a level's children are all done when it starts): the
+    // levels >= com_deep in one launch, then one launch per level below.
     const uint32_t cg = ng < 512 ? ng : 512;
     const bool ref = c->p.cells == NB_CELLS_REF_CPU;
     const bool m3 = c->b.mom3 != nullptr;
 #define NB_COM(R, M)                                                                           \
-    hipLaunchKernelGGL((k_com_level<R, M>), dim3(cg), dim3(256), 0, c->stream, L, c->b.meta,   \
-                       (uint64_t)c->icap, ng, c->nsys, c->b.lvl_cnt, c->b.by_level,            \
-                       c->b.inode_info, c->b.nodes, c->b.mom3)
-    for (int L = kMaxLevel - 1; L >= 0; L--) {
+    do {                                                                                       \
+        if (L == kMaxLevel)                                                                    \
+            hipLaunchKernelGGL((k_com_deep<R, M>), dim3(1), dim3(1024), 0, c->stream,          \
+                               c->com_deep, c->b.meta, (uint64_t)c->icap, ng, c->nsys,         \
+                               c->b.lvl_cnt, c->b.by_level, c->b.inode_info, c->b.nodes,       \
+                               c->b.mom3);                                                     \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_com_level<R, M>), dim3(cg), dim3(256), 0, c->stream, L,      \
+                               c->b.meta, (uint64_t)c->icap, ng, c->nsys, c->b.lvl_cnt,        \
+                               c->b.by_level, c->b.inode_info, c->b.nodes, c->b.mom3);         \
+    } while (0)
+    for (int L = kMaxLevel; L >= 0; L--) {
+        if (L < kMaxLevel && L >= c->com_deep) continue;  // the one-workgroup launch's
         if (ref) {
             if (m3) NB_COM(true, true); else NB_COM(true, false);
         } else {

@@ -46,7 +46,7 @@ EXPORTS = (
     "nb_mlp_generate", "nb_mlp_get_pairs", "nb_mlp_set_pairs", "nb_mlp_train", "nb_mlp_grad",
     "nb_mlp_forward", "nb_debug_far_field", "nb_create_batch", "nb_autotune",
     "nb_snapshot_begin", "nb_snapshot_query", "nb_snapshot_wait", "nb_debug_fail_alloc",
-    "nb_direct_sum_f64_noalloc",
+    "nb_direct_sum_f64_noalloc", "nb_debug_com_deep",
 )
 NOALLOC_SCRATCH_BYTES = 16 << 20
 MLP_PARAMS = 5122
@@ -137,6 +137,7 @@ def lib() -> C.CDLL:
                                          C.POINTER(C.c_uint32)]
         L.nb_autotune.argtypes = [P, C.c_uint32, C.POINTER(Tuning)]
         L.nb_debug_fail_alloc.argtypes = [P, C.c_uint32]
+        L.nb_debug_com_deep.argtypes = [P, C.c_int32, C.POINTER(C.c_int32)]
         L.nb_direct_sum_f64_noalloc.argtypes = [C.c_uint64, P, P, C.c_double, C.c_double,
                                                 C.c_uint64, P, P, P]
         L.nb_snapshot_begin.argtypes = [P, C.POINTER(C.c_uint64)]
@@ -284,6 +285,13 @@ class Context:
     def debug_fail_alloc(self, count: int) -> None:
         """Test hook: the next `count` device allocations of the context fail."""
         self._check(lib().nb_debug_fail_alloc(self._h, int(count)))
+
+    def debug_com_deep(self, set_to: int = -1) -> int:
+        """Test hook: the first tree level that the COM reduction's one-workgroup
+        launch takes; `set_to` in 0..30 replaces it. Returns the value in use."""
+        cur = C.c_int32()
+        self._check(lib().nb_debug_com_deep(self._h, int(set_to), C.byref(cur)))
+        return int(cur.value)
 
     def debug_tree(self, count: int):
         nodes = np.zeros(count, dtype=NODE_DTYPE)

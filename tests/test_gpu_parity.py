@@ -134,6 +134,37 @@ def test_tree_com_large_disk():
     _assert_same_tree(g, O.tree_build(keys[perm], p[perm], m[perm], root_wsq))
 
 
+@pytest.mark.parametrize("deep", [0, 12, 30])
+def test_tree_com_deep_level_launch(deep):
+    """The levels from com_deep down are reduced by one workgroup, the rest one
+    launch per level: every split gives the oracle's tree bit for bit (0: the
+    one workgroup takes every level, including levels of ~40,000 nodes; 30:
+    none) and the same states over graph-replayed steps. A context picks
+    com_deep = 3 + the deepest level holding internal nodes, and picks it
+    again when the one workgroup met a level of more than 2,048 nodes."""
+    m, p, v = ics.exp_disk(1 << 18, seed=5)
+    _, keys, perm, root_wsq = O.prepare(p, m)
+    r = O.tree_build(keys[perm], p[perm], m[perm], root_wsq)
+    counts = np.bincount(r["level"][r["c0"] >= 0], minlength=30)
+    fitted = min(np.flatnonzero(counts).max() + 3, 30)
+    refit = bool((counts[deep:] > 2048).any())
+    with _ctx(m, p, v) as c:
+        assert c.debug_com_deep() == fitted
+        assert c.debug_com_deep(deep) == deep
+        _, _, _, cnt = c.debug_build()
+        _assert_same_tree(c.debug_tree(cnt), r)
+        assert c.debug_com_deep() == (fitted if refit else deep)
+        # graph-replayed steps with the hook's value give the same states
+        c.debug_com_deep(deep)
+        c.step(3)
+        got = c.state()
+    with _ctx(m, p, v) as c:
+        c.step(3)
+        want = c.state()
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
 def test_tree_com_many_systems_deep_chains():
     """260 systems, each a cluster 1e-6 across plus one far body: every system
     has a ~20-level chain of single-child nodes above a cluster of ~600
