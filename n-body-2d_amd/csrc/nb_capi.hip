@@ -112,6 +112,10 @@ void destroy_graphs(nb_ctx* c) {
         if (g) (void)hipGraphExecDestroy(g);
         g = nullptr;
     }
+    for (auto& g : c->graph4) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
 }
 
 void free_snapshots(nb_ctx* c) {
@@ -309,6 +313,12 @@ int enqueue_step(nb_ctx* c, int src) {
     return s;
 }
 
+// Steps are replayed from captured graphs: one step from either copy of the
+// body arrays, and four steps in one graph (which ends on the copy it started
+// from), so that a multi-step call pays the gap between two graph replays
+// (~8 us on MI355X) once per four steps.
+constexpr int kGraphSteps = 4;
+
 int build_graphs(nb_ctx* c) {
     destroy_graphs(c);
     if (!c->p.use_graph || c->n == 0) return NB_OK;
@@ -316,17 +326,21 @@ int build_graphs(nb_ctx* c) {
     c->timing = false;
     const int variants = is_bh(c) ? 2 : 1;
     for (int v = 0; v < variants; v++) {
-        hipGraph_t g = nullptr;
-        NB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_step(c, v);
-        const hipError_t le = hipGetLastError();
-        NB_HIP(c, hipStreamEndCapture(c->stream, &g));
-        NB_HIP(c, le);
-        const hipError_t ie = hipGraphInstantiate(&c->graph[v], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        NB_HIP(c, ie);
-        // upload now, not at the first launch of the step
-        NB_HIP(c, hipGraphUpload(c->graph[v], c->stream));
+        for (int multi = 0; multi < 2; multi++) {
+            hipGraph_t g = nullptr;
+            NB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int s = v;
+            for (int k = 0; k < (multi ? kGraphSteps : 1); k++) s = enqueue_step(c, s);
+            const hipError_t le = hipGetLastError();
+            NB_HIP(c, hipStreamEndCapture(c->stream, &g));
+            NB_HIP(c, le);
+            hipGraphExec_t* dst = multi ? &c->graph4[v] : &c->graph[v];
+            const hipError_t ie = hipGraphInstantiate(dst, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            NB_HIP(c, ie);
+            // upload now, not at the first launch of the step
+            NB_HIP(c, hipGraphUpload(*dst, c->stream));
+        }
     }
     c->timing = timing;
     return NB_OK;
@@ -742,7 +756,10 @@ int nb_step(nb_ctx* c, uint64_t k) {
         int rc = clear_meta(c);
         if (rc) return rc;
         for (uint64_t s = 0; s < remaining; s++) {
-            if (c->timing) {
+            if (!c->timing && c->graph4[c->cur] && remaining - s >= (uint64_t)kGraphSteps) {
+                NB_HIP(c, hipGraphLaunch(c->graph4[c->cur], c->stream));  // ends on c->cur
+                s += kGraphSteps - 1;
+            } else if (c->timing) {
                 c->marked = 0;
                 const int dst = enqueue_step(c, c->cur);
                 NB_HIP(c, hipGetLastError());

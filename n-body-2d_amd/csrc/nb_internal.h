@@ -196,7 +196,8 @@ struct nb_ctx {
     uint32_t ap_rows = 4;        // all-pairs bodies per thread (2, 4 or 8)
     uint32_t ap_split_max = 1;   // j splits the partial buffer holds
     uint64_t device_bytes = 0;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph[2] = {nullptr, nullptr};   // one step from copy v
+    hipGraphExec_t graph4[2] = {nullptr, nullptr};  // four steps from copy v (ending on v)
     // COM reduction: levels >= com_deep in one single-workgroup launch, the
     // others one launch each (fixed in the captured steps)
     int com_deep = nb::kMaxLevel;

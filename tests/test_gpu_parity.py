@@ -276,19 +276,23 @@ def test_trajectory_matches_oracle_model():
         np.percentile(d, 50), np.percentile(d, 99))
 
 
-def test_graph_eager_timing_identical():
+@pytest.mark.parametrize("integrator", ["kdk", "dk"])
+def test_graph_eager_timing_identical(integrator):
+    """Graph-replayed steps (9 = two four-step graphs and a one-step graph),
+    eager steps and timed eager steps give the same states bit for bit."""
     m, p, v = ics.exp_disk(3000, seed=6)
+    kw0 = {"integrator": nbody2d.abi.INTEG_DK} if integrator == "dk" else {}
     res = []
     for kw in ({"use_graph": 1}, {"use_graph": 0}):
-        with _ctx(m, p, v, **kw) as c:
-            c.step(3)
+        with _ctx(m, p, v, **kw0, **kw) as c:
+            c.step(9)
             res.append(c.state())
-    with _ctx(m, p, v) as c:
+    with _ctx(m, p, v, **kw0) as c:
         c.set_timing(True)
-        c.step(3)
+        c.step(9)
         tm, steps = c.timing()
         res.append(c.state())
-    assert steps == 3 and tm["force"] > 0.0
+    assert steps == 9 and tm["force"] > 0.0
     for r in res[1:]:
         assert np.array_equal(r[0], res[0][0]) and np.array_equal(r[1], res[0][1])
 
