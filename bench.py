@@ -269,11 +269,16 @@ def run_leg(nbody2d, ics, mass, pos, vel, theta, learned, traversal, args):
         leg["far_field_cells_per_group"] = info.far_field_cells / ((n + 63) // 64)
 
     ctx.step(args.warmup)
+    before = ctx.info()
     t0 = time.perf_counter()
     ctx.step(args.steps)
     wall = time.perf_counter() - t0
+    after = ctx.info()
     leg["body_steps_per_s"] = n * args.steps / wall
     leg["ms_per_step"] = wall * 1e3 / args.steps
+    # buffer regrowths and graph captures inside the timed call (each ~ms)
+    leg["regrowths_timed"] = int(after.regrowths - before.regrowths)
+    leg["graph_captures_timed"] = int(after.graph_captures - before.graph_captures)
     print(f"[bench] {leg['far_field']} theta={theta}: {leg['ms_per_step']:.3f} ms/step",
           flush=True)
 

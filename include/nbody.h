@@ -157,6 +157,11 @@ typedef struct nb_info {
                                  LEARNED): body terms that also added the cell's
                                  softened quadrupole term at the body (the
                                  cells near each traversal subgroup) */
+    uint64_t regrowths;       /* buffers regrown after a step overflowed one
+                                 (the step halted and ran again), since creation */
+    uint64_t graph_captures;  /* times the steps were captured as graphs (at
+                                 creation, after a regrowth, a tuning, or a new
+                                 cut of the tree's deepest levels; ~ms each) */
 } nb_info;
 
 /* Timing phases reported by nb_get_timing (milliseconds, summed over the

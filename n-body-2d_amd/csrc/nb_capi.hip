@@ -322,6 +322,7 @@ constexpr int kGraphSteps = 4;
 int build_graphs(nb_ctx* c) {
     destroy_graphs(c);
     if (!c->p.use_graph || c->n == 0) return NB_OK;
+    c->graph_captures++;
     const bool timing = c->timing;
     c->timing = false;
     const int variants = is_bh(c) ? 2 : 1;
@@ -402,6 +403,7 @@ int check_alive(nb_ctx* c) {
 // must not leave a graph that a later step could replay.
 int grow_for(nb_ctx* c, uint32_t flags) {
     destroy_graphs(c);
+    c->regrowths++;
     if (flags & FLAG_TREE_CAPACITY) {
         const uint64_t icap = c->last_internal + c->last_internal / 4 + 1024;
         int rc = alloc_tree(c, icap);
@@ -974,6 +976,8 @@ int nb_get_info(nb_ctx* c, nb_info* info) {
     info->device_bytes = c->device_bytes;
     info->far_field_cells = c->last_cells;
     info->quad_terms = c->last_quad;
+    info->regrowths = c->regrowths;
+    info->graph_captures = c->graph_captures;
     return NB_OK;
 }
 

@@ -201,6 +201,8 @@ struct nb_ctx {
     // COM reduction: levels >= com_deep in one single-workgroup launch, the
     // others one launch each (fixed in the captured steps)
     int com_deep = nb::kMaxLevel;
+    uint64_t regrowths = 0;       // buffer regrowths after a halted step
+    uint64_t graph_captures = 0;  // build_graphs calls that captured steps
     bool timing = false;
     hipEvent_t ev[2 * NB_PHASE_COUNT] = {};  // start/stop per phase
     uint32_t marked = 0;                     // phases recorded in the current step

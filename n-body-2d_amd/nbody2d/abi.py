@@ -71,6 +71,7 @@ class Info(C.Structure):
         ("tree_nodes", C.c_uint64), ("internal_nodes", C.c_uint64),
         ("node_capacity", C.c_uint64), ("device_bytes", C.c_uint64),
         ("far_field_cells", C.c_uint64), ("quad_terms", C.c_uint64),
+        ("regrowths", C.c_uint64), ("graph_captures", C.c_uint64),
     ]
 
 

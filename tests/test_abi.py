@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     assert C.sizeof(abi.Params) == 4 * 8 + 16 * 4  # 15 int32 fields + padding to 8
-    assert C.sizeof(abi.Info) == 9 * 8
+    assert C.sizeof(abi.Info) == 11 * 8
     assert C.sizeof(abi.Tuning) == 4 * 4 + 2 * 8 + 4 * 4
     assert abi.NODE_DTYPE.itemsize == 4 * 4 + 4 * 8 + 4 * 4 + 4 * 4
 
