@@ -330,9 +330,11 @@ int build_graphs(nb_ctx* c) {
         for (int multi = 0; multi < 2; multi++) {
             hipGraph_t g = nullptr;
             NB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            c->capturing = true;
             int s = v;
             for (int k = 0; k < (multi ? kGraphSteps : 1); k++) s = enqueue_step(c, s);
             const hipError_t le = hipGetLastError();
+            c->capturing = false;
             NB_HIP(c, hipStreamEndCapture(c->stream, &g));
             NB_HIP(c, le);
             hipGraphExec_t* dst = multi ? &c->graph4[v] : &c->graph[v];
@@ -759,6 +761,7 @@ int nb_step(nb_ctx* c, uint64_t k) {
         if (rc) return rc;
         for (uint64_t s = 0; s < remaining; s++) {
             if (!c->timing && c->graph4[c->cur] && remaining - s >= (uint64_t)kGraphSteps) {
+                ensure_box_parts(c, c->cur);
                 NB_HIP(c, hipGraphLaunch(c->graph4[c->cur], c->stream));  // ends on c->cur
                 s += kGraphSteps - 1;
             } else if (c->timing) {
@@ -775,6 +778,7 @@ int nb_step(nb_ctx* c, uint64_t k) {
                 c->steps_timed++;
                 c->cur = dst;
             } else if (c->graph[c->cur]) {
+                ensure_box_parts(c, c->cur);
                 NB_HIP(c, hipGraphLaunch(c->graph[c->cur], c->stream));
                 if (is_bh(c)) c->cur ^= 1;
             } else {

@@ -84,4 +84,51 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nblocks) {
     return start + k;
 }
 
+// Min/max of x and y over a workgroup of `threads` (a multiple of 64; at
+// most 1024); the result is left in thread 0's arguments. Every thread of the
+// workgroup must call it.
+template <int threads>
+__device__ __forceinline__ void box_block_reduce(double& xmin, double& xmax, double& ymin,
+                                                 double& ymax) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        xmin = fmin(xmin, __shfl_xor(xmin, d, 64));
+        xmax = fmax(xmax, __shfl_xor(xmax, d, 64));
+        ymin = fmin(ymin, __shfl_xor(ymin, d, 64));
+        ymax = fmax(ymax, __shfl_xor(ymax, d, 64));
+    }
+    if (threads == 64) return;
+    __shared__ double s[4][threads / 64];
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        s[0][w] = xmin;
+        s[1][w] = xmax;
+        s[2][w] = ymin;
+        s[3][w] = ymax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < threads / 64; k++) {
+            xmin = fmin(xmin, s[0][k]);
+            xmax = fmax(xmax, s[1][k]);
+            ymin = fmin(ymin, s[2][k]);
+            ymax = fmax(ymax, s[3][k]);
+        }
+    }
+}
+
+// The same, stored by thread 0 as part[4 b + 0..3] = xmin, xmax, ymin, ymax
+// for b = blockIdx.x.
+template <int threads>
+__device__ __forceinline__ void box_block_store(double xmin, double xmax, double ymin,
+                                                double ymax, double* __restrict__ part) {
+    box_block_reduce<threads>(xmin, xmax, ymin, ymax);
+    if (threadIdx.x == 0) {
+        part[4 * blockIdx.x + 0] = xmin;
+        part[4 * blockIdx.x + 1] = xmax;
+        part[4 * blockIdx.x + 2] = ymin;
+        part[4 * blockIdx.x + 3] = ymax;
+    }
+}
+
 }  // namespace nb

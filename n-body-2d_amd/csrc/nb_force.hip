@@ -5,6 +5,7 @@
 //                 BarnesHutCuda.cu:382-486, re-designed for wave64)
 //   k_allpairs    LDS-tiled direct sum (AllPairsSim, AllPairs.cpp:38-49)
 //   k_drift       x += v dt (update_positions, BarnesHutCuda.cu:489-495)
+//   k_drift_box   the same, plus the next step's box partials (one system)
 //   k_direct64    FP64 direct sum, the reference for force errors
 //
 // Precision: the cell offset dx = com - x, its square d^2 and the opening
@@ -963,8 +964,65 @@ __global__ __launch_bounds__(256) void k_drift(double2* __restrict__ pos,
     pos[i] = x;
 }
 
+// The drift of a one-system Barnes-Hut context, which also leaves the box
+// partials of the new positions for the next step's box (one per workgroup,
+// bbox_blocks workgroups striding over the bodies, as k_bbox_partial does).
+// A halted step leaves the positions, and so their partials, as they were.
+// HBM traffic: 32 B read + 16 B written per body.
+constexpr int kDriftRounds = 8;  // bodies per thread loaded before the first update
+
+__global__ __launch_bounds__(kBboxThreads) void k_drift_box(double2* __restrict__ pos,
+                                                            const double2* __restrict__ vel,
+                                                            uint32_t n,
+                                                            const StepConsts* __restrict__ k,
+                                                            uint32_t* __restrict__ meta,
+                                                            int check, int count_step,
+                                                            double* __restrict__ part) {
+    if (check && (meta[META_FLAGS] & kHaltFlags)) return;
+    if (count_step && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&meta[META_STEPS_DONE], 1u);
+    const double dt = k->dt;
+    const uint64_t stride = (uint64_t)gridDim.x * kBboxThreads;
+    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    for (uint64_t base = (uint64_t)blockIdx.x * kBboxThreads + threadIdx.x; base < n;
+         base += kDriftRounds * stride) {
+        double2 x[kDriftRounds], v[kDriftRounds];
+#pragma unroll
+        for (int r = 0; r < kDriftRounds; r++) {
+            const uint64_t i = base + r * stride;
+            if (i < n) {
+                x[r] = pos[i];
+                v[r] = vel[i];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kDriftRounds; r++) {
+            const uint64_t i = base + r * stride;
+            if (i < n) {
+                x[r].x += v[r].x * dt;
+                x[r].y += v[r].y * dt;
+                pos[i] = x[r];
+                xmin = fmin(xmin, x[r].x);
+                xmax = fmax(xmax, x[r].x);
+                ymin = fmin(ymin, x[r].y);
+                ymax = fmax(ymax, x[r].y);
+            }
+        }
+    }
+    box_block_store<kBboxThreads>(xmin, xmax, ymin, ymax, part);
+}
+
 void launch_drift(nb_ctx* c, int src, bool check, bool count_step) {
     const uint32_t n = (uint32_t)c->n;
+    if (box_from_drift(c)) {
+        hipLaunchKernelGGL(k_drift_box, dim3(c->bbox_blocks), dim3(kBboxThreads), 0, c->stream,
+                           c->b.pos[src], c->b.vel[src], n, c->b.consts, c->b.meta, (int)check,
+                           (int)count_step, c->b.bbox_part);
+        // a halted drift skips both the update and the partials (which then
+        // still describe the unchanged positions); a captured drift runs at
+        // the replays, which ensure_box_parts precedes
+        if (!c->capturing) c->box_parts_ok = true;
+        return;
+    }
     hipLaunchKernelGGL(k_drift, dim3(n ? (n + 255) / 256 : 1), dim3(256), 0, c->stream,
                        c->b.pos[src], c->b.vel[src], n, c->b.consts, c->b.meta, (int)check,
                        (int)count_step);

@@ -42,6 +42,7 @@ constexpr int kSortGlobEpoch = kSortGlobTile + 16;
 constexpr int kSortGlobScanEpoch = kSortGlobEpoch + 1;  // tag of launch_scan_u32's look-back
 constexpr int kSortGlobWords = kSortGlobEpoch + 16;
 constexpr int kBoxStride = 8;          // doubles of box data per system
+constexpr int kBboxThreads = 256;      // threads of the box partial kernels (and the box-writing drift)
 constexpr int kRefBoxTile = 2048;      // bodies per block of the REF_CPU box scan
 // Group traversal: list entries after which a group is split and walked again
 // (nb_params.group_budget = 0). Splits restart walks from the root, so a low
@@ -201,6 +202,13 @@ struct nb_ctx {
     // COM reduction: levels >= com_deep in one single-workgroup launch, the
     // others one launch each (fixed in the captured steps)
     int com_deep = nb::kMaxLevel;
+    // The drift also writes the box partials (bbox_part) of the positions it
+    // makes, so a step's box needs only the final reduction. box_parts_ok: the
+    // partials describe pos[cur] (a gather permutes the same positions, and a
+    // halted drift leaves them unchanged). capturing: steps are being recorded
+    // into graphs, whose replays are preceded by ensure_box_parts.
+    bool box_parts_ok = false;
+    bool capturing = false;
     uint64_t regrowths = 0;       // buffer regrowths after a halted step
     uint64_t graph_captures = 0;  // build_graphs calls that captured steps
     bool timing = false;
@@ -272,6 +280,10 @@ void launch_tree_build(nb_ctx* c, int src);
 void launch_traverse(nb_ctx* c, int src, bool kick, bool count_step);
 void launch_allpairs(nb_ctx* c, int src, bool kick, bool count_step);
 void launch_drift(nb_ctx* c, int src, bool check, bool count_step);
+// Whether the drift writes the box partials (one system, Morton cells).
+bool box_from_drift(const nb_ctx* c);
+// Computes the box partials of pos[src] (the state) unless they are current.
+void ensure_box_parts(nb_ctx* c, int src);
 // ---- far-field network (nb_mlp.hip) ----
 void free_mlp(nb_ctx* c);
 int prepare_mlp(nb_ctx* c);

@@ -20,8 +20,6 @@ namespace nb {
 // then a single-block kernel squares the box about the shorter axis.
 // HBM traffic: 16 B/body read.
 // ---------------------------------------------------------------------------
-constexpr int kBboxThreads = 256;
-
 __global__ __launch_bounds__(kBboxThreads) void k_bbox_partial(const double2* __restrict__ pos,
                                                                uint32_t n,
                                                                double* __restrict__ part) {
@@ -34,34 +32,7 @@ __global__ __launch_bounds__(kBboxThreads) void k_bbox_partial(const double2* __
         ymin = fmin(ymin, p.y);
         ymax = fmax(ymax, p.y);
     }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        xmin = fmin(xmin, __shfl_xor(xmin, d, 64));
-        xmax = fmax(xmax, __shfl_xor(xmax, d, 64));
-        ymin = fmin(ymin, __shfl_xor(ymin, d, 64));
-        ymax = fmax(ymax, __shfl_xor(ymax, d, 64));
-    }
-    __shared__ double s[4][kBboxThreads / 64];
-    const int w = threadIdx.x >> 6;
-    if (lane_id() == 0) {
-        s[0][w] = xmin;
-        s[1][w] = xmax;
-        s[2][w] = ymin;
-        s[3][w] = ymax;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < kBboxThreads / 64; k++) {
-            xmin = fmin(xmin, s[0][k]);
-            xmax = fmax(xmax, s[1][k]);
-            ymin = fmin(ymin, s[2][k]);
-            ymax = fmax(ymax, s[3][k]);
-        }
-        part[4 * blockIdx.x + 0] = xmin;
-        part[4 * blockIdx.x + 1] = xmax;
-        part[4 * blockIdx.x + 2] = ymin;
-        part[4 * blockIdx.x + 3] = ymax;
-    }
+    box_block_store<kBboxThreads>(xmin, xmax, ymin, ymax, part);
 }
 
 // Clears the radix sort's global histograms and tile counters and advances
@@ -79,25 +50,21 @@ __device__ __forceinline__ double root_size_sq(double w, int diag) {
     return diag ? s2 + s2 : s2;
 }
 
-__global__ __launch_bounds__(64) void k_bbox_final(const double* __restrict__ part, uint32_t nparts,
-                                                   double* __restrict__ box,
-                                                   uint32_t* __restrict__ meta, int diag,
-                                                   uint32_t* __restrict__ sort_glob) {
-    sort_glob_reset(sort_glob, threadIdx.x, 64);
+__global__ __launch_bounds__(kBboxThreads) void k_bbox_final(const double* __restrict__ part,
+                                                             uint32_t nparts,
+                                                             double* __restrict__ box,
+                                                             uint32_t* __restrict__ meta,
+                                                             int diag,
+                                                             uint32_t* __restrict__ sort_glob) {
+    sort_glob_reset(sort_glob, threadIdx.x, kBboxThreads);
     double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
-    for (uint32_t k = threadIdx.x; k < nparts; k += 64) {
+    for (uint32_t k = threadIdx.x; k < nparts; k += kBboxThreads) {
         xmin = fmin(xmin, part[4 * k + 0]);
         xmax = fmax(xmax, part[4 * k + 1]);
         ymin = fmin(ymin, part[4 * k + 2]);
         ymax = fmax(ymax, part[4 * k + 3]);
     }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        xmin = fmin(xmin, __shfl_xor(xmin, d, 64));
-        xmax = fmax(xmax, __shfl_xor(xmax, d, 64));
-        ymin = fmin(ymin, __shfl_xor(ymin, d, 64));
-        ymax = fmax(ymax, __shfl_xor(ymax, d, 64));
-    }
+    box_block_reduce<kBboxThreads>(xmin, xmax, ymin, ymax);
     if (threadIdx.x == 0) {
         // Square about the midpoint of the shorter axis (BarnesHutCuda.cu:665-677).
         const double xr = xmax - xmin, yr = ymax - ymin;
@@ -387,6 +354,18 @@ __global__ __launch_bounds__(64) void k_refbox_final(const double4* __restrict__
     }
 }
 
+bool box_from_drift(const nb_ctx* c) {
+    return c->nsys == 1 && c->p.cells != NB_CELLS_REF_CPU &&
+           c->p.algorithm == NB_ALGO_BARNES_HUT;
+}
+
+void ensure_box_parts(nb_ctx* c, int src) {
+    if (!box_from_drift(c) || c->box_parts_ok || c->n == 0) return;
+    hipLaunchKernelGGL(k_bbox_partial, dim3(c->bbox_blocks), dim3(kBboxThreads), 0, c->stream,
+                       c->b.pos[src], (uint32_t)c->n, c->b.bbox_part);
+    c->box_parts_ok = true;
+}
+
 void launch_bbox(nb_ctx* c, int src) {
     const int diag = c->p.mac == NB_MAC_DIAG;
     if (c->p.cells == NB_CELLS_REF_CPU) {
@@ -407,9 +386,15 @@ void launch_bbox(nb_ctx* c, int src) {
                            c->b.sort_glob);
         return;
     }
-    hipLaunchKernelGGL(k_bbox_partial, dim3(c->bbox_blocks), dim3(kBboxThreads), 0, c->stream,
-                       c->b.pos[src], (uint32_t)c->n, c->b.bbox_part);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, c->stream, c->b.bbox_part,
+    if (box_from_drift(c)) {
+        // the previous drift wrote the partials (graph replays are preceded
+        // by ensure_box_parts, so a capture records only the final reduction)
+        if (!c->capturing) ensure_box_parts(c, src);
+    } else {
+        hipLaunchKernelGGL(k_bbox_partial, dim3(c->bbox_blocks), dim3(kBboxThreads), 0,
+                           c->stream, c->b.pos[src], (uint32_t)c->n, c->b.bbox_part);
+    }
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBboxThreads), 0, c->stream, c->b.bbox_part,
                        c->bbox_blocks, c->b.box, c->b.meta, diag, c->b.sort_glob);
 }
 
