@@ -57,6 +57,46 @@ def test_morton_sort_box_bit_exact(name):
     assert np.array_equal(orig, perm)
 
 
+def _paired_grid(n=2048, dt=1e-3):
+    """G = 0 grid whose bodies drift into pairs 1e-9 apart in one step: the
+    pairs' deep single-child chains overflow the first node pool, so a step
+    halts and is run again after the pool grows."""
+    side = int(np.ceil(np.sqrt(n)))
+    g = np.stack(np.meshgrid(np.arange(side), np.arange(side)), -1).reshape(-1, 2).astype(float)
+    p = g[:n].copy()
+    target = p.copy()
+    target[1::2] = target[0::2] + 1e-9
+    return np.ones(n), p, (target - p) / dt
+
+
+@pytest.mark.parametrize("integrator", ["kdk", "dk"])
+@pytest.mark.parametrize("case", ["disk", "regrowth"])
+def test_box_after_steps_is_the_states_box(integrator, case):
+    """The drift leaves the box partials of the positions it makes, and the
+    next step's box reduces them: after graph-replayed and single steps, and
+    after a step that halted and ran again on a larger node pool, the box and
+    the sorted keys are the oracle's for the state read back."""
+    kw = {"integrator": nbody2d.abi.INTEG_DK} if integrator == "dk" else {}
+    if case == "disk":
+        m, p, v = ics.exp_disk(1 << 16, seed=21)
+        steps = (5, 1)
+    else:
+        m, p, v = _paired_grid()
+        kw.update(G=0.0, dt=1e-3)
+        steps = (2, 1)  # the second step of the first call halts
+    with _ctx(m, p, v, **kw) as c:
+        for k in steps:
+            c.step(k)
+            gp, _ = c.state()
+            box, skeys, orig, _ = c.debug_build()
+            rbox, keys, perm, _ = O.prepare(gp, m)
+            assert np.array_equal(box, rbox)
+            assert np.array_equal(skeys, keys[perm])
+            assert np.array_equal(keys[orig], skeys)
+        if case == "regrowth":
+            assert c.info().regrowths >= 1
+
+
 def _canon(level, bs, be, mask):
     return sorted(zip(level.tolist(), bs.tolist(), be.tolist(), mask.tolist()))
 
