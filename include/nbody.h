@@ -128,7 +128,8 @@ typedef struct nb_params {
     int32_t walk_bodies;  /* GROUP: bodies per traversal wavefront, 64 (one per
                              lane) or 32 (two lanes per body, each summing half
                              of the interaction list; smaller groups accept
-                             cells sooner) (0 = 64) */
+                             cells sooner) (0 = 32 below 524,288 bodies
+                             with the monopole far field, else 64) */
     int32_t list_flush;   /* GROUP: interaction-list entries held in LDS before
                              they are summed, 64 .. 192 (0 = 128) */
     int32_t quad_near;    /* LEARNED: an accepted cell whose centre of mass lies

@@ -255,7 +255,15 @@ namespace nb {
 
 // Group walk settings of a context (nb_params; 0 = the default).
 inline int walk_stack(const nb_ctx* c) { return c->p.walk_stack > 0 ? c->p.walk_stack : 2048; }
-inline int walk_bodies(const nb_ctx* c) { return c->p.walk_bodies > 0 ? c->p.walk_bodies : 64; }
+// Default bodies per walk wavefront: 32 below kWalk32Below bodies with the
+// monopole far field, where 64-body groups leave too few wavefronts per CU to
+// hide the walk's dependent node loads; 64 above, and for the learned far
+// field (whose subgroup bookkeeping is laid out per 64 bodies by default).
+constexpr uint64_t kWalk32Below = 1ull << 19;
+inline int walk_bodies(const nb_ctx* c) {
+    if (c->p.walk_bodies > 0) return c->p.walk_bodies;
+    return c->n < kWalk32Below && c->p.far_field != NB_FAR_LEARNED ? 32 : 64;
+}
 inline int list_flush(const nb_ctx* c) { return c->p.list_flush > 0 ? c->p.list_flush : 128; }
 // Squared near radius of the learned walk's per-body quadrupole terms, in
 // squared subgroup half-diagonals (< 0: every cell is near).

@@ -33,6 +33,11 @@ import oracle_lib as O
 from nbody2d import ics
 from nbody2d.abi import FAR_LEARNED, FAR_MONOPOLE, MLP_INPUTS, MLP_PARAMS, TRAV_GROUP
 
+# The learned far field walks 64-body groups by default; a small monopole
+# context would take 32, so the monopole contexts compared with learned ones
+# are given the learned contexts' groups.
+MONO_BODIES = 64
+
 pytestmark = pytest.mark.gpu
 
 EPS = 0.01
@@ -151,7 +156,7 @@ def test_learned_walk_matches_monopole_walk(theta):
     m, p, v = ics.exp_disk(16384, seed=6)
     kw = dict(theta=theta, softening=EPS, traversal=TRAV_GROUP, self_guard=int(theta > 0.71))
     w = (np.random.default_rng(5).normal(size=MLP_PARAMS) * 0.05).astype(np.float32)
-    with _ctx(m, p, v, far_field=FAR_MONOPOLE, **kw) as a, \
+    with _ctx(m, p, v, far_field=FAR_MONOPOLE, walk_bodies=MONO_BODIES, **kw) as a, \
             _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as b:
         b.mlp_set_weights(w)
         fa, ia, ta = a.forces()
@@ -181,7 +186,7 @@ def test_trained_network_lowers_force_error(theta):
     tg = ics.sample_targets(len(m), 2048, 1)
     ref = O.direct_sum(p, m, EPS * EPS, tg)
     kw = dict(theta=theta, softening=EPS, traversal=TRAV_GROUP, self_guard=int(theta > 0.71))
-    with _ctx(m, p, v, far_field=FAR_MONOPOLE, **kw) as a, \
+    with _ctx(m, p, v, far_field=FAR_MONOPOLE, walk_bodies=MONO_BODIES, **kw) as a, \
             _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as b, \
             _ctx(m, p, v, far_field=FAR_LEARNED, **kw) as z:
         b.mlp_set_weights(w)
@@ -257,7 +262,7 @@ def test_far_field_correction_matches_oracle(quad_near):
     w = (np.random.default_rng(3).normal(size=MLP_PARAMS) * 0.1).astype(np.float32)
     kw = dict(theta=0.8, softening=EPS, traversal=TRAV_GROUP, self_guard=1)
     with _ctx(m, p, v, far_field=FAR_LEARNED, quad_near=quad_near, **kw) as b, \
-            _ctx(m, p, v, far_field=FAR_MONOPOLE, **kw) as a:
+            _ctx(m, p, v, far_field=FAR_MONOPOLE, walk_bodies=MONO_BODIES, **kw) as a:
         b.mlp_set_weights(w)
         fb, _, _ = b.forces()
         quad_terms = int(b.info().quad_terms)
